@@ -1,0 +1,263 @@
+#!/usr/bin/env python3
+"""Benchmark: IISPH dam break, 2,097,152 FP32 fluid particles, one MI355X.
+
+Workload: a 128^3 fluid block [0,1]^3 at spacing s = 1/128 in the box
+[0,2]x[0,1.25]x[0,2] with 2 wall layers (~600k boundary samples) and a seeded
+sphere obstacle; cubic B-spline kernel with support 2s, rho_0 = 1000,
+nu = 0.001, relaxed Jacobi omega = 0.5, stop at mean compression error
+<= 0.1% or 100 iterations, CFL time step lambda = 0.4. One step is the
+reference's per-step loop (src/main.cu:60-78): CFL time step, grid build and
+reorder of every particle field (and the warm-start pressure), the IISPH
+step, bounds enforcement.
+
+Prints one JSON line with particle-steps/s, the roofline of the dominant
+kernel (algorithmic bytes per launch / its average HIP-event duration, timed
+live on the launch stream over the timed steps) and the CPU baseline (the
+oracle's FP32 OpenMP port of the same step, on the same state, on this host).
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "particle-steps/sec + % HBM roofline (IISPH 2M); serving p50/p99 vs plain"
+
+
+def algorithmic_bytes(name: str, n: int, nb: int, ncells: int, nfields: int) -> float | None:
+    """Compulsory DRAM bytes of one launch: every array it reads once plus
+    every array it writes once (neighbour re-reads count zero)."""
+    table = {
+        # x,y,z m + boundary x,y,z,m -> rho, grad-sum(3), boundary density
+        "k_density": 16 * n + 16 * nb + 20 * n,
+        # x v m rho -> a
+        "k_nonpressure": 32 * n + 12 * n,
+        # v a -> v
+        "k_integrate_v": 24 * n + 12 * n,
+        # x v a -> x v
+        "k_integrate": 36 * n + 24 * n,
+        # x v m rho grad-sum p -> a_ii s_i p
+        "k_iisph_aii_s": 48 * n + 12 * n,
+        # x p m grad-sum -> a
+        "k_iisph_pacc": 32 * n + 12 * n,
+        # x a m a_ii s_i p grad-sum -> p
+        "k_iisph_jacobi": 52 * n + 4 * n,
+        # x -> keys slots (+ counts)
+        "k_count": 12 * n + 8 * n,
+        "k_scan_reduce": 4 * ncells,
+        "k_scan_down": 12 * ncells,
+        "k_scatter": 12 * n + 4 * n,
+        "k_stabilize": 16 * n + 4 * n,
+        "k_gather": 4 * n + 8 * nfields * n,
+        "k_bounds": 24 * n + 24 * n,
+    }
+    return float(table[name]) if name in table else None
+
+
+def step_bytes(n: int, nb: int, ncells: int, iters: float) -> float:
+    """Compulsory bytes of one IISPH step under the reference's phase
+    structure: 372N + 48N_b + 28C + l (72N + 32N_b)."""
+    return 372.0 * n + 48.0 * nb + 28.0 * ncells + iters * (72.0 * n + 32.0 * nb)
+
+
+def parse_profile(text: str) -> dict:
+    out = {}
+    for line in text.strip().splitlines():
+        name, cnt, ms = line.split()
+        out[name] = (int(cnt), float(ms))
+    return out
+
+
+def cpu_baseline(sc, host_state, p, dt_steps, args, log) -> dict | None:
+    """The oracle's FP32 port of the same step on this host's cores, from the
+    GPU's state after the timed steps (bounded sample of `cpu_steps` steps)."""
+    from oracle import binding as orc
+    from reyn_amd import capi
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    orc.set_threads(threads)
+    h = sc.spacing
+    r = 2.0 * h
+    fields = {k: host_state[k].astype(np.float32) for k in ("xx", "xy", "xz", "vx", "vy", "vz", "m")}
+    pr = p.astype(np.float32)
+    xb = sc.xb_sorted
+    mb = sc.mb
+    bgeom = orc.geometry(sc.bdy_min, sc.bdy_max, r, 0.5)
+    bgrid = orc.grid_build(bgeom, xb[:, 0], xb[:, 1], xb[:, 2]).resorted()
+    bdy = orc.Bdy(bgrid, xb[:, 0], xb[:, 1], xb[:, 2], mb, dtype=np.float32)
+    geom = orc.geometry(sc.bound_min, sc.bound_max, r, 0.5)
+    prm = orc.sph(capi.KERNEL_B3, r, h, 0.001, sc.rho_0)
+    ip = orc.OrcIisph(0.5, 0.001, 3, 100, 1, 1)
+    t0 = time.perf_counter()
+    iters = []
+    for _ in range(args.cpu_steps):
+        dt, _ = orc.cfl_time_step(0.4, h, np.stack([fields["vx"], fields["vy"], fields["vz"]], 1))
+        g = orc.grid_build(geom, fields["xx"], fields["xy"], fields["xz"])
+        for k in fields:
+            fields[k] = orc.gather(g.sorted, fields[k])
+        pr = orc.gather(g.sorted, pr)
+        st = orc.State({"x": fields["xx"], "y": fields["xy"], "z": fields["xz"], "vx": fields["vx"],
+                        "vy": fields["vy"], "vz": fields["vz"], "m": fields["m"]}, dtype=np.float32)
+        res = orc.iisph_step(g.resorted(), st, bdy, prm, ip, pr, dt)
+        pr = res["p"]
+        iters.append(res["iterations"])
+        x, v = orc.hard_enforce_bounds(sc.bound_min, sc.bound_max, np.stack([st.x, st.y, st.z], 1),
+                                       np.stack([st.vx, st.vy, st.vz], 1))
+        for a, k in enumerate(("xx", "xy", "xz")):
+            fields[k] = np.ascontiguousarray(x[:, a])
+        for a, k in enumerate(("vx", "vy", "vz")):
+            fields[k] = np.ascontiguousarray(v[:, a])
+        fields["m"] = st.m
+    wall = time.perf_counter() - t0
+    value = sc.n * args.cpu_steps / wall
+    log(f"cpu baseline: {args.cpu_steps} step(s) in {wall:.2f} s on {threads} threads, iterations {iters}")
+    return {"value": value, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{args.cpu_steps} full IISPH step(s) of the same {sc.n}-particle scene from the GPU's state "
+                      f"after the timed steps (Jacobi iterations {iters}), oracle FP32 OpenMP port, "
+                      f"{wall:.2f} s wall"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--side", type=int, default=128, help="fluid lattice side (128 -> 2,097,152 particles)")
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-profile", action="store_true", help="time without per-launch events")
+    args = ap.parse_args()
+
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
+
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); this build's path is "
+              "single-GPU, see the sharding plan in DESIGN.md", file=sys.stderr)
+        return 2
+
+    from reyn_amd import capi, scenes
+    from reyn_amd.device import Context
+    from reyn_amd.sim import IISPH, BoundarySamples, Particles, Simulation, reduce
+
+    t_setup = time.perf_counter()
+    sc = scenes.config3_iisph(side=args.side)
+    log(f"scene: {sc.n} fluid, {sc.n_bdy} boundary, spacing {sc.spacing:.6f}, sphere {sc.meta.get('sphere')}")
+    ctx = Context(0)
+    state = Particles(ctx, sc.x, None, sc.mass, sc.rho_0)
+    bdy = BoundarySamples(ctx, sc.xb, sc.spacing, sc.rho_0)
+    solver = IISPH(ctx, "B3", sc.n, 0.001, sc.spacing, sc.rho_0, omega=0.5, eta_rho_max=0.001, min_iter=3,
+                   max_iter=100, stop_rule=capi.IISPH_STOP_MEAN, warm_start=True, check_every=4)
+    sim = Simulation(ctx, solver, state, bdy, sc.spacing, sc.bound_min, sc.bound_max, cfl_lambda=0.4,
+                     cell_ratio=0.5, extra_reorder=("p",))
+    ncells = sim.builder.geometry().ncells
+    log(f"setup {time.perf_counter() - t_setup:.1f} s; fluid grid {ncells} cells")
+
+    for k in range(args.warmup):
+        s = sim.step()
+        log(f"warmup {k}: dt {s.dt:.3e} iters {s.iterations} mean err {s.mean_rel_error:.2e}")
+
+    ctx.synchronize()
+    ctx._lib.reyn_profile_enable(ctx.h, 0 if args.no_profile else 1)
+    iters = []
+    errs = []
+    t0 = time.perf_counter()
+    ctx.timer_start()
+    for k in range(args.steps):
+        s = sim.step()
+        iters.append(s.iterations)
+        errs.append(s.mean_rel_error)
+    gpu_ms = ctx.timer_stop()
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    log(f"timed {args.steps} steps: {gpu_ms:.1f} ms (events), {wall * 1e3:.1f} ms (wall); iterations {iters}")
+
+    prof = {}
+    if not args.no_profile:
+        import ctypes as C
+        need = C.c_size_t()
+        ctx._lib.reyn_profile_summary(ctx.h, None, 0, C.byref(need))
+        buf = C.create_string_buffer(need.value + 16)
+        ctx._lib.reyn_profile_summary(ctx.h, buf, len(buf), None)
+        prof = parse_profile(buf.value.decode())
+        ctx._lib.reyn_profile_enable(ctx.h, 0)
+
+    ms_per_step = wall * 1e3 / args.steps
+    value = sc.n * args.steps / wall
+    nfields = 8
+    roofline = None
+    if prof:
+        log(f"{'kernel':<18}{'launches':>9}{'total ms':>11}{'avg us':>10}{'GB/s':>9}{'HBM %':>7}")
+        best = None
+        for name, (cnt, tot) in sorted(prof.items(), key=lambda kv: -kv[1][1]):
+            by = algorithmic_bytes(name, sc.n, sc.n_bdy, ncells, nfields)
+            avg_us = tot * 1e3 / cnt
+            gbs = by / (avg_us * 1e-6) / 1e9 if by else float("nan")
+            log(f"{name:<18}{cnt:>9}{tot:>11.3f}{avg_us:>10.1f}{gbs:>9.0f}{100 * gbs / HBM_PEAK_GBS:>7.1f}")
+            if best is None and by:
+                best = (name, cnt, tot, by, avg_us, gbs)
+        name, cnt, tot, by, avg_us, gbs = best
+        roofline = {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(gbs / HBM_PEAK_GBS, 5), "traffic": None, "kernel": name,
+                    "bytes_per_launch": by, "avg_launch_us": round(avg_us, 2), "launches": cnt,
+                    "share_of_step_time": round(tot / gpu_ms, 4)}
+    mean_iters = float(np.mean(iters))
+    sbytes = step_bytes(sc.n, sc.n_bdy, ncells, mean_iters)
+    step_frac = sbytes / (ms_per_step * 1e-3) / (HBM_PEAK_GBS * 1e9)
+
+    rho_err = abs(reduce(ctx, solver.rho, "avg") / sc.rho_0 - 1.0)
+    cpu = None
+    if not args.no_cpu_baseline:
+        host = state.host()
+        # boundary samples in their stored (cell) order, with the GPU's masses
+        sc.xb_sorted = bdy.positions()
+        sc.mb = bdy.m.numpy()
+        sc.bdy_min, sc.bdy_max = bdy.bound_min, bdy.bound_max
+        cpu = cpu_baseline(sc, host, solver.p.numpy(), None, args, log)
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"IISPH dam break, {sc.n} FP32 fluid particles (128^3 block at s=1/128) + {sc.n_bdy} "
+                        "boundary samples (2 wall layers of box [0,2]x[0,1.25]x[0,2] + sphere obstacle), B3 "
+                        "kernel, omega 0.5, stop at mean error <= 0.1% or 100 iterations, CFL lambda 0.4, "
+                        "grid rebuild + reorder every step",
+            "fluid_particles": sc.n,
+            "boundary_particles": sc.n_bdy,
+            "grid_cells": ncells,
+            "cell_ratio": 0.5,
+            "jacobi_iterations_mean": round(mean_iters, 2),
+            "jacobi_iterations": iters,
+            "mean_density_error_last": errs[-1] if errs else None,
+            "density_mean_rel_dev": rho_err,
+            "step_hbm_frac": round(step_frac, 5),
+            "step_bytes": sbytes,
+            "event_ms_timed": round(gpu_ms, 3),
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

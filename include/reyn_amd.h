@@ -1,0 +1,379 @@
+/*
+ * reyn_amd.h — C-ABI of the MI355X (gfx950) SPH per-timestep hot path.
+ *
+ * Every entry point takes plain device pointers and sizes, enqueues its work
+ * on the stream owned by a `reyn_ctx`, returns an `int` status (0 = ok,
+ * negative = error) and never throws or exits. `reyn_last_error(ctx)` holds a
+ * message for the last failing call on that context. Caller memory is never
+ * freed by the library; buffers the library allocates itself (grid-builder
+ * tables, solver scratch) belong to the handle that allocated them.
+ *
+ * The reference (JulianKarrer/reyn) exposes this path only as C++ templates
+ * with device lambdas; each declaration below cites the reference interface
+ * it replaces as `path:line` in that repository. The C++ host API in
+ * include/reyn/ restores the reference's class shapes on top of these calls.
+ */
+#ifndef REYN_AMD_H_
+#define REYN_AMD_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* the library is built with hidden visibility; everything declared here is
+ * exported */
+#if defined(__GNUC__)
+#pragma GCC visibility push(default)
+#endif
+
+#define REYN_ABI_VERSION 1
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+    REYN_OK = 0,
+    REYN_ERR_INVALID_ARGUMENT = -1,
+    REYN_ERR_HIP = -2,
+    REYN_ERR_OUT_OF_MEMORY = -3,
+    /* a particle lay outside the cells a query may touch; it was clamped to
+     * the grid so no memory was accessed out of bounds */
+    REYN_ERR_OUT_OF_BOUNDS = -4,
+    REYN_ERR_NOT_FINITE = -5,
+    REYN_ERR_NOT_CONVERGED = -6,
+    REYN_ERR_CAPACITY = -7
+};
+
+/* ---- plain value types ------------------------------------------------- */
+
+/* Smoothing kernel families; replaces the CRTP `Kernel<D>` classes and the
+ * `FOREACH_KERNEL` list (src/kernels.cuh:19-174). */
+typedef enum reyn_kernel {
+    REYN_KERNEL_C2 = 0, /* Wendland C2, src/kernels.cuh:78-95 */
+    REYN_KERNEL_B3 = 1, /* cubic B-spline, src/kernels.cuh:101-118 */
+    REYN_KERNEL_W6 = 2, /* Wendland C6, src/kernels.cuh:121-145 */
+    REYN_KERNEL_COS = 3 /* double cosine, src/kernels.cuh:148-165 */
+} reyn_kernel;
+
+typedef struct reyn_float3 {
+    float x, y, z;
+} reyn_float3;
+
+/* Query-side view of a uniform grid; replaces the POD `UniformGrid<R>` and
+ * `MaybeSorted<R>` (src/datastructure/uniformgrid.cuh:70-106).
+ * `sorted == NULL` is `Resort::yes` (the indexed arrays are stored in cell
+ * order); otherwise it is `Resort::no` and `sorted[k]` is the index of the
+ * k-th point in cell order. `prefix` has ncells + 1 entries; the last one is
+ * the number of points. */
+typedef struct reyn_grid {
+    reyn_float3 bound_min; /* padded lower corner of cell (0,0,0) */
+    float cell_size;
+    float r_c_2; /* squared search radius, formed in float */
+    int32_t nx, ny, nz;
+    int32_t range; /* cells searched on each side of a query's cell */
+    const uint32_t* prefix;
+    const uint32_t* sorted;
+    uint32_t n; /* number of indexed points */
+} reyn_grid;
+
+/* Fluid particle state in SoA layout; replaces `Particles`
+ * (src/particles.cuh:25-75). All pointers are device pointers of n floats. */
+typedef struct reyn_particles {
+    float *xx, *xy, *xz;
+    float *vx, *vy, *vz;
+    float* m;
+    uint32_t n;
+} reyn_particles;
+
+/* Static boundary samples with their own cell-ordered grid; replaces the POD
+ * `Boundary` (src/scene/scene.cuh:14-20). `grid.sorted` must be NULL. */
+typedef struct reyn_boundary {
+    const float *xx, *xy, *xz, *m;
+    uint32_t n;
+    reyn_grid grid;
+} reyn_boundary;
+
+/* Constants shared by every solver (the constructor arguments of SESPH,
+ * PCISPH and IISPH: src/solvers/SESPH.cuh:37-40, PCISPH.cuh:80-82,
+ * IISPH.cuh:51-55). `h` is the particle spacing, `h_bar` the kernel support. */
+typedef struct reyn_sph_params {
+    int32_t kernel; /* reyn_kernel */
+    float h_bar;
+    float h;
+    float nu;
+    float rho_0;
+    reyn_float3 g;
+} reyn_sph_params;
+
+/* ---- context, memory and timing ---------------------------------------- */
+
+typedef struct reyn_ctx reyn_ctx;
+
+/* Creates a context bound to HIP device `device` with its own stream of
+ * default priority. No reference counterpart (the reference uses the default
+ * stream and global state, src/common.h:11-36). */
+int reyn_ctx_create(int device, reyn_ctx** out);
+int reyn_ctx_destroy(reyn_ctx* ctx);
+const char* reyn_last_error(const reyn_ctx* ctx);
+int reyn_ctx_synchronize(reyn_ctx* ctx);
+/* The context's hipStream_t, as an opaque pointer. */
+void* reyn_ctx_stream(reyn_ctx* ctx);
+int reyn_device_count(int* count);
+int reyn_abi_version(void);
+
+/* Device memory helpers (replace `DeviceBuffer<T>` storage,
+ * src/buffer.cuh:15-269, for hosts that have no HIP binding of their own). */
+int reyn_malloc(reyn_ctx* ctx, void** ptr, size_t bytes);
+int reyn_free(reyn_ctx* ctx, void* ptr);
+/* kind: 0 host->device, 1 device->host, 2 device->device; synchronous with
+ * respect to the context's stream. */
+int reyn_memcpy(reyn_ctx* ctx, void* dst, const void* src, size_t bytes, int kind);
+int reyn_fill_u32(reyn_ctx* ctx, uint32_t* ptr, uint32_t value, size_t count);
+int reyn_fill_f32(reyn_ctx* ctx, float* ptr, float value, size_t count);
+
+/* Event timing on the context's stream: `reyn_timer_start` records a start
+ * event, `reyn_timer_stop` records a stop event, waits for it and returns the
+ * elapsed milliseconds. */
+int reyn_timer_start(reyn_ctx* ctx);
+int reyn_timer_stop(reyn_ctx* ctx, float* ms);
+
+/* Per-launch profiling with HIP events on the context's stream (no
+ * reference counterpart; the reference has no profiler hooks, see its
+ * nanobench micro-benchmarks, src/datastructure/uniformgrid.cu:385-423).
+ * `reyn_profile_enable` clears the records and switches recording on or off;
+ * `reyn_profile_summary` waits for the stream and writes one line per kernel,
+ * "name launches total_ms", into buf (at most len bytes, NUL-terminated);
+ * *needed receives the full size. */
+int reyn_profile_enable(reyn_ctx* ctx, int on);
+int reyn_profile_summary(reyn_ctx* ctx, char* buf, size_t len, size_t* needed);
+
+/* ---- uniform grid (src/datastructure/uniformgrid.cu:65-181) ------------- */
+
+typedef struct reyn_grid_builder reyn_grid_builder;
+
+/* Replaces `UniformGridBuilder::UniformGridBuilder(bound_min, bound_max,
+ * search_radius)` (src/datastructure/uniformgrid.cu:65-96). The reference
+ * fixes cell_ratio = 0.5 (cell = search_radius / 2, :69); this entry point
+ * takes the ratio of cell size to search radius (0.5, 1.0 and 2.0 are the
+ * supported sweep values; any ratio in [0.25, 4] works). The lower bound is
+ * padded by 2 * search_radius and the cell counts are ceil-divided as in
+ * :73-80. Fails with REYN_ERR_INVALID_ARGUMENT where the reference throws
+ * (:93-95). */
+int reyn_grid_builder_create(reyn_ctx* ctx, reyn_float3 bound_min,
+    reyn_float3 bound_max, float search_radius, float cell_ratio,
+    reyn_grid_builder** out);
+int reyn_grid_builder_destroy(reyn_grid_builder* b);
+/* Geometry of the builder (prefix/sorted are NULL until a construct call). */
+int reyn_grid_builder_geometry(const reyn_grid_builder* b, reyn_grid* out);
+
+/* Replaces `UniformGridBuilder::construct` (uniformgrid.cu:145-156) and its
+ * `_construct` (uniformgrid.cu:98-143): counts points per cell, scans, and
+ * writes `sorted` — a stable counting sort (ascending point index inside a
+ * cell; the reference's in-cell order is racy, uniformgrid.cu:59). The
+ * returned grid is `Resort::no` and points into builder-owned tables that
+ * stay valid until the next construct call on the same builder. Returns
+ * REYN_ERR_OUT_OF_BOUNDS (after finishing the build with clamped cells) if a
+ * point lies outside the cells that queries may touch. */
+int reyn_grid_construct(reyn_ctx* ctx, reyn_grid_builder* b,
+    float search_radius, const float* xx, const float* xy, const float* xz,
+    uint32_t n, reyn_grid* out);
+
+/* Replaces `construct_and_reorder(search_radius, tmp, xx, xy, xz, bufs...)`
+ * and the `Particles&` overloads (uniformgrid.cuh:243-483,
+ * uniformgrid.cu:158-181; `DeviceBuffer::reorder`, buffer.cuh:184-209).
+ * fields[0..2] must be x, y, z; fields[3..nfields) are further per-point
+ * float arrays. Every field is gathered into tmp[k] in cell order and then
+ * the caller's pointers fields[k] and tmp[k] are swapped, as the reference
+ * swaps storage with its tmp buffer. The returned grid is `Resort::yes`. */
+int reyn_grid_construct_and_reorder(reyn_ctx* ctx, reyn_grid_builder* b,
+    float search_radius, float** fields, float** tmp, uint32_t nfields,
+    uint32_t n, reyn_grid* out);
+
+/* Device copies of the builder's last per-cell counts (ncells) and per-point
+ * cell keys (n), for tests. */
+int reyn_grid_debug_tables(reyn_grid_builder* b, const uint32_t** counts,
+    const uint32_t** keys);
+
+/* Gather: dst[k][i] = src[k][sorted[i]] for every field k
+ * (buffer.cuh:184-209, thrust::gather). */
+int reyn_gather_f32(reyn_ctx* ctx, const uint32_t* sorted, uint32_t n,
+    const float* const* src, float* const* dst, uint32_t nfields);
+
+/* The reference's grid test and micro-benchmark kernels
+ * (uniformgrid.cu:185-245): for every point i of (xx,xy,xz) the number of
+ * neighbours j != i with |x_ij|^2 <= r_c_2, the sum of |x_ij|^2 and of x_ij,
+ * found through `grid`. Any output may be NULL. */
+int reyn_grid_query_stats(reyn_ctx* ctx, const reyn_grid* grid,
+    const float* xx, const float* xy, const float* xz, uint32_t n,
+    uint32_t* count, float* len2, float* vx, float* vy, float* vz);
+/* res[i] = sum_j x_i.x * x_j.x * W(x_ij) with the given kernel
+ * (`_benchmark_kernel`, uniformgrid.cu:231-245). */
+int reyn_grid_query_bench(reyn_ctx* ctx, const reyn_grid* grid,
+    int32_t kernel, float h_bar, const float* xx, const float* xy,
+    const float* xz, uint32_t n, float* res);
+
+/* ---- smoothing kernels (src/kernels.cuh:19-174) ------------------------- */
+
+/* Evaluates W and grad W at n offsets (the reference's `_test_kernels`,
+ * src/kernels.cu:6-15). */
+int reyn_kernel_eval(reyn_ctx* ctx, int32_t kernel, float h_bar,
+    const float* dx, const float* dy, const float* dz, uint32_t n, float* w,
+    float* gx, float* gy, float* gz);
+
+/* ---- shared solver phases (src/solvers/solver.cuh, solver.cu) ---------- */
+
+/* rho_i = sum_j m_j W_ij + sum_k m_k W_ik, compensated sum
+ * (`compute_densities`, src/solvers/solver.cuh:18-38). bdy may be NULL. */
+int reyn_density(reyn_ctx* ctx, const reyn_sph_params* prm,
+    const reyn_particles* fluid, const reyn_grid* grid,
+    const reyn_boundary* bdy, float* rho);
+
+/* Gravity plus fluid-fluid viscosity (`write_non_prs_acc_ff`,
+ * src/solvers/solver.cuh:40-69). */
+int reyn_nonpressure_acc(reyn_ctx* ctx, const reyn_sph_params* prm,
+    const reyn_particles* fluid, const reyn_grid* grid, const float* rho,
+    float* ax, float* ay, float* az);
+
+/* PCISPH predicted density (`predict_rho`, src/solvers/solver.cuh:71-105). */
+int reyn_predict_rho(reyn_ctx* ctx, const reyn_sph_params* prm,
+    const reyn_particles* fluid, const reyn_grid* grid,
+    const reyn_boundary* bdy, float dt, float* rho);
+
+/* Semi-implicit Euler (`integrate`, `integrate_a_v_only`,
+ * src/solvers/solver.cu:3-32). */
+int reyn_integrate(reyn_ctx* ctx, const reyn_particles* fluid,
+    const float* ax, const float* ay, const float* az, float dt);
+int reyn_integrate_a_v_only(reyn_ctx* ctx, const reyn_particles* fluid,
+    const float* ax, const float* ay, const float* az, float dt);
+
+/* ---- SESPH (src/solvers/SESPH.cuh:11-70, SESPH.cu:6-104) ---------------- */
+
+typedef struct reyn_sesph_params {
+    reyn_sph_params sph;
+    float k; /* stiffness; the reference hard-codes 500 (SESPH.cuh:50) */
+} reyn_sesph_params;
+
+typedef struct reyn_sesph_work {
+    float* rho;
+    float *ax, *ay, *az;
+} reyn_sesph_work;
+
+/* One `SESPH<K,R>::step` (SESPH.cu:138-166): densities, accelerations with
+ * the normalised EOS max(0, k(rho/rho_0 - 1)), integration. */
+int reyn_sesph_step(reyn_ctx* ctx, const reyn_sesph_params* prm,
+    const reyn_particles* fluid, const reyn_grid* grid,
+    const reyn_boundary* bdy, const reyn_sesph_work* work, float dt);
+
+/* ---- PCISPH (src/solvers/PCISPH.cuh:8-111, PCISPH.cu:15-118) ------------ */
+
+/* `k_c_pcisph` (PCISPH.cuh:49-77): delta = 1 / (2 V0^2 (|sum grad W|^2 +
+ * sum |grad W|^2)) over a 5^3 template lattice of spacing h, rounded to
+ * float. Host-only, no context needed. */
+int reyn_pcisph_delta(int32_t kernel, float h_bar, float h, double* delta);
+
+typedef struct reyn_pcisph_params {
+    reyn_sph_params sph;
+    double delta;      /* from reyn_pcisph_delta */
+    float eta_rho_max; /* 0.001 (PCISPH.cuh:36) */
+    uint32_t min_iter; /* 3 (PCISPH.cuh:38) */
+    uint32_t max_iter; /* 0 = no cap, as in the reference */
+} reyn_pcisph_params;
+
+typedef struct reyn_pcisph_work {
+    float* rho;
+    float *ax, *ay, *az;
+} reyn_pcisph_work;
+
+typedef struct reyn_solve_info {
+    uint32_t iterations;
+    uint32_t n_violating;   /* particles above the threshold, last iterate */
+    double mean_rel_error;  /* FP64 mean error / rho_0, last iterate */
+    double avg_density;     /* PCISPH: FP64 mean predicted density */
+} reyn_solve_info;
+
+/* One `PCISPH<K,R>::step` (PCISPH.cu:171-224). The loop
+ * `while (l < min_iter || avg(rho) > (1 + eta) rho_0)` runs on the device,
+ * with the average formed in FP64. */
+int reyn_pcisph_step(reyn_ctx* ctx, const reyn_pcisph_params* prm,
+    const reyn_particles* fluid, const reyn_grid* grid,
+    const reyn_boundary* bdy, const reyn_pcisph_work* work, float dt,
+    reyn_solve_info* info);
+
+/* ---- IISPH (src/solvers/IISPH.cuh:8-93, IISPH.cu:10-269) ---------------- */
+
+enum {
+    /* the reference rule: stop once no particle has a predicted density error
+     * above eta * rho_0 (IISPH.cu:249-255, :340) */
+    REYN_IISPH_STOP_ANY = 0,
+    /* stop once mean_i max(0, err_i) / rho_0 <= eta, with the mean formed in
+     * FP64 on the device */
+    REYN_IISPH_STOP_MEAN = 1
+};
+
+typedef struct reyn_iisph_params {
+    reyn_sph_params sph;
+    float omega;       /* 0.5 (IISPH.cuh:45) */
+    float eta_rho_max; /* 0.005 (IISPH.cuh:41) */
+    uint32_t min_iter; /* 3 (IISPH.cuh:43) */
+    uint32_t max_iter; /* 0 = no cap, as in the reference */
+    int32_t stop_rule; /* REYN_IISPH_STOP_* */
+    int32_t warm_start; /* 1: p *= 0.5 (IISPH.cuh:48, IISPH.cu:293-309) */
+    uint32_t check_every; /* iterations launched between host checks; 0 = 4 */
+} reyn_iisph_params;
+
+typedef struct reyn_iisph_work {
+    float* rho;
+    float* p; /* persists across steps (warm start) */
+    float* a_ii;
+    float* s_i;
+    float *ax, *ay, *az;
+} reyn_iisph_work;
+
+/* One `IISPH<K,R>::step` (IISPH.cu:258-356): densities, non-pressure
+ * accelerations, a_ii and s_i, the relaxed-Jacobi pressure loop with its
+ * convergence test on the device, final pressure accelerations and
+ * integration. */
+int reyn_iisph_step(reyn_ctx* ctx, const reyn_iisph_params* prm,
+    const reyn_particles* fluid, const reyn_grid* grid,
+    const reyn_boundary* bdy, const reyn_iisph_work* work, float dt,
+    reyn_solve_info* info);
+
+/* ---- boundary, time step, bounds, reductions --------------------------- */
+
+/* Akinci boundary masses (`calculate_boundary_masses`,
+ * src/scene/sample_boundary.cu:251-297): m starts at 1 and is refined
+ * `iterations` times with m_b <- rho_0 m_b / sum_j m_j W_bj using B3 with
+ * support 2h. Each refinement reads the previous masses only (Jacobi), so
+ * the result is deterministic. */
+int reyn_boundary_masses(reyn_ctx* ctx, const reyn_boundary* bdy, float* m,
+    float h, float rho_0, uint32_t iterations);
+
+/* `cfl_time_step` (src/timestep/cfl.cu:5-37): dt = min(lambda h / max|v|,
+ * sqrt(2 lambda h / |g|)); max|v| is reduced in FP64 on the device. */
+int reyn_cfl_time_step(reyn_ctx* ctx, float lambda, float h,
+    const reyn_particles* fluid, reyn_float3 g, float* dt, double* v_max);
+
+/* `simple_dt_controller` (src/timestep/cfl.cu:39-52). */
+int reyn_simple_dt_controller(reyn_ctx* ctx, uint32_t iter_last,
+    float dt_last, float lambda, float h, const reyn_particles* fluid,
+    reyn_float3 g, uint32_t iter_target, float change_factor, float* dt);
+
+/* `Scene::hard_enforce_bounds` (src/scene/scene.cu:466-491). */
+int reyn_hard_enforce_bounds(reyn_ctx* ctx, reyn_float3 bound_min,
+    reyn_float3 bound_max, const reyn_particles* fluid);
+
+/* FP64 reductions of a float array (`DeviceBuffer::min/max/sum/avg`,
+ * src/buffer.cuh:213-258). op: 0 sum, 1 min, 2 max, 3 mean. */
+int reyn_reduce_f32(reyn_ctx* ctx, const float* x, uint32_t n, int32_t op,
+    double* out);
+
+#if defined(__GNUC__)
+#pragma GCC visibility pop
+#endif
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* REYN_AMD_H_ */

@@ -1,0 +1,856 @@
+// SPH phases and the three pressure solvers.
+//
+// Every neighbour phase is one launch with one thread per fluid particle over
+// the cell-ordered arrays (`walk` in sph.hpp). Boundary contributions that
+// depend on positions only — sum_k m_k W_ik and sum_k m_k grad W_ik — are
+// formed once per step in the density launch and reused by every later phase
+// and every solver iteration, so the iteration loops walk the fluid grid only.
+//
+// Solver loops keep their convergence test on the device: each iteration ends
+// with a one-workgroup control launch that reduces per-block partials in a
+// fixed order (FP64), updates the iteration count and sets `done`. Every loop
+// launch returns at once when `done` is set, so the host enqueues iterations
+// in chunks and synchronises once per chunk instead of once per iteration.
+#include <algorithm>
+#include <cmath>
+
+#include "sph.hpp"
+
+namespace reyn {
+namespace {
+
+struct SphK {
+    KParams kp;
+    float h, nu, rho0;
+    f3 g;
+    float visc_c; // 10 nu
+    float visc_eps; // 0.01 h^2
+};
+
+SphK make_sphk(const reyn_sph_params& p)
+{
+    SphK c;
+    c.kp = make_kparams(p.kernel, p.h_bar);
+    c.h = p.h;
+    c.nu = p.nu;
+    c.rho0 = p.rho_0;
+    c.g = { p.g.x, p.g.y, p.g.z };
+    c.visc_c = 10.f * p.nu;
+    c.visc_eps = static_cast<float>(0.01 * double(p.h) * double(p.h));
+    return c;
+}
+
+struct SolveCtl {
+    uint32_t iter;
+    uint32_t done;
+    uint32_t n_violating;
+    uint32_t pad;
+    double mean_rel_error;
+    double avg_density;
+};
+
+// Per-step scratch carved out of the context scratch.
+struct Scratch {
+    float *gbx, *gby, *gbz; // sum_k m_k grad W_ik
+    float* rb; // sum_k m_k W_ik
+    double* part_d; // per-block FP64 partials
+    uint32_t* part_u; // per-block counts
+    SolveCtl* ctl;
+};
+
+int get_scratch(reyn_ctx* ctx, uint32_t n, Scratch* s)
+{
+    const size_t nb = blocks_for(n);
+    auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+    const size_t f = al(size_t(n) * 4);
+    const size_t bytes = 4 * f + al(nb * 8) + al(nb * 4) + al(sizeof(SolveCtl));
+    char* p = nullptr;
+    REYN_TRY(ctx_scratch(ctx, bytes, reinterpret_cast<void**>(&p)));
+    s->gbx = reinterpret_cast<float*>(p);
+    s->gby = reinterpret_cast<float*>(p + f);
+    s->gbz = reinterpret_cast<float*>(p + 2 * f);
+    s->rb = reinterpret_cast<float*>(p + 3 * f);
+    s->part_d = reinterpret_cast<double*>(p + 4 * f);
+    s->part_u = reinterpret_cast<uint32_t*>(p + 4 * f + al(nb * 8));
+    s->ctl = reinterpret_cast<SolveCtl*>(p + 4 * f + al(nb * 8) + al(nb * 4));
+    return REYN_OK;
+}
+
+__device__ __forceinline__ bool loop_done(const SolveCtl* ctl) { return ctl && ctl->done; }
+
+// ---- density ------------------------------------------------------------------
+
+template <int K, bool IND, bool GB>
+__global__ __launch_bounds__(kBlock) void k_density(FluidDev f, GridDev g, BdyDev b, KParams kp,
+    float* __restrict__ rho, float* __restrict__ gbx, float* __restrict__ gby,
+    float* __restrict__ gbz, float* __restrict__ rb)
+{
+    const uint32_t t = global_index();
+    if (t >= f.n)
+        return;
+    const uint32_t i = particle_of<IND>(g, t);
+    const f3 xi = ld3(f.x, f.y, f.z, i);
+    ksum acc;
+    walk<IND>(g, xi, f.x, f.y, f.z, [&](uint32_t j, f3, float d2) { acc.add(f.m[j] * kW<K>(kp, d2)); });
+    ksum accb;
+    ksum3 grad;
+    if (b.present) {
+        walk<false>(b.grid, xi, b.x, b.y, b.z, [&](uint32_t k, f3 xik, float d2) {
+            const float mk = b.m[k];
+            accb.add(mk * kW<K>(kp, d2));
+            if (GB)
+                grad.add(mk * kdW<K>(kp, xik, d2));
+        });
+    }
+    const float rbi = accb.value();
+    rho[i] = acc.value() + rbi;
+    if (GB) {
+        const f3 gb = grad.value();
+        gbx[i] = gb.x;
+        gby[i] = gb.y;
+        gbz[i] = gb.z;
+        rb[i] = rbi;
+    }
+}
+
+// ---- non-pressure accelerations (gravity + fluid-fluid viscosity) -----------
+
+template <int K, bool IND>
+__global__ __launch_bounds__(kBlock) void k_nonpressure(FluidDev f, GridDev g, SphK c,
+    const float* __restrict__ rho, float* __restrict__ ax, float* __restrict__ ay,
+    float* __restrict__ az)
+{
+    const uint32_t t = global_index();
+    if (t >= f.n)
+        return;
+    const uint32_t i = particle_of<IND>(g, t);
+    const f3 xi = ld3(f.x, f.y, f.z, i);
+    const f3 vi = ld3(f.vx, f.vy, f.vz, i);
+    f3 acc = c.g;
+    walk<IND>(g, xi, f.x, f.y, f.z, [&](uint32_t j, f3 xij, float d2) {
+        const f3 vij = vi - ld3(f.vx, f.vy, f.vz, j);
+        const float s = c.visc_c * f.m[j] / rho[j] * dot(vij, xij) / (d2 + c.visc_eps);
+        acc += s * kdW<K>(c.kp, xij, d2);
+    });
+    st3(ax, ay, az, i, acc);
+}
+
+// ---- integration --------------------------------------------------------------
+
+__global__ __launch_bounds__(kBlock) void k_integrate(FluidDev f, const float* __restrict__ ax,
+    const float* __restrict__ ay, const float* __restrict__ az, float dt, const SolveCtl* ctl,
+    int positions)
+{
+    if (loop_done(ctl))
+        return;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= f.n)
+        return;
+    const f3 v = ld3(f.vx, f.vy, f.vz, i) + dt * ld3(ax, ay, az, i);
+    st3(f.vx, f.vy, f.vz, i, v);
+    if (positions)
+        st3(f.x, f.y, f.z, i, ld3(f.x, f.y, f.z, i) + dt * v);
+}
+
+// ---- SESPH ---------------------------------------------------------------------
+
+__device__ __forceinline__ float eos_sesph(float rho, float k, float rho0)
+{
+    return fmaxf(0.f, k * (rho / rho0 - 1.f));
+}
+
+template <int K, bool IND>
+__global__ __launch_bounds__(kBlock) void k_sesph_acc(FluidDev f, GridDev g, int bdy, SphK c,
+    float k, const float* __restrict__ rho, const float* __restrict__ gbx,
+    const float* __restrict__ gby, const float* __restrict__ gbz, float* __restrict__ ax,
+    float* __restrict__ ay, float* __restrict__ az)
+{
+    const uint32_t t = global_index();
+    if (t >= f.n)
+        return;
+    const uint32_t i = particle_of<IND>(g, t);
+    const f3 xi = ld3(f.x, f.y, f.z, i);
+    const f3 vi = ld3(f.vx, f.vy, f.vz, i);
+    const float rhoi = rho[i];
+    const float pri = eos_sesph(rhoi, k, c.rho0) / (rhoi * rhoi);
+    f3 acc = c.g;
+    walk<IND>(g, xi, f.x, f.y, f.z, [&](uint32_t j, f3 xij, float d2) {
+        const float rhoj = rho[j];
+        const float mj = f.m[j];
+        const f3 vij = vi - ld3(f.vx, f.vy, f.vz, j);
+        const f3 dW = kdW<K>(c.kp, xij, d2);
+        const float pj = eos_sesph(rhoj, k, c.rho0);
+        const float visc = c.visc_c * mj / rhoj * dot(vij, xij) / (d2 + c.visc_eps);
+        acc += (visc - mj * (pri + pj / (rhoj * rhoj))) * dW;
+    });
+    if (bdy)
+        acc += -pri * ld3(gbx, gby, gbz, i);
+    st3(ax, ay, az, i, acc);
+}
+
+// ---- PCISPH ----------------------------------------------------------------------
+
+// predicted density; per-block FP64 partial sums feed the mean-density test
+template <int K, bool IND>
+__global__ __launch_bounds__(kBlock) void k_predict_rho(FluidDev f, GridDev g, int bdy, KParams kp,
+    float dt, const float* __restrict__ gbx, const float* __restrict__ gby,
+    const float* __restrict__ gbz, const float* __restrict__ rb, float* __restrict__ rho,
+    double* __restrict__ part, const SolveCtl* ctl)
+{
+    __shared__ double lds[kBlock / kWave];
+    if (loop_done(ctl))
+        return;
+    const uint32_t t = global_index();
+    double mine = 0.0;
+    if (t < f.n) {
+        const uint32_t i = particle_of<IND>(g, t);
+        const f3 xi = ld3(f.x, f.y, f.z, i);
+        const f3 vi = ld3(f.vx, f.vy, f.vz, i);
+        ksum acc;
+        walk<IND>(g, xi, f.x, f.y, f.z, [&](uint32_t j, f3 xij, float d2) {
+            const f3 vij = vi - ld3(f.vx, f.vy, f.vz, j);
+            acc.add(f.m[j] * (kW<K>(kp, d2) + dt * dot(kdW<K>(kp, xij, d2), vij)));
+        });
+        if (bdy)
+            acc.add(rb[i] + dt * dot(ld3(gbx, gby, gbz, i), vi));
+        const float r = acc.value();
+        rho[i] = r;
+        mine = r;
+    }
+    const double s = block_reduce<0>(mine, lds);
+    if (part && threadIdx.x == 0)
+        part[xcd_block(blockIdx.x, gridDim.x)] = s;
+}
+
+template <int K, bool IND>
+__global__ __launch_bounds__(kBlock) void k_pcisph_prs_acc(FluidDev f, GridDev g, int bdy,
+    KParams kp, float k, float rho0, float rho0_sq_inv, const float* __restrict__ rho,
+    const float* __restrict__ gbx, const float* __restrict__ gby, const float* __restrict__ gbz,
+    float* __restrict__ ax, float* __restrict__ ay, float* __restrict__ az, const SolveCtl* ctl)
+{
+    if (loop_done(ctl))
+        return;
+    const uint32_t t = global_index();
+    if (t >= f.n)
+        return;
+    const uint32_t i = particle_of<IND>(g, t);
+    const f3 xi = ld3(f.x, f.y, f.z, i);
+    const float rhoi = rho[i];
+    const float pi = fmaxf(0.f, k * (rhoi - rho0));
+    const float pri = pi / (rhoi * rhoi);
+    const float pr0 = pi * rho0_sq_inv;
+    f3 acc { 0.f, 0.f, 0.f };
+    walk<IND>(g, xi, f.x, f.y, f.z, [&](uint32_t j, f3 xij, float d2) {
+        const float rhoj = rho[j];
+        const float pj = fmaxf(0.f, k * (rhoj - rho0));
+        acc += (f.m[j] * (pri + pj / (rhoj * rhoj))) * kdW<K>(kp, xij, d2);
+    });
+    f3 a = -acc;
+    if (bdy)
+        a += -(0.5f * (pri + pr0)) * ld3(gbx, gby, gbz, i);
+    st3(ax, ay, az, i, a);
+}
+
+__global__ __launch_bounds__(kBlock) void k_ctl_pcisph(SolveCtl* ctl, const double* __restrict__ part,
+    uint32_t nb, uint32_t n, float thr, uint32_t min_iter, uint32_t max_iter)
+{
+    __shared__ double lds[kBlock / kWave];
+    if (ctl->done)
+        return;
+    double s = 0.0;
+    for (uint32_t k = threadIdx.x; k < nb; k += blockDim.x)
+        s += part[k];
+    s = block_reduce<0>(s, lds);
+    if (threadIdx.x == 0) {
+        const uint32_t l = ctl->iter + 1;
+        const double avg = s / double(n);
+        const bool keep = (l < min_iter) || (avg > double(thr));
+        ctl->iter = l;
+        ctl->avg_density = avg;
+        ctl->done = (!keep || (max_iter && l >= max_iter)) ? 1u : 0u;
+    }
+}
+
+// ---- IISPH -------------------------------------------------------------------------
+
+__device__ __forceinline__ float jacobi_update(float omega, float aii, float Ap, float s)
+{
+    // |a_ii| < 1e-6 compared in double in the reference; these float bounds
+    // select exactly the same floats
+    const float lim = 9.99999997475242708e-7f;
+    return (aii <= lim && aii >= -lim) ? 0.f : fmaxf(0.f, omega * (s - Ap) / aii);
+}
+
+template <int K, bool IND>
+__global__ __launch_bounds__(kBlock) void k_iisph_aii_s(FluidDev f, GridDev g, int bdy, KParams kp,
+    float rho0, float dt, float omega, int warm, const float* __restrict__ rho,
+    const float* __restrict__ gbx, const float* __restrict__ gby, const float* __restrict__ gbz,
+    float* __restrict__ aii, float* __restrict__ si, float* __restrict__ p)
+{
+    const uint32_t t = global_index();
+    if (t >= f.n)
+        return;
+    const uint32_t i = particle_of<IND>(g, t);
+    const f3 xi = ld3(f.x, f.y, f.z, i);
+    const f3 vi = ld3(f.vx, f.vy, f.vz, i);
+    ksum3 grad;
+    ksum sq, div;
+    walk<IND>(g, xi, f.x, f.y, f.z, [&](uint32_t j, f3 xij, float d2) {
+        const f3 dW = kdW<K>(kp, xij, d2);
+        const f3 mdW = f.m[j] * dW;
+        div.add(dot(mdW, vi - ld3(f.vx, f.vy, f.vz, j)));
+        sq.add(dot(mdW, dW));
+        grad.add(mdW);
+    });
+    f3 gb { 0.f, 0.f, 0.f };
+    if (bdy)
+        gb = ld3(gbx, gby, gbz, i);
+    const f3 G = grad.value() + gb;
+    const float a = -dt * (dot(G, G) + f.m[i] * sq.value()) / (rho0 * rho0);
+    const float s = (rho0 - rho[i]) / dt - (div.value() + dot(vi, gb));
+    aii[i] = a;
+    si[i] = s;
+    p[i] = warm ? p[i] * 0.5f : jacobi_update(omega, a, 0.f, s);
+}
+
+template <int K, bool IND>
+__global__ __launch_bounds__(kBlock) void k_iisph_pacc(FluidDev f, GridDev g, int bdy, KParams kp,
+    float rho0_sq_inv, const float* __restrict__ p, const float* __restrict__ gbx,
+    const float* __restrict__ gby, const float* __restrict__ gbz, float* __restrict__ ax,
+    float* __restrict__ ay, float* __restrict__ az, const SolveCtl* ctl)
+{
+    if (loop_done(ctl))
+        return;
+    const uint32_t t = global_index();
+    if (t >= f.n)
+        return;
+    const uint32_t i = particle_of<IND>(g, t);
+    const f3 xi = ld3(f.x, f.y, f.z, i);
+    const float pi = p[i];
+    f3 acc { 0.f, 0.f, 0.f };
+    walk<IND>(g, xi, f.x, f.y, f.z,
+        [&](uint32_t j, f3 xij, float d2) { acc += (f.m[j] * (pi + p[j])) * kdW<K>(kp, xij, d2); });
+    if (bdy)
+        acc += pi * ld3(gbx, gby, gbz, i);
+    st3(ax, ay, az, i, -rho0_sq_inv * acc);
+}
+
+template <int K, bool IND>
+__global__ __launch_bounds__(kBlock) void k_iisph_jacobi(FluidDev f, GridDev g, int bdy, KParams kp,
+    float dt, float omega, float thr, const float* __restrict__ ax, const float* __restrict__ ay,
+    const float* __restrict__ az, const float* __restrict__ aii, const float* __restrict__ si,
+    const float* __restrict__ gbx, const float* __restrict__ gby, const float* __restrict__ gbz,
+    float* __restrict__ p, double* __restrict__ part_err, uint32_t* __restrict__ part_cnt,
+    const SolveCtl* ctl)
+{
+    __shared__ double lds[kBlock / kWave];
+    if (loop_done(ctl))
+        return;
+    const uint32_t t = global_index();
+    double err_pos = 0.0;
+    uint32_t viol = 0;
+    if (t < f.n) {
+        const uint32_t i = particle_of<IND>(g, t);
+        const f3 xi = ld3(f.x, f.y, f.z, i);
+        const f3 ai = ld3(ax, ay, az, i);
+        float lap = 0.f;
+        walk<IND>(g, xi, f.x, f.y, f.z, [&](uint32_t j, f3 xij, float d2) {
+            lap += f.m[j] * dot(ai - ld3(ax, ay, az, j), kdW<K>(kp, xij, d2));
+        });
+        if (bdy)
+            lap += dot(ai, ld3(gbx, gby, gbz, i));
+        const float Ap = dt * lap;
+        const float s = si[i];
+        p[i] = p[i] + jacobi_update(omega, aii[i], Ap, s);
+        const float err = (Ap - s) * dt;
+        viol = err > thr ? 1u : 0u;
+        err_pos = err > 0.f ? double(err) : 0.0;
+    }
+    const uint64_t vb = __ballot(viol != 0);
+    const double es = block_reduce<0>(err_pos, lds);
+    __shared__ uint32_t vc[kBlock / kWave];
+    if ((threadIdx.x & (kWave - 1)) == 0)
+        vc[threadIdx.x / kWave] = static_cast<uint32_t>(__popcll(vb));
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t c = 0;
+        for (int w = 0; w < kBlock / kWave; ++w)
+            c += vc[w];
+        const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
+        part_err[b] = es;
+        part_cnt[b] = c;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_ctl_iisph(SolveCtl* ctl, const double* __restrict__ part_err,
+    const uint32_t* __restrict__ part_cnt, uint32_t nb, uint32_t n, float rho0, float eta,
+    uint32_t min_iter, uint32_t max_iter, int rule)
+{
+    __shared__ double lds[kBlock / kWave];
+    if (ctl->done)
+        return;
+    double s = 0.0, c = 0.0;
+    for (uint32_t k = threadIdx.x; k < nb; k += blockDim.x) {
+        s += part_err[k];
+        c += double(part_cnt[k]);
+    }
+    s = block_reduce<0>(s, lds);
+    c = block_reduce<0>(c, lds);
+    if (threadIdx.x == 0) {
+        const uint32_t l = ctl->iter + 1;
+        const double mean = (n > 0) ? s / double(n) / double(rho0) : 0.0;
+        const uint32_t viol = static_cast<uint32_t>(c);
+        const bool ok = (rule == REYN_IISPH_STOP_MEAN) ? (mean <= double(eta)) : (viol == 0);
+        ctl->iter = l;
+        ctl->n_violating = viol;
+        ctl->mean_rel_error = mean;
+        ctl->done = ((l >= min_iter && ok) || (max_iter && l >= max_iter)) ? 1u : 0u;
+    }
+}
+
+__global__ void k_ctl_reset(SolveCtl* ctl)
+{
+    if (threadIdx.x == 0) {
+        ctl->iter = 0;
+        ctl->done = 0;
+        ctl->n_violating = 0;
+        ctl->pad = 0;
+        ctl->mean_rel_error = 0.0;
+        ctl->avg_density = 0.0;
+    }
+}
+
+// ---- host-side helpers ---------------------------------------------------------------
+
+struct Common {
+    FluidDev f;
+    GridDev g;
+    BdyDev b;
+    bool ind;
+    uint32_t nb;
+};
+
+int prepare(reyn_ctx* ctx, const reyn_sph_params* prm, const reyn_particles* fluid,
+    const reyn_grid* grid, const reyn_boundary* bdy, Common* c)
+{
+    if (!ctx)
+        return REYN_ERR_INVALID_ARGUMENT;
+    REYN_TRY(check_sph(ctx, prm));
+    REYN_TRY(check_fluid(ctx, fluid));
+    REYN_TRY(check_grid(ctx, grid, fluid->n, "fluid"));
+    REYN_TRY(check_bdy(ctx, bdy));
+    REYN_HIP(ctx, hipSetDevice(ctx->device));
+    c->f = to_dev(*fluid);
+    c->g = to_dev(*grid);
+    c->b = to_dev(bdy);
+    c->ind = grid->sorted != nullptr;
+    c->nb = blocks_for(fluid->n);
+    return REYN_OK;
+}
+
+template <bool GB>
+int launch_density(reyn_ctx* ctx, int kernel, const Common& c, const KParams& kp, float* rho,
+    const Scratch* s)
+{
+    return with_kernel(ctx, kernel, c.ind, [&](auto KC, auto IC) -> int {
+        constexpr int K = decltype(KC)::value;
+        constexpr bool I = decltype(IC)::value;
+        {
+            ProfScope _prof(ctx, "k_density");
+            k_density<K, I, GB><<<c.nb, kBlock, 0, ctx->stream>>>(c.f, c.g, c.b, kp, rho,
+                GB ? s->gbx : nullptr, GB ? s->gby : nullptr, GB ? s->gbz : nullptr, GB ? s->rb : nullptr);
+        }
+        REYN_LAUNCHED(ctx, "k_density");
+        return REYN_OK;
+    });
+}
+
+int launch_nonpressure(reyn_ctx* ctx, int kernel, const Common& c, const SphK& k, const float* rho,
+    float* ax, float* ay, float* az)
+{
+    return with_kernel(ctx, kernel, c.ind, [&](auto KC, auto IC) -> int {
+        constexpr int K = decltype(KC)::value;
+        constexpr bool I = decltype(IC)::value;
+        {
+            ProfScope _prof(ctx, "k_nonpressure");
+            k_nonpressure<K, I><<<c.nb, kBlock, 0, ctx->stream>>>(c.f, c.g, k, rho, ax, ay, az);
+        }
+        REYN_LAUNCHED(ctx, "k_nonpressure");
+        return REYN_OK;
+    });
+}
+
+int launch_integrate(reyn_ctx* ctx, const Common& c, const float* ax, const float* ay,
+    const float* az, float dt, const SolveCtl* ctl, int positions)
+{
+    {
+        ProfScope _prof(ctx, positions ? "k_integrate" : "k_integrate_v");
+        k_integrate<<<c.nb, kBlock, 0, ctx->stream>>>(c.f, ax, ay, az, dt, ctl, positions);
+    }
+    REYN_LAUNCHED(ctx, "k_integrate");
+    return REYN_OK;
+}
+
+// Enqueue loop iterations in chunks until the control block reports done.
+template <class Iter>
+int run_loop(reyn_ctx* ctx, SolveCtl* ctl, uint32_t min_iter, uint32_t max_iter, uint32_t check_every,
+    Iter&& iter, SolveCtl* out)
+{
+    SolveCtl* host = nullptr;
+    REYN_TRY(ctx_pinned(ctx, sizeof(SolveCtl), reinterpret_cast<void**>(&host)));
+    {
+        ProfScope _prof(ctx, "k_ctl_reset");
+        k_ctl_reset<<<1, kWave, 0, ctx->stream>>>(ctl);
+    }
+    REYN_LAUNCHED(ctx, "k_ctl_reset");
+    const uint32_t chunk = check_every ? check_every : 4;
+    // without a cap the reference loops until convergence; this safety cap
+    // only turns a runaway solve into an error instead of an endless loop
+    const uint32_t hard_cap = max_iter ? max_iter : 100000u;
+    uint32_t launched = 0;
+    for (;;) {
+        uint32_t m = (launched == 0) ? std::max(min_iter, chunk) : chunk;
+        m = std::min(m, hard_cap - launched);
+        for (uint32_t k = 0; k < m; ++k)
+            REYN_TRY(iter());
+        launched += m;
+        REYN_HIP(ctx, hipMemcpyAsync(host, ctl, sizeof(SolveCtl), hipMemcpyDeviceToHost, ctx->stream));
+        REYN_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        if (host->done)
+            break;
+        if (launched >= hard_cap)
+            return set_error(ctx, REYN_ERR_NOT_CONVERGED, "solver did not converge in %u iterations", launched);
+    }
+    *out = *host;
+    return REYN_OK;
+}
+
+} // namespace
+} // namespace reyn
+
+using namespace reyn;
+
+extern "C" {
+
+int reyn_density(reyn_ctx* ctx, const reyn_sph_params* prm, const reyn_particles* fluid,
+    const reyn_grid* grid, const reyn_boundary* bdy, float* rho)
+{
+    Common c;
+    REYN_TRY(prepare(ctx, prm, fluid, grid, bdy, &c));
+    if (!rho && fluid->n)
+        return set_error(ctx, REYN_ERR_INVALID_ARGUMENT, "density: rho is NULL");
+    if (fluid->n == 0)
+        return REYN_OK;
+    return launch_density<false>(ctx, prm->kernel, c, make_kparams(prm->kernel, prm->h_bar), rho, nullptr);
+}
+
+int reyn_nonpressure_acc(reyn_ctx* ctx, const reyn_sph_params* prm, const reyn_particles* fluid,
+    const reyn_grid* grid, const float* rho, float* ax, float* ay, float* az)
+{
+    Common c;
+    REYN_TRY(prepare(ctx, prm, fluid, grid, nullptr, &c));
+    if (fluid->n == 0)
+        return REYN_OK;
+    if (!rho || !ax || !ay || !az)
+        return set_error(ctx, REYN_ERR_INVALID_ARGUMENT, "nonpressure_acc: NULL buffer");
+    return launch_nonpressure(ctx, prm->kernel, c, make_sphk(*prm), rho, ax, ay, az);
+}
+
+int reyn_predict_rho(reyn_ctx* ctx, const reyn_sph_params* prm, const reyn_particles* fluid,
+    const reyn_grid* grid, const reyn_boundary* bdy, float dt, float* rho)
+{
+    Common c;
+    REYN_TRY(prepare(ctx, prm, fluid, grid, bdy, &c));
+    if (fluid->n == 0)
+        return REYN_OK;
+    if (!rho)
+        return set_error(ctx, REYN_ERR_INVALID_ARGUMENT, "predict_rho: rho is NULL");
+    Scratch s;
+    REYN_TRY(get_scratch(ctx, fluid->n, &s));
+    const KParams kp = make_kparams(prm->kernel, prm->h_bar);
+    // the boundary sums come from a density pass into scratch; its density
+    // output lands in `rho` and is overwritten by the prediction
+    if (c.b.present)
+        REYN_TRY(launch_density<true>(ctx, prm->kernel, c, kp, rho, &s));
+    return with_kernel(ctx, prm->kernel, c.ind, [&](auto KC, auto IC) -> int {
+        constexpr int K = decltype(KC)::value;
+        constexpr bool I = decltype(IC)::value;
+        {
+            ProfScope _prof(ctx, "k_predict_rho");
+            k_predict_rho<K, I><<<c.nb, kBlock, 0, ctx->stream>>>(c.f, c.g, c.b.present, kp, dt, s.gbx,
+                s.gby, s.gbz, s.rb, rho, nullptr, nullptr);
+        }
+        REYN_LAUNCHED(ctx, "k_predict_rho");
+        return REYN_OK;
+    });
+}
+
+int reyn_integrate(reyn_ctx* ctx, const reyn_particles* fluid, const float* ax, const float* ay,
+    const float* az, float dt)
+{
+    if (!ctx)
+        return REYN_ERR_INVALID_ARGUMENT;
+    REYN_TRY(check_fluid(ctx, fluid));
+    if (fluid->n == 0)
+        return REYN_OK;
+    Common c {};
+    c.f = to_dev(*fluid);
+    c.nb = blocks_for(fluid->n);
+    return launch_integrate(ctx, c, ax, ay, az, dt, nullptr, 1);
+}
+
+int reyn_integrate_a_v_only(reyn_ctx* ctx, const reyn_particles* fluid, const float* ax,
+    const float* ay, const float* az, float dt)
+{
+    if (!ctx)
+        return REYN_ERR_INVALID_ARGUMENT;
+    REYN_TRY(check_fluid(ctx, fluid));
+    if (fluid->n == 0)
+        return REYN_OK;
+    Common c {};
+    c.f = to_dev(*fluid);
+    c.nb = blocks_for(fluid->n);
+    return launch_integrate(ctx, c, ax, ay, az, dt, nullptr, 0);
+}
+
+int reyn_sesph_step(reyn_ctx* ctx, const reyn_sesph_params* prm, const reyn_particles* fluid,
+    const reyn_grid* grid, const reyn_boundary* bdy, const reyn_sesph_work* work, float dt)
+{
+    if (!prm || !work)
+        return set_error(ctx, REYN_ERR_INVALID_ARGUMENT, "sesph: NULL parameters or work buffers");
+    Common c;
+    REYN_TRY(prepare(ctx, &prm->sph, fluid, grid, bdy, &c));
+    if (fluid->n == 0)
+        return REYN_OK;
+    if (!work->rho || !work->ax || !work->ay || !work->az)
+        return set_error(ctx, REYN_ERR_INVALID_ARGUMENT, "sesph: NULL work buffer");
+    Scratch s;
+    REYN_TRY(get_scratch(ctx, fluid->n, &s));
+    const SphK k = make_sphk(prm->sph);
+    REYN_TRY(launch_density<true>(ctx, prm->sph.kernel, c, k.kp, work->rho, &s));
+    REYN_TRY(with_kernel(ctx, prm->sph.kernel, c.ind, [&](auto KC, auto IC) -> int {
+        constexpr int K = decltype(KC)::value;
+        constexpr bool I = decltype(IC)::value;
+        {
+            ProfScope _prof(ctx, "k_sesph_acc");
+            k_sesph_acc<K, I><<<c.nb, kBlock, 0, ctx->stream>>>(c.f, c.g, c.b.present, k, prm->k,
+                work->rho, s.gbx, s.gby, s.gbz, work->ax, work->ay, work->az);
+        }
+        REYN_LAUNCHED(ctx, "k_sesph_acc");
+        return REYN_OK;
+    }));
+    return launch_integrate(ctx, c, work->ax, work->ay, work->az, dt, nullptr, 1);
+}
+
+int reyn_pcisph_delta(int32_t kernel, float h_bar, float h, double* delta)
+{
+    if (!delta || kernel < 0 || kernel > 3 || !(h_bar > 0.f) || !(h > 0.f))
+        return REYN_ERR_INVALID_ARGUMENT;
+    // template particle on a 5^3 lattice of spacing h (PCISPH.cuh:49-77); the
+    // gradient is evaluated in float on the host as in the reference
+    const KParams kp = make_kparams(kernel, h_bar);
+    float sx = 0.f, sy = 0.f, sz = 0.f;
+    double sum_sq = 0.0;
+    const int kappa = 2;
+    for (int x = -kappa; x <= kappa; ++x)
+        for (int y = -kappa; y <= kappa; ++y)
+            for (int z = -kappa; z <= kappa; ++z) {
+                const float dx = -(static_cast<float>(x) * h), dy = -(static_cast<float>(y) * h),
+                            dz = -(static_cast<float>(z) * h);
+                const float d2 = std::fma(dx, dx, std::fma(dy, dy, dz * dz));
+                const float len = std::sqrt(d2);
+                const float q = len * kp.hinv;
+                float dw;
+                const float t1 = std::fmax(0.f, 1.f - q);
+                switch (kernel) {
+                case KC2:
+                    dw = t1 * t1 * t1 * (-20.f * q);
+                    break;
+                case KB3: {
+                    const float t2 = std::fmin(std::fmax(0.5f - q, 0.f), 1.f);
+                    dw = -3.f * t1 * t1 + 12.f * t2 * t2;
+                    break;
+                }
+                case KW6: {
+                    const float t2 = t1 * t1;
+                    const float t7 = t2 * t2 * t2 * t1;
+                    dw = t7 * ((16.f * q * q + 7.f * q + 1.f) * (-22.f * q));
+                    break;
+                }
+                default: {
+                    const float pis = 3.14159265358979323846f * std::fmin(1.f, q);
+                    const float two_pi = 6.28318530717958647692f;
+                    dw = -2.f * two_pi * std::sin(pis) - std::sin(2.f * pis) * two_pi;
+                    break;
+                }
+                }
+                float scale = kp.dws * dw / len;
+                if (!std::isfinite(scale))
+                    scale = 0.f;
+                const float gx = scale * dx, gy = scale * dy, gz = scale * dz;
+                sx += gx;
+                sy += gy;
+                sz += gz;
+                sum_sq += static_cast<double>(std::fma(gx, gx, std::fma(gy, gy, gz * gz)));
+            }
+    const double sq_of_sum = static_cast<double>(std::fma(sx, sx, std::fma(sy, sy, sz * sz)));
+    const double V0 = double(h) * double(h) * double(h);
+    const double beta = 2.0 * V0 * V0;
+    *delta = static_cast<double>(static_cast<float>(1.0 / (beta * (sq_of_sum + sum_sq))));
+    return REYN_OK;
+}
+
+int reyn_pcisph_step(reyn_ctx* ctx, const reyn_pcisph_params* prm, const reyn_particles* fluid,
+    const reyn_grid* grid, const reyn_boundary* bdy, const reyn_pcisph_work* work, float dt,
+    reyn_solve_info* info)
+{
+    if (!prm || !work)
+        return set_error(ctx, REYN_ERR_INVALID_ARGUMENT, "pcisph: NULL parameters or work buffers");
+    Common c;
+    REYN_TRY(prepare(ctx, &prm->sph, fluid, grid, bdy, &c));
+    if (info)
+        *info = reyn_solve_info {};
+    if (fluid->n == 0)
+        return REYN_OK;
+    if (!work->rho || !work->ax || !work->ay || !work->az)
+        return set_error(ctx, REYN_ERR_INVALID_ARGUMENT, "pcisph: NULL work buffer");
+    if (!(dt > 0.f))
+        return set_error(ctx, REYN_ERR_INVALID_ARGUMENT, "pcisph: dt must be positive");
+    Scratch s;
+    REYN_TRY(get_scratch(ctx, fluid->n, &s));
+    const SphK k = make_sphk(prm->sph);
+    const float kk = static_cast<float>((prm->delta / dt) / dt);
+    const float rho0 = prm->sph.rho_0;
+    const float thr = (prm->eta_rho_max + 1.f) * rho0;
+    REYN_TRY(launch_density<true>(ctx, prm->sph.kernel, c, k.kp, work->rho, &s));
+    REYN_TRY(launch_nonpressure(ctx, prm->sph.kernel, c, k, work->rho, work->ax, work->ay, work->az));
+    SolveCtl res;
+    REYN_TRY(run_loop(
+        ctx, s.ctl, prm->min_iter, prm->max_iter, 4,
+        [&]() -> int {
+            REYN_TRY(launch_integrate(ctx, c, work->ax, work->ay, work->az, dt, s.ctl, 0));
+            REYN_TRY(with_kernel(ctx, prm->sph.kernel, c.ind, [&](auto KC, auto IC) -> int {
+                constexpr int K = decltype(KC)::value;
+                constexpr bool I = decltype(IC)::value;
+                {
+                    ProfScope _prof(ctx, "k_predict_rho");
+                    k_predict_rho<K, I><<<c.nb, kBlock, 0, ctx->stream>>>(c.f, c.g, c.b.present, k.kp, dt,
+                        s.gbx, s.gby, s.gbz, s.rb, work->rho, s.part_d, s.ctl);
+                }
+                REYN_LAUNCHED(ctx, "k_predict_rho");
+                {
+                    ProfScope _prof(ctx, "k_pcisph_prs_acc");
+                    k_pcisph_prs_acc<K, I><<<c.nb, kBlock, 0, ctx->stream>>>(c.f, c.g, c.b.present, k.kp, kk,
+                        rho0, 1.f / (rho0 * rho0), work->rho, s.gbx, s.gby, s.gbz, work->ax, work->ay,
+                        work->az, s.ctl);
+                }
+                REYN_LAUNCHED(ctx, "k_pcisph_prs_acc");
+                return REYN_OK;
+            }));
+            {
+                ProfScope _prof(ctx, "k_ctl_pcisph");
+                k_ctl_pcisph<<<1, kBlock, 0, ctx->stream>>>(s.ctl, s.part_d, c.nb, fluid->n, thr,
+                    prm->min_iter, prm->max_iter);
+            }
+            REYN_LAUNCHED(ctx, "k_ctl_pcisph");
+            return REYN_OK;
+        },
+        &res));
+    REYN_TRY(launch_integrate(ctx, c, work->ax, work->ay, work->az, dt, nullptr, 1));
+    if (info) {
+        info->iterations = res.iter;
+        info->avg_density = res.avg_density;
+    }
+    return REYN_OK;
+}
+
+int reyn_iisph_step(reyn_ctx* ctx, const reyn_iisph_params* prm, const reyn_particles* fluid,
+    const reyn_grid* grid, const reyn_boundary* bdy, const reyn_iisph_work* work, float dt,
+    reyn_solve_info* info)
+{
+    if (!prm || !work)
+        return set_error(ctx, REYN_ERR_INVALID_ARGUMENT, "iisph: NULL parameters or work buffers");
+    Common c;
+    REYN_TRY(prepare(ctx, &prm->sph, fluid, grid, bdy, &c));
+    if (info)
+        *info = reyn_solve_info {};
+    if (fluid->n == 0)
+        return REYN_OK;
+    if (!work->rho || !work->p || !work->a_ii || !work->s_i || !work->ax || !work->ay || !work->az)
+        return set_error(ctx, REYN_ERR_INVALID_ARGUMENT, "iisph: NULL work buffer");
+    if (!(dt > 0.f))
+        return set_error(ctx, REYN_ERR_INVALID_ARGUMENT, "iisph: dt must be positive");
+    if (prm->stop_rule != REYN_IISPH_STOP_ANY && prm->stop_rule != REYN_IISPH_STOP_MEAN)
+        return set_error(ctx, REYN_ERR_INVALID_ARGUMENT, "iisph: unknown stop rule %d", prm->stop_rule);
+    Scratch s;
+    REYN_TRY(get_scratch(ctx, fluid->n, &s));
+    const SphK k = make_sphk(prm->sph);
+    const float rho0 = prm->sph.rho_0;
+    const float rho0_sq_inv = 1.f / (rho0 * rho0);
+    const float thr = prm->eta_rho_max * rho0;
+    const int kind = prm->sph.kernel;
+    const int bd = c.b.present;
+    REYN_TRY(launch_density<true>(ctx, kind, c, k.kp, work->rho, &s));
+    REYN_TRY(launch_nonpressure(ctx, kind, c, k, work->rho, work->ax, work->ay, work->az));
+    REYN_TRY(launch_integrate(ctx, c, work->ax, work->ay, work->az, dt, nullptr, 0));
+    REYN_TRY(with_kernel(ctx, kind, c.ind, [&](auto KC, auto IC) -> int {
+        constexpr int K = decltype(KC)::value;
+        constexpr bool I = decltype(IC)::value;
+        {
+            ProfScope _prof(ctx, "k_iisph_aii_s");
+            k_iisph_aii_s<K, I><<<c.nb, kBlock, 0, ctx->stream>>>(c.f, c.g, bd, k.kp, rho0, dt, prm->omega,
+                prm->warm_start, work->rho, s.gbx, s.gby, s.gbz, work->a_ii, work->s_i, work->p);
+        }
+        REYN_LAUNCHED(ctx, "k_iisph_aii_s");
+        return REYN_OK;
+    }));
+    auto pacc = [&](const SolveCtl* ctl) -> int {
+        return with_kernel(ctx, kind, c.ind, [&](auto KC, auto IC) -> int {
+            constexpr int K = decltype(KC)::value;
+            constexpr bool I = decltype(IC)::value;
+            {
+                ProfScope _prof(ctx, "k_iisph_pacc");
+                k_iisph_pacc<K, I><<<c.nb, kBlock, 0, ctx->stream>>>(c.f, c.g, bd, k.kp, rho0_sq_inv,
+                    work->p, s.gbx, s.gby, s.gbz, work->ax, work->ay, work->az, ctl);
+            }
+            REYN_LAUNCHED(ctx, "k_iisph_pacc");
+            return REYN_OK;
+        });
+    };
+    SolveCtl res;
+    REYN_TRY(run_loop(
+        ctx, s.ctl, prm->min_iter, prm->max_iter, prm->check_every,
+        [&]() -> int {
+            REYN_TRY(pacc(s.ctl));
+            REYN_TRY(with_kernel(ctx, kind, c.ind, [&](auto KC, auto IC) -> int {
+                constexpr int K = decltype(KC)::value;
+                constexpr bool I = decltype(IC)::value;
+                {
+                    ProfScope _prof(ctx, "k_iisph_jacobi");
+                    k_iisph_jacobi<K, I><<<c.nb, kBlock, 0, ctx->stream>>>(c.f, c.g, bd, k.kp, dt, prm->omega,
+                        thr, work->ax, work->ay, work->az, work->a_ii, work->s_i, s.gbx, s.gby, s.gbz, work->p,
+                        s.part_d, s.part_u, s.ctl);
+                }
+                REYN_LAUNCHED(ctx, "k_iisph_jacobi");
+                return REYN_OK;
+            }));
+            {
+                ProfScope _prof(ctx, "k_ctl_iisph");
+                k_ctl_iisph<<<1, kBlock, 0, ctx->stream>>>(s.ctl, s.part_d, s.part_u, c.nb, fluid->n, rho0,
+                    prm->eta_rho_max, prm->min_iter, prm->max_iter, prm->stop_rule);
+            }
+            REYN_LAUNCHED(ctx, "k_ctl_iisph");
+            return REYN_OK;
+        },
+        &res));
+    REYN_TRY(pacc(nullptr));
+    REYN_TRY(launch_integrate(ctx, c, work->ax, work->ay, work->az, dt, nullptr, 1));
+    if (info) {
+        info->iterations = res.iter;
+        info->n_violating = res.n_violating;
+        info->mean_rel_error = res.mean_rel_error;
+    }
+    return REYN_OK;
+}
+
+} // extern "C"

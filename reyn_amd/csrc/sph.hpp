@@ -1,0 +1,282 @@
+// Device-side building blocks of every SPH phase: smoothing kernels, the
+// grid view and the fixed-radius neighbour walk.
+#pragma once
+
+#include <type_traits>
+
+#include "common.hpp"
+
+namespace reyn {
+
+// ---- smoothing kernels (src/kernels.cuh:19-174) ---------------------------
+// W(x) = alpha/h^3 w(sat(|x|/h)), grad W(x) = alpha/h^4 dw(|x|/h)/|x| x, with
+// the gradient forced to 0 when the scale is not finite (|x| = 0). alpha is
+// formed in double and rounded to float as in src/kernels.cuh:32-37.
+
+enum : int { KC2 = REYN_KERNEL_C2, KB3 = REYN_KERNEL_B3, KW6 = REYN_KERNEL_W6, KCOS = REYN_KERNEL_COS };
+
+struct KParams {
+    float hinv; // 1/h_bar
+    float ws; // alpha / h_bar^3
+    float dws; // alpha / h_bar^4
+};
+
+inline double kernel_alpha(int kind)
+{
+    constexpr double one_over_pi = 0.318309886183790671537767526745028724;
+    constexpr double pi = 3.14159265358979323846264338327950288;
+    switch (kind) {
+    case KC2:
+        return 10.5 * one_over_pi; // 21/(2 pi)
+    case KB3:
+        return 16.0 * one_over_pi; // 16/pi
+    case KW6:
+        return 21.328125 * one_over_pi; // 1365/(64 pi)
+    case KCOS:
+        return pi / (4.0 * pi * pi - 30.0);
+    default:
+        return 0.0;
+    }
+}
+
+inline KParams make_kparams(int kind, float h_bar)
+{
+    const float alpha = static_cast<float>(kernel_alpha(kind));
+    KParams p;
+    p.hinv = 1.0f / h_bar;
+    p.ws = alpha / (h_bar * h_bar * h_bar);
+    p.dws = alpha / (h_bar * h_bar * h_bar * h_bar);
+    return p;
+}
+
+__device__ __forceinline__ float sat(float a) { return fminf(fmaxf(a, 0.f), 1.f); }
+
+template <int K> struct Shape;
+
+template <> struct Shape<KC2> {
+    __device__ __forceinline__ static float w(float q)
+    {
+        const float t1 = fmaxf(0.f, 1.f - q);
+        const float t12 = t1 * t1;
+        return t12 * t12 * (4.f * q + 1.f);
+    }
+    __device__ __forceinline__ static float dw(float q)
+    {
+        const float t1 = fmaxf(0.f, 1.f - q);
+        return t1 * t1 * t1 * (-20.f * q);
+    }
+};
+template <> struct Shape<KB3> {
+    __device__ __forceinline__ static float w(float q)
+    {
+        const float t1 = fmaxf(0.f, 1.f - q);
+        const float t2 = sat(0.5f - q);
+        return t1 * t1 * t1 - 4.f * t2 * t2 * t2;
+    }
+    __device__ __forceinline__ static float dw(float q)
+    {
+        const float t1 = fmaxf(0.f, 1.f - q);
+        const float t2 = sat(0.5f - q);
+        return -3.f * t1 * t1 + 12.f * t2 * t2;
+    }
+};
+template <> struct Shape<KW6> {
+    __device__ __forceinline__ static float w(float q)
+    {
+        const float t1 = fmaxf(0.f, 1.f - q);
+        const float t2 = t1 * t1;
+        const float t4 = t2 * t2;
+        const float q2 = q * q;
+        return (t4 * t4) * (32.f * q2 * q + 25.f * q2 + 8.f * q + 1.f);
+    }
+    __device__ __forceinline__ static float dw(float q)
+    {
+        const float t1 = fmaxf(0.f, 1.f - q);
+        const float t2 = t1 * t1;
+        const float t7 = t2 * t2 * t2 * t1;
+        return t7 * ((16.f * q * q + 7.f * q + 1.f) * (-22.f * q));
+    }
+};
+template <> struct Shape<KCOS> {
+    __device__ __forceinline__ static float w(float q)
+    {
+        const float pis = 3.14159265358979323846f * fminf(1.f, q);
+        return 4.f * cosf(pis) + cosf(2.f * pis) + 3.f;
+    }
+    __device__ __forceinline__ static float dw(float q)
+    {
+        const float pis = 3.14159265358979323846f * fminf(1.f, q);
+        const float two_pi = 6.28318530717958647692f;
+        return -2.f * two_pi * sinf(pis) - sinf(2.f * pis) * two_pi;
+    }
+};
+
+// W from the squared distance (the neighbour walk already formed it).
+template <int K> __device__ __forceinline__ float kW(const KParams& p, float d2)
+{
+    const float len = __builtin_sqrtf(d2);
+    return p.ws * Shape<K>::w(sat(len * p.hinv));
+}
+
+// grad W from the offset and its squared length.
+template <int K> __device__ __forceinline__ f3 kdW(const KParams& p, f3 x, float d2)
+{
+    const float len = __builtin_sqrtf(d2);
+    const float scale = p.dws * Shape<K>::dw(len * p.hinv) / len;
+    return (__builtin_isfinite(scale) ? scale : 0.f) * x;
+}
+
+// ---- grid view and neighbour walk -----------------------------------------
+// (src/datastructure/uniformgrid.cuh:17-64, 113-178)
+
+struct GridDev {
+    f3 bmin;
+    float cell;
+    float r2;
+    int nx, ny, nz, nxny;
+    int range;
+    const uint32_t* __restrict__ prefix;
+    const uint32_t* __restrict__ sorted;
+};
+
+inline GridDev to_dev(const reyn_grid& g)
+{
+    GridDev d;
+    d.bmin = { g.bound_min.x, g.bound_min.y, g.bound_min.z };
+    d.cell = g.cell_size;
+    d.r2 = g.r_c_2;
+    d.nx = g.nx;
+    d.ny = g.ny;
+    d.nz = g.nz;
+    d.nxny = g.nx * g.ny;
+    d.range = g.range;
+    d.prefix = g.prefix;
+    d.sorted = g.sorted;
+    return d;
+}
+
+// floor((x - bound_min) / cell) per axis with IEEE division, clamped to
+// [0, n-1]; NaN maps to 0. Inside the grid this is the reference's `_index3`
+// (uniformgrid.cuh:17-22, vector.cuh:137-141); the clamp only changes points
+// the reference would index out of bounds.
+__device__ __forceinline__ int cell_axis(float x, float lo, float cell, int n)
+{
+    float f = floorf((x - lo) / cell);
+    f = fmaxf(f, 0.f); // NaN -> 0
+    f = fminf(f, static_cast<float>(n - 1));
+    return static_cast<int>(f);
+}
+struct int3c {
+    int x, y, z;
+};
+__device__ __forceinline__ int3c cell_of(const GridDev& g, f3 x)
+{
+    return { cell_axis(x.x, g.bmin.x, g.cell, g.nx), cell_axis(x.y, g.bmin.y, g.cell, g.ny),
+        cell_axis(x.z, g.bmin.z, g.cell, g.nz) };
+}
+
+constexpr uint32_t kWalkBudget = 1u << 15;
+
+// Calls f(j, x_ij, |x_ij|^2) for every indexed point j with
+// |x_ij|^2 = fmaf-dot <= r_c_2 (the point itself included), walking
+// (2R+1)^2 x-strips of 2R+1 cells, each one contiguous range of the
+// cell-ordered arrays. Strips are clipped to the grid, which leaves every
+// point whose cell lies R or more cells inside the grid unchanged.
+template <bool IND, class F>
+__device__ __forceinline__ void walk(const GridDev& g, f3 xi, const float* __restrict__ xs,
+    const float* __restrict__ ys, const float* __restrict__ zs, F&& f)
+{
+    const int3c c = cell_of(g, xi);
+    const int R = g.range;
+    const int x0 = max(c.x - R, 0), x1 = min(c.x + R, g.nx - 1);
+    const int y0 = max(c.y - R, 0), y1 = min(c.y + R, g.ny - 1);
+    const int z0 = max(c.z - R, 0), z1 = min(c.z + R, g.nz - 1);
+    // candidate budget: only degenerate inputs (thousands of points per cell,
+    // already reported by the build) reach it; it keeps every walk bounded
+    uint32_t budget = kWalkBudget;
+    for (int z = z0; z <= z1; ++z) {
+        for (int y = y0; y <= y1; ++y) {
+            const int base = z * g.nxny + y * g.nx;
+            const uint32_t s = g.prefix[base + x0];
+            uint32_t e = g.prefix[base + x1 + 1];
+            if (e - s > budget)
+                e = s + budget;
+            budget -= e - s;
+            for (uint32_t q = s; q < e; ++q) {
+                const uint32_t j = IND ? g.sorted[q] : q;
+                const f3 xij = xi - f3 { xs[j], ys[j], zs[j] };
+                const float d2 = dot(xij, xij);
+                if (d2 <= g.r2)
+                    f(j, xij, d2);
+            }
+        }
+    }
+}
+
+// ---- particle views ----------------------------------------------------------
+
+struct FluidDev {
+    float *x, *y, *z, *vx, *vy, *vz, *m;
+    uint32_t n;
+};
+inline FluidDev to_dev(const reyn_particles& p)
+{
+    return { p.xx, p.xy, p.xz, p.vx, p.vy, p.vz, p.m, p.n };
+}
+
+struct BdyDev {
+    const float *x, *y, *z, *m;
+    uint32_t n;
+    GridDev grid;
+    int present;
+};
+inline BdyDev to_dev(const reyn_boundary* b)
+{
+    BdyDev d {};
+    if (b && b->n > 0) {
+        d.x = b->xx;
+        d.y = b->xy;
+        d.z = b->xz;
+        d.m = b->m;
+        d.n = b->n;
+        d.grid = to_dev(b->grid);
+        d.present = 1;
+    }
+    return d;
+}
+
+// Thread -> particle map: in Resort::no mode threads follow the cell order
+// through `sorted`, so a wavefront's neighbourhoods overlap in cache.
+template <bool IND> __device__ __forceinline__ uint32_t particle_of(const GridDev& g, uint32_t t)
+{
+    return IND ? g.sorted[t] : t;
+}
+
+// Dispatch a generic launcher over the four kernels and the two Resort modes.
+template <class F> int with_kernel(reyn_ctx* ctx, int kind, bool ind, F&& f)
+{
+    switch (kind) {
+    case KC2:
+        return ind ? f(std::integral_constant<int, KC2> {}, std::true_type {})
+                   : f(std::integral_constant<int, KC2> {}, std::false_type {});
+    case KB3:
+        return ind ? f(std::integral_constant<int, KB3> {}, std::true_type {})
+                   : f(std::integral_constant<int, KB3> {}, std::false_type {});
+    case KW6:
+        return ind ? f(std::integral_constant<int, KW6> {}, std::true_type {})
+                   : f(std::integral_constant<int, KW6> {}, std::false_type {});
+    case KCOS:
+        return ind ? f(std::integral_constant<int, KCOS> {}, std::true_type {})
+                   : f(std::integral_constant<int, KCOS> {}, std::false_type {});
+    default:
+        return set_error(ctx, REYN_ERR_INVALID_ARGUMENT, "unknown kernel kind %d", kind);
+    }
+}
+
+// Validation helpers shared by the entry points.
+int check_grid(reyn_ctx* ctx, const reyn_grid* g, uint32_t n_expected, const char* what);
+int check_fluid(reyn_ctx* ctx, const reyn_particles* p);
+int check_bdy(reyn_ctx* ctx, const reyn_boundary* b);
+int check_sph(reyn_ctx* ctx, const reyn_sph_params* prm);
+
+} // namespace reyn

@@ -1,0 +1,146 @@
+"""Seeded synthetic scenes for the SPH step: fluid lattices, wall shells and
+a sphere obstacle, generated on the host in numpy.
+
+The reference builds scenes from OBJ meshes (src/scene/scene.cu:316-464);
+mesh scenes are outside the per-step path. These generators produce the
+lattice scenes the workloads are defined on: a lattice of spacing s with
+points at (i + 1/2) s, an optional uniform jitter, masses rho_0 s^3
+(scene.cu:14-37), and boundary samples at the same spacing.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+
+def lattice(counts, spacing: float, origin=(0.0, 0.0, 0.0), jitter: float = 0.0, seed: int = 0) -> np.ndarray:
+    """Points origin + (i + 1/2) s for i in [0, counts) per axis, x fastest,
+    each coordinate moved by U(-jitter s, +jitter s)."""
+    nx, ny, nz = (int(c) for c in counts)
+    s = float(spacing)
+    ix = np.arange(nx, dtype=np.float64)
+    iy = np.arange(ny, dtype=np.float64)
+    iz = np.arange(nz, dtype=np.float64)
+    z, y, x = np.meshgrid(iz, iy, ix, indexing="ij")
+    pts = np.stack([x.ravel(), y.ravel(), z.ravel()], axis=1)
+    pts = (pts + 0.5) * s + np.asarray(origin, dtype=np.float64)
+    if jitter:
+        rng = np.random.default_rng(seed)
+        pts += rng.uniform(-jitter * s, jitter * s, size=pts.shape)
+    return pts.astype(np.float32)
+
+
+def box_walls(lo, hi, spacing: float, layers: int = 2) -> np.ndarray:
+    """Lattice points of spacing s in the shell of `layers` layers around the
+    box [lo, hi]: (i + 1/2) s offsets from lo - layers s, keeping the points
+    outside the box in at least one axis."""
+    lo = np.asarray(lo, dtype=np.float64)
+    hi = np.asarray(hi, dtype=np.float64)
+    s = float(spacing)
+    start = lo - layers * s
+    counts = np.rint((hi - lo) / s).astype(int) + 2 * layers
+    axes = [start[a] + (np.arange(counts[a]) + 0.5) * s for a in range(3)]
+    z, y, x = np.meshgrid(axes[2], axes[1], axes[0], indexing="ij")
+    pts = np.stack([x.ravel(), y.ravel(), z.ravel()], axis=1)
+    inside = np.all((pts > lo) & (pts < hi), axis=1)
+    return pts[~inside].astype(np.float32)
+
+
+def sphere_shells(center, radius: float, spacing: float, layers: int = 2) -> np.ndarray:
+    """Points on `layers` concentric spheres of radii r - (k + 1/2) s, each a
+    Fibonacci lattice with about one point per s^2 of area."""
+    c = np.asarray(center, dtype=np.float64)
+    s = float(spacing)
+    out = []
+    golden = np.pi * (3.0 - np.sqrt(5.0))
+    for k in range(layers):
+        r = radius - (k + 0.5) * s
+        if r <= 0:
+            break
+        n = max(4, int(round(4.0 * np.pi * r * r / (s * s))))
+        i = np.arange(n, dtype=np.float64)
+        y = 1.0 - 2.0 * (i + 0.5) / n
+        rad = np.sqrt(np.maximum(0.0, 1.0 - y * y))
+        th = golden * i
+        out.append(c + r * np.stack([np.cos(th) * rad, y, np.sin(th) * rad], axis=1))
+    return np.concatenate(out).astype(np.float32) if out else np.zeros((0, 3), np.float32)
+
+
+@dataclass
+class Scene:
+    """A fluid block, its boundary samples and the scene bounds (the boundary
+    samples' AABB, which `hard_enforce_bounds` clamps to)."""
+
+    x: np.ndarray
+    xb: np.ndarray
+    spacing: float
+    rho_0: float
+    box_lo: np.ndarray
+    box_hi: np.ndarray
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def n(self) -> int:
+        return int(self.x.shape[0])
+
+    @property
+    def n_bdy(self) -> int:
+        return int(self.xb.shape[0])
+
+    @property
+    def mass(self) -> float:
+        return float(np.float32(self.rho_0) * np.float32(self.spacing) ** 3)
+
+    @property
+    def bound_min(self) -> np.ndarray:
+        return self.xb.min(axis=0) if self.n_bdy else self.box_lo.astype(np.float32)
+
+    @property
+    def bound_max(self) -> np.ndarray:
+        return self.xb.max(axis=0) if self.n_bdy else self.box_hi.astype(np.float32)
+
+
+def dam_break(fluid_counts, spacing: float, box_hi, rho_0: float = 1000.0, jitter: float = 0.01, seed: int = 1,
+              sphere: tuple | None = None, layers: int = 2) -> Scene:
+    """Fluid block of fluid_counts lattice points in the corner (0,0,0) of the
+    box [0, box_hi], walls of `layers` layers, optional sphere obstacle
+    (center, radius)."""
+    x = lattice(fluid_counts, spacing, (0.0, 0.0, 0.0), jitter, seed)
+    lo = np.zeros(3)
+    hi = np.asarray(box_hi, dtype=np.float64)
+    xb = box_walls(lo, hi, spacing, layers)
+    meta = {"fluid_counts": tuple(int(c) for c in fluid_counts), "box": tuple(float(v) for v in hi),
+            "jitter": jitter, "seed": seed}
+    if sphere is not None:
+        center, radius = sphere
+        sb = sphere_shells(center, radius, spacing, layers)
+        # fluid points inside the obstacle are removed
+        d = np.linalg.norm(x.astype(np.float64) - np.asarray(center), axis=1)
+        x = x[d > radius + 0.5 * spacing]
+        xb = np.concatenate([xb, sb])
+        meta["sphere"] = (tuple(float(v) for v in center), float(radius))
+    return Scene(x, xb, float(spacing), float(rho_0), lo, hi, meta)
+
+
+def config1_sesph() -> Scene:
+    """32^3 fluid at s = 0.02 in the corner of [0,1]^3, jitter +-1%, seed 1."""
+    return dam_break((32, 32, 32), 0.02, (1.0, 1.0, 1.0), jitter=0.01, seed=1)
+
+
+def config2_grid(side: int = 100) -> Scene:
+    """side^3 lattice at s = 0.02 filling [0,2]^3, jitter +-5%, seed 2."""
+    s = 2.0 / side
+    return dam_break((side, side, side), s, (2.0, 2.0, 2.0), jitter=0.05, seed=2)
+
+
+def config3_iisph(side: int = 128, seed: int = 3) -> Scene:
+    """side^3 fluid block [0,1]^3 at s = 1/side in the box [0,2]x[0,1.25]x[0,2]
+    with 2 wall layers and a sphere obstacle at (1.5, 0.25, 1.0) of radius
+    U[0.1, 0.25] (seeded)."""
+    s = 1.0 / side
+    rng = np.random.default_rng(seed)
+    radius = float(rng.uniform(0.1, 0.25))
+    return dam_break((side, side, side), s, (2.0, 1.25, 2.0), jitter=0.01, seed=seed,
+                     sphere=((1.5, 0.25, 1.0), radius))

@@ -1,0 +1,341 @@
+"""Python mirror of reyn's host API for the SPH step, over the C-ABI.
+
+The names and call shapes follow the reference's C++ classes so that tests
+and drivers read like the reference's own code:
+
+* ``Particles``            src/particles.cuh:25-75
+* ``UniformGridBuilder``   src/datastructure/uniformgrid.cuh:200-489
+* ``BoundarySamples``      src/scene/scene.cuh:29-46
+* ``SESPH``/``PCISPH``/``IISPH``  src/solvers/{SESPH,PCISPH,IISPH}.cuh
+* ``cfl_time_step``/``simple_dt_controller``  src/timestep/cfl.cuh:13-32
+* ``Scene.hard_enforce_bounds``/``Scene.get_grid``  src/scene/scene.cuh:136-156
+
+Every call goes to hand-written HIP in ``libreyn_amd.so``; nothing here
+computes on the CPU beyond argument marshalling.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import capi
+from .device import Context, DeviceArray
+
+GRAVITY = (0.0, -9.81, 0.0)
+
+
+class Particles:
+    """SoA fluid state: xx, xy, xz, vx, vy, vz, m (+ the reference's rho_0)."""
+
+    FIELDS = ("xx", "xy", "xz", "vx", "vy", "vz", "m")
+
+    def __init__(self, ctx: Context, x: np.ndarray, v: np.ndarray | None, m, rho_0: float):
+        x = np.asarray(x, dtype=np.float32)
+        n = x.shape[0]
+        v = np.zeros((n, 3), np.float32) if v is None else np.asarray(v, dtype=np.float32)
+        m = np.broadcast_to(np.asarray(m, dtype=np.float32), (n,))
+        self.ctx = ctx
+        self.n = n
+        self.rho_0 = float(rho_0)
+        cols = [x[:, 0], x[:, 1], x[:, 2], v[:, 0], v[:, 1], v[:, 2], m]
+        for name, col in zip(self.FIELDS, cols):
+            setattr(self, name, ctx.array(np.ascontiguousarray(col)))
+
+    def view(self) -> capi.Particles:
+        return capi.Particles(*(getattr(self, f).ptr for f in self.FIELDS), self.n)
+
+    def positions(self) -> np.ndarray:
+        return np.stack([self.xx.numpy(), self.xy.numpy(), self.xz.numpy()], axis=1)
+
+    def velocities(self) -> np.ndarray:
+        return np.stack([self.vx.numpy(), self.vy.numpy(), self.vz.numpy()], axis=1)
+
+    def host(self) -> dict:
+        return {f: getattr(self, f).numpy() for f in self.FIELDS}
+
+
+class UniformGridBuilder:
+    """Owns counts/prefix/sorted; ``construct`` and ``construct_and_reorder``
+    return a ``capi.Grid`` view (uniformgrid.cu:65-181)."""
+
+    def __init__(self, ctx: Context, bound_min, bound_max, search_radius: float, cell_ratio: float = 0.5):
+        self.ctx = ctx
+        h = C.c_void_p()
+        ctx.check(
+            ctx._lib.reyn_grid_builder_create(
+                ctx.h, capi.f3(bound_min), capi.f3(bound_max), float(search_radius), float(cell_ratio), C.byref(h)
+            ),
+            "UniformGridBuilder",
+        )
+        self.h = h
+        self.search_radius = float(search_radius)
+        self._tmp: dict[int, list[DeviceArray]] = {}
+
+    def geometry(self) -> capi.Grid:
+        g = capi.Grid()
+        self.ctx.check(self.ctx._lib.reyn_grid_builder_geometry(self.h, C.byref(g)))
+        return g
+
+    def construct(self, search_radius: float, xx: DeviceArray, xy: DeviceArray, xz: DeviceArray) -> capi.Grid:
+        g = capi.Grid()
+        self.ctx.check(
+            self.ctx._lib.reyn_grid_construct(
+                self.ctx.h, self.h, float(search_radius), xx._ptr, xy._ptr, xz._ptr, xx.n, C.byref(g)
+            ),
+            "construct",
+        )
+        return g
+
+    def construct_and_reorder(self, search_radius: float, fields) -> capi.Grid:
+        """Reorders every ``(owner, name)`` array in ``fields`` (x, y, z first)
+        and swaps the reordered arrays in, as the reference swaps storage with
+        its tmp buffer (src/buffer.cuh:184-209, src/main.cu:68)."""
+        arrays = [getattr(o, nm) for o, nm in fields]
+        n = arrays[0].n
+        tmps = self._tmp.setdefault(n, [])
+        while len(tmps) < len(arrays):
+            tmps.append(self.ctx.empty(n))
+        fptr = (C.c_void_p * len(arrays))(*[a.ptr for a in arrays])
+        tptr = (C.c_void_p * len(arrays))(*[t.ptr for t in tmps[: len(arrays)]])
+        g = capi.Grid()
+        rc = self.ctx._lib.reyn_grid_construct_and_reorder(
+            self.ctx.h, self.h, float(search_radius), fptr, tptr, len(arrays), n, C.byref(g)
+        )
+        if rc in (capi.OK, capi.ERR_OUT_OF_BOUNDS, capi.ERR_CAPACITY):
+            # the gather ran: swap storage with the tmp arrays
+            for k, (o, nm) in enumerate(fields):
+                setattr(o, nm, tmps[k])
+                tmps[k] = arrays[k]
+        self.ctx.check(rc, "construct_and_reorder")
+        return g
+
+    def debug_tables(self, n: int) -> tuple[np.ndarray, np.ndarray]:
+        counts = C.c_void_p()
+        keys = C.c_void_p()
+        self.ctx.check(self.ctx._lib.reyn_grid_debug_tables(self.h, C.byref(counts), C.byref(keys)))
+        g = self.geometry()
+        c = np.empty(g.ncells, np.uint32)
+        k = np.empty(n, np.uint32)
+        self.ctx.check(self.ctx._lib.reyn_memcpy(self.ctx.h, c.ctypes.data_as(C.c_void_p), counts, c.nbytes, 1))
+        if n:
+            self.ctx.check(self.ctx._lib.reyn_memcpy(self.ctx.h, k.ctypes.data_as(C.c_void_p), keys, k.nbytes, 1))
+        return c, k
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.ctx._lib.reyn_grid_builder_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def grid_tables(ctx: Context, g: capi.Grid) -> tuple[np.ndarray, np.ndarray | None]:
+    """Host copies of a grid's prefix (C+1) and sorted (n) tables."""
+    prefix = np.empty(g.ncells + 1, np.uint32)
+    ctx.check(ctx._lib.reyn_memcpy(ctx.h, prefix.ctypes.data_as(C.c_void_p), g.prefix, prefix.nbytes, 1))
+    sorted_ = None
+    if g.sorted:
+        sorted_ = np.empty(g.n, np.uint32)
+        if g.n:
+            ctx.check(ctx._lib.reyn_memcpy(ctx.h, sorted_.ctypes.data_as(C.c_void_p), g.sorted, sorted_.nbytes, 1))
+    return prefix, sorted_
+
+
+class BoundarySamples:
+    """Boundary samples with Akinci masses and their own cell-ordered grid
+    (scene.cuh:29-46; masses: sample_boundary.cu:251-297)."""
+
+    def __init__(self, ctx: Context, xb: np.ndarray, h: float, rho_0: float, cell_ratio: float = 0.5,
+                 mass_iterations: int = 5):
+        xb = np.asarray(xb, dtype=np.float32)
+        self.ctx = ctx
+        self.n = xb.shape[0]
+        self.xs = ctx.array(np.ascontiguousarray(xb[:, 0]))
+        self.ys = ctx.array(np.ascontiguousarray(xb[:, 1]))
+        self.zs = ctx.array(np.ascontiguousarray(xb[:, 2]))
+        self.m = ctx.full(self.n, 1.0)
+        self.bound_min = xb.min(axis=0).astype(np.float32)
+        self.bound_max = xb.max(axis=0).astype(np.float32)
+        self.builder = UniformGridBuilder(ctx, self.bound_min, self.bound_max, 2.0 * h, cell_ratio)
+        # the boundary grid is built once and the samples are stored in cell order
+        self.grid = self.builder.construct_and_reorder(2.0 * h, [(self, "xs"), (self, "ys"), (self, "zs")])
+        view = self.view()
+        ctx.check(ctx._lib.reyn_boundary_masses(ctx.h, C.byref(view), self.m._ptr, float(h), float(rho_0),
+                                                int(mass_iterations)), "boundary masses")
+
+    def view(self) -> capi.Boundary:
+        return capi.Boundary(self.xs.ptr, self.ys.ptr, self.zs.ptr, self.m.ptr, self.n, self.grid)
+
+    def positions(self) -> np.ndarray:
+        return np.stack([self.xs.numpy(), self.ys.numpy(), self.zs.numpy()], axis=1)
+
+
+def sph_params(kernel: str | int, h: float, nu: float, rho_0: float, g=GRAVITY, support: float = 2.0) -> capi.SphParams:
+    k = capi.KERNELS[kernel] if isinstance(kernel, str) else int(kernel)
+    return capi.SphParams(k, float(support * h), float(h), float(nu), float(rho_0), capi.f3(g))
+
+
+class _Solver:
+    def __init__(self, ctx: Context, n: int):
+        self.ctx = ctx
+        self.n = n
+        self.rho = ctx.full(n, 0.0)
+        self.ax = ctx.full(n, 0.0)
+        self.ay = ctx.full(n, 0.0)
+        self.az = ctx.full(n, 0.0)
+
+    @staticmethod
+    def _bdy(bdy):
+        if bdy is None:
+            return None
+        return C.byref(bdy.view() if hasattr(bdy, "view") else bdy)
+
+
+class SESPH(_Solver):
+    """src/solvers/SESPH.cuh:11-70 (k is a parameter; the reference fixes 500)."""
+
+    def __init__(self, ctx, kernel, n, nu, h, rho_0, k=500.0, g=GRAVITY):
+        super().__init__(ctx, n)
+        self.prm = capi.SesphParams(sph_params(kernel, h, nu, rho_0, g), float(k))
+
+    def step(self, state: Particles, grid: capi.Grid, bdy, dt: float) -> None:
+        work = capi.SesphWork(self.rho.ptr, self.ax.ptr, self.ay.ptr, self.az.ptr)
+        view = state.view()
+        self.ctx.check(self.ctx._lib.reyn_sesph_step(self.ctx.h, C.byref(self.prm), C.byref(view), C.byref(grid),
+                                                     self._bdy(bdy), C.byref(work), float(dt)), "SESPH.step")
+
+
+class PCISPH(_Solver):
+    """src/solvers/PCISPH.cuh:8-111."""
+
+    def __init__(self, ctx, kernel, n, nu, h, rho_0, g=GRAVITY, eta_rho_max=0.001, min_iter=3, max_iter=0):
+        super().__init__(ctx, n)
+        sph = sph_params(kernel, h, nu, rho_0, g)
+        delta = C.c_double()
+        capi.lib().reyn_pcisph_delta(sph.kernel, sph.h_bar, sph.h, C.byref(delta))
+        self.prm = capi.PcisphParams(sph, delta.value, float(eta_rho_max), int(min_iter), int(max_iter))
+        self.info = capi.SolveInfo()
+
+    def step(self, state: Particles, grid: capi.Grid, bdy, dt: float) -> int:
+        work = capi.PcisphWork(self.rho.ptr, self.ax.ptr, self.ay.ptr, self.az.ptr)
+        view = state.view()
+        self.ctx.check(self.ctx._lib.reyn_pcisph_step(self.ctx.h, C.byref(self.prm), C.byref(view), C.byref(grid),
+                                                      self._bdy(bdy), C.byref(work), float(dt), C.byref(self.info)),
+                       "PCISPH.step")
+        return int(self.info.iterations)
+
+
+class IISPH(_Solver):
+    """src/solvers/IISPH.cuh:8-93; ``stop_rule`` REYN_IISPH_STOP_ANY is the
+    reference's rule, STOP_MEAN the mean-error rule."""
+
+    def __init__(self, ctx, kernel, n, nu, h, rho_0, g=GRAVITY, omega=0.5, eta_rho_max=0.005, min_iter=3,
+                 max_iter=0, stop_rule=capi.IISPH_STOP_ANY, warm_start=True, check_every=4):
+        super().__init__(ctx, n)
+        self.p = ctx.full(n, 0.0)
+        self.a_ii = ctx.full(n, 0.0)
+        self.s_i = ctx.full(n, 0.0)
+        self.prm = capi.IisphParams(sph_params(kernel, h, nu, rho_0, g), float(omega), float(eta_rho_max),
+                                    int(min_iter), int(max_iter), int(stop_rule), int(bool(warm_start)),
+                                    int(check_every))
+        self.info = capi.SolveInfo()
+
+    def step(self, state: Particles, grid: capi.Grid, bdy, dt: float) -> int:
+        work = capi.IisphWork(self.rho.ptr, self.p.ptr, self.a_ii.ptr, self.s_i.ptr, self.ax.ptr, self.ay.ptr,
+                              self.az.ptr)
+        view = state.view()
+        self.ctx.check(self.ctx._lib.reyn_iisph_step(self.ctx.h, C.byref(self.prm), C.byref(view), C.byref(grid),
+                                                     self._bdy(bdy), C.byref(work), float(dt), C.byref(self.info)),
+                       "IISPH.step")
+        return int(self.info.iterations)
+
+
+def cfl_time_step(ctx: Context, lam: float, h: float, state: Particles, g=GRAVITY) -> tuple[float, float]:
+    dt = C.c_float()
+    vmax = C.c_double()
+    view = state.view()
+    ctx.check(ctx._lib.reyn_cfl_time_step(ctx.h, float(lam), float(h), C.byref(view), capi.f3(g), C.byref(dt),
+                                          C.byref(vmax)), "cfl_time_step")
+    return float(dt.value), float(vmax.value)
+
+
+def simple_dt_controller(ctx, iter_last, dt_last, lam, h, state, g=GRAVITY, iter_target=5, change_factor=0.01):
+    dt = C.c_float()
+    view = state.view()
+    ctx.check(ctx._lib.reyn_simple_dt_controller(ctx.h, int(iter_last), float(dt_last), float(lam), float(h),
+                                                 C.byref(view), capi.f3(g), int(iter_target), float(change_factor),
+                                                 C.byref(dt)), "simple_dt_controller")
+    return float(dt.value)
+
+
+def hard_enforce_bounds(ctx: Context, bound_min, bound_max, state: Particles) -> None:
+    view = state.view()
+    ctx.check(ctx._lib.reyn_hard_enforce_bounds(ctx.h, capi.f3(bound_min), capi.f3(bound_max), C.byref(view)),
+              "hard_enforce_bounds")
+
+
+def reduce(ctx: Context, a: DeviceArray, op: str) -> float:
+    ops = {"sum": 0, "min": 1, "max": 2, "avg": 3}
+    out = C.c_double()
+    ctx.check(ctx._lib.reyn_reduce_f32(ctx.h, a._ptr, a.n, ops[op], C.byref(out)), "reduce")
+    return float(out.value)
+
+
+@dataclass
+class StepStats:
+    dt: float
+    iterations: int
+    mean_rel_error: float
+    n_violating: int
+
+
+class Simulation:
+    """The reference's per-step loop (src/main.cu:60-78): time step, grid
+    build + reorder (every ``reorder_every`` steps, ``Resort::no`` between),
+    solver step, bounds."""
+
+    def __init__(self, ctx: Context, solver, state: Particles, bdy: BoundarySamples | None, h: float,
+                 bound_min, bound_max, cfl_lambda: float = 0.4, cell_ratio: float = 0.5, reorder_every: int = 1,
+                 extra_reorder: tuple[str, ...] = ()):
+        self.ctx = ctx
+        self.solver = solver
+        self.state = state
+        self.bdy = bdy
+        self.h = float(h)
+        self.bound_min = np.asarray(bound_min, np.float32)
+        self.bound_max = np.asarray(bound_max, np.float32)
+        self.cfl_lambda = cfl_lambda
+        self.builder = UniformGridBuilder(ctx, self.bound_min, self.bound_max, 2.0 * h, cell_ratio)
+        self.reorder_every = max(1, int(reorder_every))
+        self.extra_reorder = extra_reorder
+        self.steps = 0
+        self.time = 0.0
+
+    def get_grid(self) -> capi.Grid:
+        """Scene::get_grid (src/scene/scene.cuh:149-156): build the grid and,
+        on reorder steps, reorder the state plus ``extra_reorder`` solver
+        fields (the IISPH warm-start pressure, src/main.cu:68)."""
+        r = 2.0 * self.h
+        if self.steps % self.reorder_every == 0:
+            fields = [(self.state, f) for f in Particles.FIELDS]
+            fields += [(self.solver, nm) for nm in self.extra_reorder]
+            return self.builder.construct_and_reorder(r, fields)
+        return self.builder.construct(r, self.state.xx, self.state.xy, self.state.xz)
+
+    def step(self) -> StepStats:
+        dt, _ = cfl_time_step(self.ctx, self.cfl_lambda, self.h, self.state)
+        grid = self.get_grid()
+        iters = self.solver.step(self.state, grid, self.bdy, dt) or 0
+        hard_enforce_bounds(self.ctx, self.bound_min, self.bound_max, self.state)
+        self.steps += 1
+        self.time += dt
+        info = getattr(self.solver, "info", None)
+        return StepStats(dt, int(iters), float(info.mean_rel_error) if info else 0.0,
+                         int(info.n_violating) if info else 0)

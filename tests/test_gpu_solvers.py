@@ -1,0 +1,168 @@
+"""GPU parity of the SPH phases and solver steps against the FP64 oracle.
+
+One step from identical FP32 inputs (the GPU's reordered state, boundary
+masses and iteration count) on the GPU and in the oracle. Bar, as
+max|gpu - ref| / max|ref| per field: densities, pressures and positions
+<= 1e-5; velocities, a_ii and s_i <= 1e-4 (they are differences of nearly
+cancelling sums, whose FP32 rounding the scale of the largest value does not
+absorb); displacements <= 1e-4.
+"""
+
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oracle import binding as orc
+from oracle import parity
+from reyn_amd import capi, scenes
+from reyn_amd.sim import (IISPH, BoundarySamples, Particles, Simulation, cfl_time_step, hard_enforce_bounds,
+                          reduce)
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+TOL_DIFF = 1e-4
+
+
+@pytest.mark.parametrize("kernel", ["C2", "B3", "W6", "COS"])
+def test_density_parity(ctx, kernel):
+    sc, v = parity.make_case(side=12, seed=3)
+    case = parity.GpuCase(ctx, sc, v)
+    case.grid()
+    fg, st, ob = case.oracle_inputs()
+    prm = parity.sph_oracle(capi.KERNELS[kernel], sc.spacing, sc.rho_0)
+    rho = ctx.empty(sc.n)
+    sp = capi.SphParams(capi.KERNELS[kernel], 2 * sc.spacing, sc.spacing, parity.NU, sc.rho_0, capi.f3((0, -9.81, 0)))
+    view, bview = case.state.view(), case.bdy.view()
+    ctx.check(ctx._lib.reyn_density(ctx.h, C.byref(sp), C.byref(view), C.byref(case.g), C.byref(bview), rho._ptr))
+    ref = orc.density(fg, st, ob, prm.kernel, prm.h_bar)
+    assert parity.rel_err(rho.numpy(), ref) <= TOL
+
+
+def test_boundary_masses_parity(ctx):
+    sc, _ = parity.make_case(side=4)
+    bdy = BoundarySamples(ctx, sc.xb, sc.spacing, sc.rho_0)
+    xb = bdy.positions()
+    geom = orc.geometry(bdy.bound_min, bdy.bound_max, 2 * sc.spacing, 0.5)
+    g = orc.grid_build(geom, xb[:, 0], xb[:, 1], xb[:, 2]).resorted()
+    ref = orc.boundary_masses(g, xb[:, 0], xb[:, 1], xb[:, 2], sc.spacing, sc.rho_0, 5)
+    assert parity.rel_err(bdy.m.numpy(), ref) <= TOL
+
+
+def test_nonpressure_and_predict_parity(ctx):
+    sc, v = parity.make_case(side=12, seed=5)
+    case = parity.GpuCase(ctx, sc, v)
+    case.grid()
+    fg, st, ob = case.oracle_inputs()
+    prm = parity.sph_oracle(capi.KERNEL_B3, sc.spacing, sc.rho_0)
+    sp = capi.SphParams(capi.KERNEL_B3, 2 * sc.spacing, sc.spacing, parity.NU, sc.rho_0, capi.f3((0, -9.81, 0)))
+    view, bview = case.state.view(), case.bdy.view()
+    rho = ctx.empty(sc.n)
+    ctx.check(ctx._lib.reyn_density(ctx.h, C.byref(sp), C.byref(view), C.byref(case.g), C.byref(bview), rho._ptr))
+    a = [ctx.empty(sc.n) for _ in range(3)]
+    ctx.check(ctx._lib.reyn_nonpressure_acc(ctx.h, C.byref(sp), C.byref(view), C.byref(case.g), rho._ptr,
+                                            a[0]._ptr, a[1]._ptr, a[2]._ptr))
+    ref_rho = orc.density(fg, st, ob, prm.kernel, prm.h_bar)
+    ref_a = orc.nonpressure(fg, st, prm, ref_rho)
+    got = np.stack([x.numpy() for x in a], 1)
+    assert parity.rel_err(got, ref_a) <= TOL_DIFF
+
+
+def test_sesph_step_parity(ctx):
+    errs, gpu, ref = parity.run_sesph_case(ctx)
+    assert errs["rho"] <= TOL and errs["x"] <= TOL, errs
+    assert errs["a"] <= TOL_DIFF and errs["v"] <= TOL_DIFF and errs["dx"] <= TOL_DIFF, errs
+
+
+def test_pcisph_step_parity(ctx):
+    errs, gpu, ref = parity.run_pcisph_case(ctx)
+    assert gpu["iterations"] >= 3
+    assert errs["rho"] <= TOL and errs["x"] <= TOL and errs["avg"] <= 1e-6, errs
+    assert errs["a"] <= TOL_DIFF and errs["v"] <= TOL_DIFF and errs["dx"] <= TOL_DIFF, errs
+
+
+@pytest.mark.parametrize("kernel", ["B3", "C2"])
+@pytest.mark.parametrize("warm", [True, False])
+def test_iisph_step_parity(ctx, kernel, warm):
+    errs, gpu, ref = parity.run_iisph_case(ctx, kernel=kernel, warm=warm)
+    assert gpu["iterations"] >= 3
+    assert errs["rho"] <= TOL and errs["p"] <= TOL and errs["x"] <= TOL, errs
+    assert errs["a_ii"] <= TOL_DIFF and errs["s_i"] <= TOL_DIFF and errs["v"] <= TOL_DIFF, errs
+    assert errs["dx"] <= TOL_DIFF, errs
+
+
+def test_iisph_stop_rules(ctx):
+    # the reference rule stops once no particle exceeds eta rho_0 (after >= 3)
+    errs, gpu, ref = parity.run_iisph_case(ctx, stop_rule=capi.IISPH_STOP_ANY, eta=0.005)
+    assert gpu["n_violating"] == 0
+    # the mean rule with a cap: either converged or stopped at the cap
+    errs, gpu, ref = parity.run_iisph_case(ctx, stop_rule=capi.IISPH_STOP_MEAN, eta=0.001, max_iter=100)
+    assert gpu["iterations"] <= 100
+    assert gpu["iterations"] == 100 or gpu["mean_rel_error"] <= 0.001
+    assert abs(gpu["mean_rel_error"] - ref["mean_rel_error"]) <= 1e-3 * max(ref["mean_rel_error"], 1e-6) + 1e-9
+
+
+def test_iisph_isolated_particle(ctx):
+    """A single particle far from everything: a_ii = 0, so the Jacobi update is
+    0 (IISPH.cu:202-211) and the step is a free fall v = g dt."""
+    x = np.array([[0.5, 0.5, 0.5]], np.float32)
+    st = Particles(ctx, x, None, 0.125, 1000.0)
+    from reyn_amd.sim import UniformGridBuilder
+    b = UniformGridBuilder(ctx, (0, 0, 0), (1, 1, 1), 0.1, 0.5)
+    g = b.construct_and_reorder(0.1, [(st, f) for f in Particles.FIELDS])
+    s = IISPH(ctx, "B3", 1, 0.001, 0.05, 1000.0)
+    it = s.step(st, g, None, 0.01)
+    assert it == 3
+    assert s.p.numpy()[0] == 0.0
+    np.testing.assert_allclose(st.velocities()[0], [0.0, -9.81 * 0.01, 0.0], rtol=1e-6)
+
+
+def test_cfl_and_bounds(ctx):
+    rng = np.random.default_rng(1)
+    x = rng.uniform(-0.2, 1.2, size=(5000, 3)).astype(np.float32)
+    v = rng.normal(0, 2.0, size=(5000, 3)).astype(np.float32)
+    st = Particles(ctx, x, v, 1.0, 1000.0)
+    dt, vmax = cfl_time_step(ctx, 0.4, 0.02, st)
+    rdt, rvmax = orc.cfl_time_step(0.4, 0.02, v)
+    assert vmax == rvmax and dt == rdt
+    hard_enforce_bounds(ctx, (0, 0, 0), (1, 1, 1), st)
+    rx, rv = orc.hard_enforce_bounds((0, 0, 0), (1, 1, 1), x, v)
+    np.testing.assert_array_equal(st.positions(), rx)
+    np.testing.assert_array_equal(st.velocities(), rv)
+    # at rest the CFL step is the gravity bound sqrt(2 lambda h / |g|)
+    st0 = Particles(ctx, x, None, 1.0, 1000.0)
+    dt0, vm0 = cfl_time_step(ctx, 0.4, 0.02, st0)
+    assert vm0 == 0.0 and dt0 == pytest.approx(np.sqrt(2 * 0.4 * 0.02 / 9.81), rel=1e-6)
+
+
+def test_reductions(ctx):
+    a = np.random.default_rng(2).normal(3.0, 1.0, 100_001).astype(np.float32)
+    d = ctx.array(a)
+    assert reduce(ctx, d, "sum") == pytest.approx(float(np.sum(a, dtype=np.float64)), rel=1e-12)
+    assert reduce(ctx, d, "min") == float(a.min())
+    assert reduce(ctx, d, "max") == float(a.max())
+    assert reduce(ctx, d, "avg") == pytest.approx(float(np.mean(a, dtype=np.float64)), rel=1e-12)
+
+
+def test_iisph_dam_break_rollout(ctx):
+    """A small IISPH dam break with the mean-error rule: every step converges to
+    <= 0.1% mean compression (or reports the cap), the state stays finite and
+    inside the bounds, fluid mass is unchanged."""
+    sc = scenes.dam_break((16, 16, 16), 0.025, (1.0, 0.8, 0.6), jitter=0.01, seed=3)
+    st = Particles(ctx, sc.x, None, sc.mass, sc.rho_0)
+    bdy = BoundarySamples(ctx, sc.xb, sc.spacing, sc.rho_0)
+    solver = IISPH(ctx, "B3", sc.n, 0.001, sc.spacing, sc.rho_0, eta_rho_max=0.001, max_iter=100,
+                   stop_rule=capi.IISPH_STOP_MEAN)
+    sim = Simulation(ctx, solver, st, bdy, sc.spacing, sc.bound_min, sc.bound_max, cfl_lambda=0.4,
+                     extra_reorder=("p",))
+    m0 = reduce(ctx, st.m, "sum")
+    for _ in range(40):
+        s = sim.step()
+        assert s.iterations == 100 or s.mean_rel_error <= 1e-3
+    x = st.positions()
+    assert np.all(np.isfinite(x)) and np.all(np.isfinite(st.velocities()))
+    assert np.all(x >= sc.bound_min - 1e-6) and np.all(x <= sc.bound_max + 1e-6)
+    assert reduce(ctx, st.m, "sum") == m0
+    rho_mean = reduce(ctx, solver.rho, "avg")
+    assert abs(rho_mean / sc.rho_0 - 1.0) < 0.05
