@@ -1,0 +1,623 @@
+// reyn C++ host API over the gfx950 C-ABI (include/reyn_amd.h).
+//
+// Header-only, C++20. It restores the shapes of the reference's host classes
+// so code written against reyn's API compiles against this one:
+//
+//   DeviceBuffer<T>        src/buffer.cuh:15-269
+//   Particles              src/particles.cuh:25-75
+//   Kernel<D>, C2/B3/W6/COS, IsKernel   src/kernels.cuh:19-174
+//   Resort, UniformGrid<R>, UniformGridBuilder   src/datastructure/uniformgrid.cuh
+//   Boundary, BoundarySamples, Scene bounds/grid  src/scene/scene.cuh:14-156
+//   SESPH<K,R>, PCISPH<K,R>, IISPH<K,R>  src/solvers/*.cuh
+//   cfl_time_step, simple_dt_controller  src/timestep/cfl.cuh:13-32
+//
+// Device errors and misuse throw std::runtime_error (the reference throws on
+// misuse and exits on CUDA errors, src/common.h:24-36; a library must not
+// exit). All work runs on the stream of one `Context` per host thread.
+#pragma once
+
+#include <algorithm>
+#include <concepts>
+#include <cstring>
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <utility>
+#include <vector>
+
+#include "../reyn_amd.h"
+
+namespace reyn {
+
+using float3 = reyn_float3;
+inline float3 v3(float x, float y, float z) { return float3 { x, y, z }; }
+inline float3 v3(float x) { return float3 { x, x, x }; }
+
+// ---- context ----------------------------------------------------------------
+
+class Context {
+public:
+    explicit Context(int device = 0)
+    {
+        const int rc = reyn_ctx_create(device, &ctx_);
+        if (rc != REYN_OK)
+            throw std::runtime_error("reyn: cannot create a context on HIP device " + std::to_string(device));
+    }
+    ~Context() { reyn_ctx_destroy(ctx_); }
+    Context(const Context&) = delete;
+    Context& operator=(const Context&) = delete;
+
+    reyn_ctx* get() const { return ctx_; }
+    void check(int rc, const char* what) const
+    {
+        if (rc != REYN_OK)
+            throw std::runtime_error(std::string("reyn: ") + what + ": " + reyn_last_error(ctx_) + " (code "
+                + std::to_string(rc) + ")");
+    }
+    void synchronize() const { check(reyn_ctx_synchronize(ctx_), "synchronize"); }
+
+    // the context used by objects constructed without an explicit one
+    static Context& current()
+    {
+        thread_local std::unique_ptr<Context> c;
+        if (!c)
+            c = std::make_unique<Context>(0);
+        return *c;
+    }
+
+private:
+    reyn_ctx* ctx_ = nullptr;
+};
+
+// ---- DeviceBuffer<T> (src/buffer.cuh:15-269) ------------------------------------
+
+template <typename T> class DeviceBuffer {
+    static_assert(std::is_trivially_copyable_v<T>, "DeviceBuffer holds trivially copyable types");
+
+public:
+    explicit DeviceBuffer(size_t n = 1, Context& ctx = Context::current())
+        : ctx_(&ctx)
+    {
+        alloc(n);
+    }
+    DeviceBuffer(size_t n, const T& value, Context& ctx = Context::current())
+        : ctx_(&ctx)
+    {
+        alloc(n);
+        fill(value);
+    }
+    ~DeviceBuffer() { release(); }
+    DeviceBuffer(const DeviceBuffer&) = delete;
+    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+    DeviceBuffer(DeviceBuffer&& o) noexcept
+        : ctx_(o.ctx_)
+        , ptr_(std::exchange(o.ptr_, nullptr))
+        , n_(std::exchange(o.n_, 0))
+    {
+    }
+    DeviceBuffer& operator=(DeviceBuffer&& o) noexcept
+    {
+        if (this != &o) {
+            release();
+            ctx_ = o.ctx_;
+            ptr_ = std::exchange(o.ptr_, nullptr);
+            n_ = std::exchange(o.n_, 0);
+        }
+        return *this;
+    }
+
+    T* ptr() const { return ptr_; }
+    size_t size() const { return n_; }
+    Context& context() const { return *ctx_; }
+
+    // contents are kept up to min(old, new) elements
+    void resize(size_t n)
+    {
+        if (n == n_)
+            return;
+        T* old = ptr_;
+        const size_t keep = std::min(n, n_);
+        ptr_ = nullptr;
+        alloc_raw(n);
+        if (old && keep)
+            ctx_->check(reyn_memcpy(ctx_->get(), ptr_, old, keep * sizeof(T), 2), "resize");
+        if (old)
+            reyn_free(ctx_->get(), old);
+        n_ = n;
+    }
+    void resize(size_t n, const T& value)
+    {
+        resize(n);
+        fill(value);
+    }
+
+    void fill(const T& value)
+    {
+        if (!n_)
+            return;
+        if constexpr (std::is_same_v<T, float>)
+            ctx_->check(reyn_fill_f32(ctx_->get(), ptr_, value, n_), "fill");
+        else if constexpr (sizeof(T) == 4) {
+            uint32_t bits;
+            std::memcpy(&bits, &value, 4);
+            ctx_->check(reyn_fill_u32(ctx_->get(), reinterpret_cast<uint32_t*>(ptr_), bits, n_), "fill");
+        } else {
+            std::vector<T> h(n_, value);
+            upload(h);
+        }
+    }
+    void upload(const std::vector<T>& h)
+    {
+        if (h.size() != n_)
+            resize(h.size());
+        if (n_)
+            ctx_->check(reyn_memcpy(ctx_->get(), ptr_, h.data(), n_ * sizeof(T), 0), "upload");
+    }
+    std::vector<T> download() const
+    {
+        std::vector<T> h(n_);
+        if (n_)
+            ctx_->check(reyn_memcpy(ctx_->get(), h.data(), ptr_, n_ * sizeof(T), 1), "download");
+        return h;
+    }
+
+    // gather by `sorted` into tmp, then swap storage (buffer.cuh:184-209)
+    void reorder(const DeviceBuffer<uint32_t>& sorted, DeviceBuffer<T>& tmp)
+        requires(sizeof(T) == 4)
+    {
+        if (sorted.size() != n_)
+            throw std::runtime_error("reyn: reorder map and buffer differ in size");
+        tmp.resize(n_);
+        const float* src[1] = { reinterpret_cast<const float*>(ptr_) };
+        float* dst[1] = { reinterpret_cast<float*>(tmp.ptr_) };
+        ctx_->check(reyn_gather_f32(ctx_->get(), sorted.ptr(), static_cast<uint32_t>(n_), src, dst, 1), "reorder");
+        std::swap(ptr_, tmp.ptr_);
+    }
+
+    // reductions (buffer.cuh:213-258), formed in FP64 on the device
+    double min() const requires std::same_as<T, float> { return reduce(1); }
+    double max() const requires std::same_as<T, float> { return reduce(2); }
+    double sum() const requires std::same_as<T, float> { return reduce(0); }
+    float avg() const requires std::same_as<T, float> { return static_cast<float>(reduce(3)); }
+
+    // swaps storage with another buffer of the same context
+    void swap(DeviceBuffer& o) noexcept
+    {
+        std::swap(ptr_, o.ptr_);
+        std::swap(n_, o.n_);
+    }
+
+private:
+    double reduce(int op) const
+    {
+        double out = 0.0;
+        ctx_->check(reyn_reduce_f32(ctx_->get(), ptr_, static_cast<uint32_t>(n_), op, &out), "reduce");
+        return out;
+    }
+    void alloc_raw(size_t n)
+    {
+        void* p = nullptr;
+        ctx_->check(reyn_malloc(ctx_->get(), &p, std::max<size_t>(n, 1) * sizeof(T)), "malloc");
+        ptr_ = static_cast<T*>(p);
+    }
+    void alloc(size_t n)
+    {
+        alloc_raw(n);
+        n_ = n;
+    }
+    void release()
+    {
+        if (ptr_)
+            reyn_free(ctx_->get(), ptr_);
+        ptr_ = nullptr;
+        n_ = 0;
+    }
+
+    Context* ctx_;
+    T* ptr_ = nullptr;
+    size_t n_ = 0;
+};
+
+// ---- Particles (src/particles.cuh:25-75) -------------------------------------------
+
+class Particles {
+public:
+    DeviceBuffer<float> xx, xy, xz, vx, vy, vz, m;
+    float rho_0;
+
+    Particles(size_t N, float rho_0_, Context& ctx = Context::current())
+        : xx(N, 0.f, ctx)
+        , xy(N, 0.f, ctx)
+        , xz(N, 0.f, ctx)
+        , vx(N, 0.f, ctx)
+        , vy(N, 0.f, ctx)
+        , vz(N, 0.f, ctx)
+        , m(N, 0.f, ctx)
+        , rho_0(rho_0_)
+    {
+    }
+    Particles(const Particles&) = delete;
+    Particles& operator=(const Particles&) = delete;
+
+    size_t size() const { return xx.size(); }
+    reyn_particles view() const
+    {
+        return reyn_particles { xx.ptr(), xy.ptr(), xz.ptr(), vx.ptr(), vy.ptr(), vz.ptr(), m.ptr(),
+            static_cast<uint32_t>(xx.size()) };
+    }
+    // reorder of the seven fields by `sorted` (particles.cu:93-113)
+    void reorder(const DeviceBuffer<uint32_t>& sorted, DeviceBuffer<float>& tmp)
+    {
+        for (DeviceBuffer<float>* b : { &vx, &vy, &vz, &m, &xx, &xy, &xz })
+            b->reorder(sorted, tmp);
+    }
+};
+
+// ---- smoothing kernels (src/kernels.cuh:19-174) -------------------------------------
+
+template <typename Derived> class Kernel {
+public:
+    float h_bar() const { return h_bar_; }
+    static constexpr int32_t kind() { return Derived::kKind; }
+
+protected:
+    explicit Kernel(float h_bar)
+        : h_bar_(h_bar)
+    {
+    }
+
+private:
+    float h_bar_;
+};
+
+template <typename T>
+concept IsKernel = std::is_base_of_v<Kernel<T>, T>;
+
+class C2 : public Kernel<C2> {
+public:
+    static constexpr int32_t kKind = REYN_KERNEL_C2;
+    explicit C2(float h_bar)
+        : Kernel(h_bar)
+    {
+    }
+};
+class B3 : public Kernel<B3> {
+public:
+    static constexpr int32_t kKind = REYN_KERNEL_B3;
+    explicit B3(float h_bar)
+        : Kernel(h_bar)
+    {
+    }
+};
+class W6 : public Kernel<W6> {
+public:
+    static constexpr int32_t kKind = REYN_KERNEL_W6;
+    explicit W6(float h_bar)
+        : Kernel(h_bar)
+    {
+    }
+};
+class COS : public Kernel<COS> {
+public:
+    static constexpr int32_t kKind = REYN_KERNEL_COS;
+    explicit COS(float h_bar)
+        : Kernel(h_bar)
+    {
+    }
+};
+
+#define FOREACH_KERNEL(X)                                                                                           \
+    X(C2)                                                                                                           \
+    X(B3)                                                                                                           \
+    X(W6)                                                                                                           \
+    X(COS)
+
+// ---- uniform grid (src/datastructure/uniformgrid.cuh) ------------------------------------
+
+enum class Resort : bool { no = false, yes = true };
+
+// Query-side POD; `grid.sorted` is NULL for Resort::yes.
+template <Resort R> struct UniformGrid {
+    reyn_grid grid;
+};
+
+template <typename T>
+concept IsFltDevBufPtr = std::is_same_v<T, DeviceBuffer<float>*>;
+
+class UniformGridBuilder {
+public:
+    // cell_ratio = cell size / search radius; the reference fixes 0.5
+    UniformGridBuilder(float3 bound_min, float3 bound_max, float search_radius, float cell_ratio = 0.5f,
+        Context& ctx = Context::current())
+        : ctx_(&ctx)
+    {
+        ctx_->check(reyn_grid_builder_create(ctx.get(), bound_min, bound_max, search_radius, cell_ratio, &b_),
+            "UniformGridBuilder");
+    }
+    ~UniformGridBuilder() { reyn_grid_builder_destroy(b_); }
+    UniformGridBuilder(const UniformGridBuilder&) = delete;
+    UniformGridBuilder& operator=(const UniformGridBuilder&) = delete;
+
+    UniformGrid<Resort::no> construct(float search_radius, const DeviceBuffer<float>& xx,
+        const DeviceBuffer<float>& xy, const DeviceBuffer<float>& xz)
+    {
+        UniformGrid<Resort::no> g {};
+        ctx_->check(reyn_grid_construct(ctx_->get(), b_, search_radius, xx.ptr(), xy.ptr(), xz.ptr(),
+                        static_cast<uint32_t>(xx.size()), &g.grid),
+            "construct");
+        return g;
+    }
+
+    // reorders x, y, z and any number of further buffers (uniformgrid.cuh:408-417)
+    template <IsFltDevBufPtr... MoreBufs>
+    UniformGrid<Resort::yes> construct_and_reorder(float search_radius, DeviceBuffer<float>& tmp,
+        DeviceBuffer<float>& xx, DeviceBuffer<float>& xy, DeviceBuffer<float>& xz, MoreBufs... reorder)
+    {
+        std::vector<DeviceBuffer<float>*> bufs { &xx, &xy, &xz, reorder... };
+        return reorder_all(search_radius, tmp, bufs);
+    }
+
+    // reorders the particle state plus further buffers (uniformgrid.cuh:475-483)
+    template <IsFltDevBufPtr... MoreBufs>
+    UniformGrid<Resort::yes> construct_and_reorder(float search_radius, DeviceBuffer<float>& tmp,
+        Particles& state, MoreBufs... reorder)
+    {
+        std::vector<DeviceBuffer<float>*> bufs { &state.xx, &state.xy, &state.xz, &state.vx, &state.vy,
+            &state.vz, &state.m, reorder... };
+        return reorder_all(search_radius, tmp, bufs);
+    }
+
+    reyn_grid geometry() const
+    {
+        reyn_grid g {};
+        reyn_grid_builder_geometry(b_, &g);
+        return g;
+    }
+
+private:
+    UniformGrid<Resort::yes> reorder_all(float r, DeviceBuffer<float>& tmp, std::vector<DeviceBuffer<float>*>& bufs)
+    {
+        const size_t n = bufs[0]->size();
+        for (auto* b : bufs)
+            if (b->size() != n)
+                throw std::runtime_error("Attempted to resort an array of grid construction the dimensions of "
+                                         "which don't match the number of particles in the sorting.");
+        // one scratch buffer per field so the whole reorder is a single gather;
+        // `tmp` keeps the reference's signature and serves as the first one
+        while (scratch_.size() + 1 < bufs.size())
+            scratch_.emplace_back(n, *ctx_);
+        std::vector<DeviceBuffer<float>*> tmps { &tmp };
+        for (size_t k = 0; k + 1 < bufs.size(); ++k)
+            tmps.push_back(&scratch_[k]);
+        std::vector<float*> f(bufs.size()), t(bufs.size());
+        for (size_t k = 0; k < bufs.size(); ++k) {
+            tmps[k]->resize(n);
+            f[k] = bufs[k]->ptr();
+            t[k] = tmps[k]->ptr();
+        }
+        UniformGrid<Resort::yes> g {};
+        const int rc = reyn_grid_construct_and_reorder(ctx_->get(), b_, r, f.data(), t.data(),
+            static_cast<uint32_t>(bufs.size()), static_cast<uint32_t>(n), &g.grid);
+        if (rc == REYN_OK || rc == REYN_ERR_OUT_OF_BOUNDS || rc == REYN_ERR_CAPACITY)
+            for (size_t k = 0; k < bufs.size(); ++k)
+                bufs[k]->swap(*tmps[k]);
+        ctx_->check(rc, "construct_and_reorder");
+        return g;
+    }
+
+    Context* ctx_;
+    reyn_grid_builder* b_ = nullptr;
+    std::vector<DeviceBuffer<float>> scratch_;
+};
+
+// ---- boundary (src/scene/scene.cuh:14-46; masses sample_boundary.cu:251-297) -------------
+
+struct Boundary {
+    reyn_boundary b;
+};
+
+class BoundarySamples {
+public:
+    DeviceBuffer<float> xs, ys, zs, m;
+    float3 bound_min, bound_max;
+
+    // samples in any order; they are stored in cell order with Akinci masses
+    BoundarySamples(const std::vector<float3>& pts, float h, float rho_0, unsigned mass_iterations = 5,
+        Context& ctx = Context::current())
+        : xs(pts.size(), ctx)
+        , ys(pts.size(), ctx)
+        , zs(pts.size(), ctx)
+        , m(pts.size(), 1.f, ctx)
+    {
+        if (pts.empty())
+            throw std::runtime_error("reyn: BoundarySamples needs at least one sample");
+        std::vector<float> x(pts.size()), y(pts.size()), z(pts.size());
+        bound_min = bound_max = pts[0];
+        for (size_t i = 0; i < pts.size(); ++i) {
+            x[i] = pts[i].x;
+            y[i] = pts[i].y;
+            z[i] = pts[i].z;
+            bound_min = v3(std::min(bound_min.x, x[i]), std::min(bound_min.y, y[i]), std::min(bound_min.z, z[i]));
+            bound_max = v3(std::max(bound_max.x, x[i]), std::max(bound_max.y, y[i]), std::max(bound_max.z, z[i]));
+        }
+        xs.upload(x);
+        ys.upload(y);
+        zs.upload(z);
+        builder_ = std::make_unique<UniformGridBuilder>(bound_min, bound_max, 2.f * h, 0.5f, ctx);
+        DeviceBuffer<float> tmp(pts.size(), ctx);
+        grid_ = builder_->construct_and_reorder(2.f * h, tmp, xs, ys, zs);
+        reyn_boundary v = get().b;
+        ctx.check(reyn_boundary_masses(ctx.get(), &v, m.ptr(), h, rho_0, mass_iterations), "boundary masses");
+    }
+
+    Boundary get() const
+    {
+        return Boundary { reyn_boundary { xs.ptr(), ys.ptr(), zs.ptr(), m.ptr(), static_cast<uint32_t>(xs.size()),
+            grid_.grid } };
+    }
+
+private:
+    std::unique_ptr<UniformGridBuilder> builder_;
+    UniformGrid<Resort::yes> grid_ {};
+};
+
+// Scene::hard_enforce_bounds (src/scene/scene.cu:466-491)
+inline void hard_enforce_bounds(float3 bound_min, float3 bound_max, Particles& state)
+{
+    Context& c = state.xx.context();
+    const reyn_particles p = state.view();
+    c.check(reyn_hard_enforce_bounds(c.get(), bound_min, bound_max, &p), "hard_enforce_bounds");
+}
+
+// ---- time step (src/timestep/cfl.cu) --------------------------------------------------------
+
+inline float cfl_time_step(float lambda, float h, const Particles& state, float3 g)
+{
+    Context& c = state.xx.context();
+    const reyn_particles p = state.view();
+    float dt = 0.f;
+    c.check(reyn_cfl_time_step(c.get(), lambda, h, &p, g, &dt, nullptr), "cfl_time_step");
+    return dt;
+}
+
+inline float simple_dt_controller(unsigned iter_last, float dt_last, float lambda, float h, const Particles& state,
+    float3 g, unsigned iter_target = 5, float change_factor = 0.01f)
+{
+    Context& c = state.xx.context();
+    const reyn_particles p = state.view();
+    float dt = 0.f;
+    c.check(reyn_simple_dt_controller(c.get(), iter_last, dt_last, lambda, h, &p, g, iter_target, change_factor,
+                &dt),
+        "simple_dt_controller");
+    return dt;
+}
+
+// ---- solvers (src/solvers) --------------------------------------------------------------------
+
+namespace detail {
+    template <IsKernel K>
+    reyn_sph_params sph(const K& W, float nu, float h, float rho_0, float3 g)
+    {
+        return reyn_sph_params { K::kKind, W.h_bar(), h, nu, rho_0, g };
+    }
+    template <Resort R> const reyn_grid* grid_of(const UniformGrid<R>& g) { return &g.grid; }
+} // namespace detail
+
+template <IsKernel K, Resort R> class SESPH {
+public:
+    SESPH(K W, uint32_t N, float nu, float h, float rho_0, DeviceBuffer<float>& rho, DeviceBuffer<float>& ax,
+        DeviceBuffer<float>& ay, DeviceBuffer<float>& az, float3 g = v3(0.f, -9.81f, 0.f), float k = 500.f)
+        : prm_ { detail::sph(W, nu, h, rho_0, g), k }
+        , rho_(rho)
+        , ax_(ax)
+        , ay_(ay)
+        , az_(az)
+    {
+        for (auto* b : { &rho_, &ax_, &ay_, &az_ })
+            b->resize(N);
+    }
+    void step(Particles& state, const UniformGrid<R> grid, const BoundarySamples& bdy, float dt)
+    {
+        const reyn_particles p = state.view();
+        const Boundary b = bdy.get();
+        const reyn_sesph_work w { rho_.ptr(), ax_.ptr(), ay_.ptr(), az_.ptr() };
+        Context& c = state.xx.context();
+        c.check(reyn_sesph_step(c.get(), &prm_, &p, detail::grid_of(grid), &b.b, &w, dt), "SESPH::step");
+    }
+    SESPH(const SESPH&) = delete;
+    SESPH& operator=(const SESPH&) = delete;
+
+private:
+    reyn_sesph_params prm_;
+    DeviceBuffer<float>&rho_, &ax_, &ay_, &az_;
+};
+
+template <IsKernel K, Resort R> class PCISPH {
+public:
+    PCISPH(K W, uint32_t N, float nu, float h, float rho_0, DeviceBuffer<float>& rho, DeviceBuffer<float>& ax,
+        DeviceBuffer<float>& ay, DeviceBuffer<float>& az, float3 g = v3(0.f, -9.81f, 0.f))
+        : rho_(rho)
+        , ax_(ax)
+        , ay_(ay)
+        , az_(az)
+    {
+        prm_.sph = detail::sph(W, nu, h, rho_0, g);
+        if (reyn_pcisph_delta(K::kKind, W.h_bar(), h, &prm_.delta) != REYN_OK)
+            throw std::runtime_error("reyn: invalid PCISPH parameters");
+        prm_.eta_rho_max = 0.001f;
+        prm_.min_iter = 3;
+        prm_.max_iter = 0;
+        for (auto* b : { &rho_, &ax_, &ay_, &az_ })
+            b->resize(N);
+    }
+    uint32_t step(Particles& state, const UniformGrid<R> grid, const BoundarySamples& bdy, float dt)
+    {
+        const reyn_particles p = state.view();
+        const Boundary b = bdy.get();
+        const reyn_pcisph_work w { rho_.ptr(), ax_.ptr(), ay_.ptr(), az_.ptr() };
+        Context& c = state.xx.context();
+        c.check(reyn_pcisph_step(c.get(), &prm_, &p, detail::grid_of(grid), &b.b, &w, dt, &info_), "PCISPH::step");
+        return info_.iterations;
+    }
+    reyn_pcisph_params& params() { return prm_; }
+    const reyn_solve_info& info() const { return info_; }
+    PCISPH(const PCISPH&) = delete;
+    PCISPH& operator=(const PCISPH&) = delete;
+
+private:
+    reyn_pcisph_params prm_ {};
+    reyn_solve_info info_ {};
+    DeviceBuffer<float>&rho_, &ax_, &ay_, &az_;
+};
+
+template <IsKernel K, Resort R> class IISPH {
+public:
+    // argument order of the reference's constructor (IISPH.cuh:51-55); the
+    // bit-flag buffer is replaced by an on-device counter and is only resized
+    IISPH(K W, uint32_t N, float nu, float h, float rho_0, DeviceBuffer<float>& rho, DeviceBuffer<float>& a_ii,
+        DeviceBuffer<float>& s_i, DeviceBuffer<float>& ax, DeviceBuffer<float>& ay, DeviceBuffer<float>& az,
+        DeviceBuffer<float>& p, DeviceBuffer<uint32_t>& rho_err_threshold, float3 g = v3(0.f, -9.81f, 0.f))
+        : rho_(rho)
+        , aii_(a_ii)
+        , si_(s_i)
+        , ax_(ax)
+        , ay_(ay)
+        , az_(az)
+        , p_(p)
+    {
+        prm_.sph = detail::sph(W, nu, h, rho_0, g);
+        prm_.omega = 0.5f;
+        prm_.eta_rho_max = 0.005f;
+        prm_.min_iter = 3;
+        prm_.max_iter = 0;
+        prm_.stop_rule = REYN_IISPH_STOP_ANY;
+        prm_.warm_start = 1;
+        prm_.check_every = 4;
+        p_.resize(N, 0.f);
+        for (auto* b : { &rho_, &aii_, &si_, &ax_, &ay_, &az_ })
+            b->resize(N);
+        rho_err_threshold.resize((N + 31) / 32, 0u);
+    }
+    uint32_t step(Particles& state, const UniformGrid<R> grid, const BoundarySamples& bdy, float dt)
+    {
+        const reyn_particles p = state.view();
+        const Boundary b = bdy.get();
+        const reyn_iisph_work w { rho_.ptr(), p_.ptr(), aii_.ptr(), si_.ptr(), ax_.ptr(), ay_.ptr(), az_.ptr() };
+        Context& c = state.xx.context();
+        c.check(reyn_iisph_step(c.get(), &prm_, &p, detail::grid_of(grid), &b.b, &w, dt, &info_), "IISPH::step");
+        return info_.iterations;
+    }
+    reyn_iisph_params& params() { return prm_; }
+    const reyn_solve_info& info() const { return info_; }
+    IISPH(const IISPH&) = delete;
+    IISPH& operator=(const IISPH&) = delete;
+
+private:
+    reyn_iisph_params prm_ {};
+    reyn_solve_info info_ {};
+    DeviceBuffer<float>&rho_, &aii_, &si_, &ax_, &ay_, &az_, &p_;
+};
+
+} // namespace reyn

@@ -1,0 +1,216 @@
+// Tests of the C++ host API (include/reyn/reyn.hpp) on the GPU, written in
+// the style of the reference's doctest cases. Run by tests/test_gpu_cpp_api.py.
+#include <cmath>
+#include <cstdio>
+#include <functional>
+#include <numeric>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "reyn/reyn.hpp"
+
+using namespace reyn;
+
+static int g_failed = 0;
+static int g_checks = 0;
+
+#define CHECK(cond)                                                                                                 \
+    do {                                                                                                            \
+        ++g_checks;                                                                                                 \
+        if (!(cond)) {                                                                                              \
+            ++g_failed;                                                                                             \
+            std::printf("  CHECK failed: %s (%s:%d)\n", #cond, __FILE__, __LINE__);                                 \
+        }                                                                                                           \
+    } while (0)
+
+static void run(const char* name, const std::function<void()>& f)
+{
+    const int before = g_failed;
+    std::printf("TEST_CASE %s\n", name);
+    std::fflush(stdout);
+    try {
+        f();
+    } catch (const std::exception& e) {
+        ++g_failed;
+        std::printf("  exception: %s\n", e.what());
+    }
+    std::printf("  %s\n", g_failed == before ? "ok" : "FAILED");
+    std::fflush(stdout);
+}
+
+struct Scene {
+    float h;
+    std::vector<float> x, y, z;
+    std::vector<float3> wall;
+    float3 box;
+};
+
+static Scene dam_break(int side, float h, float3 box)
+{
+    Scene s { h, {}, {}, {}, {}, box };
+    std::mt19937 rng(7);
+    std::uniform_real_distribution<float> jit(-0.01f * h, 0.01f * h);
+    for (int k = 0; k < side; ++k)
+        for (int j = 0; j < side; ++j)
+            for (int i = 0; i < side; ++i) {
+                s.x.push_back((i + 0.5f) * h + jit(rng));
+                s.y.push_back((j + 0.5f) * h + jit(rng));
+                s.z.push_back((k + 0.5f) * h + jit(rng));
+            }
+    const int nx = int(std::lround(box.x / h)) + 4, ny = int(std::lround(box.y / h)) + 4,
+              nz = int(std::lround(box.z / h)) + 4;
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                const float3 p = v3((i - 2 + 0.5f) * h, (j - 2 + 0.5f) * h, (k - 2 + 0.5f) * h);
+                if (!(p.x > 0 && p.x < box.x && p.y > 0 && p.y < box.y && p.z > 0 && p.z < box.z))
+                    s.wall.push_back(p);
+            }
+    return s;
+}
+
+static void upload(Particles& st, const Scene& s, float rho_0)
+{
+    st.xx.upload(s.x);
+    st.xy.upload(s.y);
+    st.xz.upload(s.z);
+    st.m.fill(rho_0 * s.h * s.h * s.h);
+}
+
+static bool all_finite(const DeviceBuffer<float>& b)
+{
+    for (float v : b.download())
+        if (!std::isfinite(v))
+            return false;
+    return true;
+}
+
+int main()
+{
+    run("DeviceBuffer upload, resize, reductions", [] {
+        std::vector<float> h(1000);
+        std::iota(h.begin(), h.end(), 1.f);
+        DeviceBuffer<float> b(10);
+        b.upload(h);
+        CHECK(b.size() == 1000);
+        CHECK(b.sum() == 500500.0);
+        CHECK(b.min() == 1.0 && b.max() == 1000.0);
+        CHECK(b.avg() == 500.5f);
+        b.resize(10);
+        const auto d = b.download();
+        CHECK(d.size() == 10 && d[9] == 10.f);
+        DeviceBuffer<uint32_t> u(5, 7u);
+        CHECK(u.download()[4] == 7u);
+    });
+
+    run("UniformGridBuilder rejects an inverted box", [] {
+        bool threw = false;
+        try {
+            UniformGridBuilder b(v3(1.f), v3(0.f), 0.1f);
+        } catch (const std::runtime_error& e) {
+            threw = std::string(e.what()).find("Negative number of grid cells") != std::string::npos;
+        }
+        CHECK(threw);
+    });
+
+    run("construct_and_reorder leaves the state in cell order", [] {
+        const float rho_0 = 1000.f;
+        Scene s = dam_break(12, 0.05f, v3(1.f, 1.f, 1.f));
+        // shuffle the input order
+        std::vector<size_t> perm(s.x.size());
+        std::iota(perm.begin(), perm.end(), 0);
+        std::shuffle(perm.begin(), perm.end(), std::mt19937(3));
+        Scene t = s;
+        for (size_t i = 0; i < perm.size(); ++i) {
+            t.x[i] = s.x[perm[i]];
+            t.y[i] = s.y[perm[i]];
+            t.z[i] = s.z[perm[i]];
+        }
+        Particles st(t.x.size(), rho_0);
+        upload(st, t, rho_0);
+        DeviceBuffer<float> tmp(1);
+        UniformGridBuilder b(v3(0.f), v3(1.f), 0.1f);
+        const auto g1 = b.construct_and_reorder(0.1f, tmp, st);
+        CHECK(g1.grid.sorted == nullptr && g1.grid.n == st.size());
+        // rebuilding the reordered state gives the identity permutation
+        const auto g2 = b.construct(0.1f, st.xx, st.xy, st.xz);
+        std::vector<uint32_t> sorted(st.size());
+        reyn_memcpy(Context::current().get(), sorted.data(), g2.grid.sorted, sorted.size() * 4, 1);
+        bool identity = true;
+        for (uint32_t i = 0; i < sorted.size(); ++i)
+            identity = identity && sorted[i] == i;
+        CHECK(identity);
+        // the positions are the same multiset
+        auto xs = st.xx.download();
+        std::sort(xs.begin(), xs.end());
+        auto ref = t.x;
+        std::sort(ref.begin(), ref.end());
+        CHECK(xs == ref);
+    });
+
+    run("IISPH dam break through the host API", [] {
+        const float rho_0 = 1000.f, h = 1.f / 24;
+        Scene s = dam_break(16, h, v3(1.5f, 1.f, 1.f));
+        const size_t N = s.x.size();
+        Particles state(N, rho_0);
+        upload(state, s, rho_0);
+        BoundarySamples bdy(s.wall, h, rho_0);
+        DeviceBuffer<float> tmp(N), rho(1), aii(1), si(1), ax(1), ay(1), az(1), prs(1);
+        DeviceBuffer<uint32_t> bits(1);
+        IISPH<B3, Resort::yes> solver(B3(2.f * h), uint32_t(N), 0.001f, h, rho_0, rho, aii, si, ax, ay, az, prs,
+            bits);
+        UniformGridBuilder builder(bdy.bound_min, bdy.bound_max, 2.f * h);
+        const double m0 = state.m.sum();
+        unsigned it_min = 1000;
+        for (int step = 0; step < 20; ++step) {
+            const float dt = cfl_time_step(0.4f, h, state, v3(0.f, -9.81f, 0.f));
+            CHECK(dt > 0.f && std::isfinite(dt));
+            const auto grid = builder.construct_and_reorder(2.f * h, tmp, state, &prs);
+            it_min = std::min(it_min, solver.step(state, grid, bdy, dt));
+            hard_enforce_bounds(bdy.bound_min, bdy.bound_max, state);
+        }
+        CHECK(it_min >= 3);
+        CHECK(solver.info().n_violating == 0);
+        CHECK(state.m.sum() == m0);
+        CHECK(all_finite(state.xx) && all_finite(state.vy) && all_finite(prs));
+        CHECK(std::fabs(rho.avg() / rho_0 - 1.f) < 0.05f);
+        CHECK(state.xy.min() >= bdy.bound_min.y && state.xy.max() <= bdy.bound_max.y);
+    });
+
+    run("SESPH and PCISPH steps through the host API", [] {
+        const float rho_0 = 1000.f, h = 0.05f;
+        Scene s = dam_break(12, h, v3(1.f, 1.f, 1.f));
+        const size_t N = s.x.size();
+        BoundarySamples bdy(s.wall, h, rho_0);
+        UniformGridBuilder builder(bdy.bound_min, bdy.bound_max, 2.f * h, 1.0f);
+        DeviceBuffer<float> tmp(N), rho(1), ax(1), ay(1), az(1);
+        {
+            Particles state(N, rho_0);
+            upload(state, s, rho_0);
+            SESPH<C2, Resort::yes> sesph(C2(2.f * h), uint32_t(N), 0.001f, h, rho_0, rho, ax, ay, az,
+                v3(0.f, -9.81f, 0.f), 1000.f);
+            for (int step = 0; step < 5; ++step) {
+                const auto grid = builder.construct_and_reorder(2.f * h, tmp, state);
+                sesph.step(state, grid, bdy, 1e-3f);
+            }
+            CHECK(all_finite(state.xy) && all_finite(state.vy));
+            CHECK(state.vy.avg() < 0.f);
+        }
+        {
+            Particles state(N, rho_0);
+            upload(state, s, rho_0);
+            PCISPH<W6, Resort::no> pcisph(W6(2.f * h), uint32_t(N), 0.001f, h, rho_0, rho, ax, ay, az);
+            unsigned iters = 0;
+            for (int step = 0; step < 5; ++step) {
+                const auto grid = builder.construct(2.f * h, state.xx, state.xy, state.xz);
+                iters = pcisph.step(state, grid, bdy, 1e-3f);
+            }
+            CHECK(iters >= 3);
+            CHECK(all_finite(state.xy) && all_finite(rho));
+        }
+    });
+
+    std::printf("%d checks, %d failed\n", g_checks, g_failed);
+    return g_failed == 0 ? 0 : 1;
+}
