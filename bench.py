@@ -97,7 +97,7 @@ def cpu_baseline(sc, host_state, p, dt_steps, args, log) -> dict | None:
     t0 = time.perf_counter()
     iters = []
     for _ in range(args.cpu_steps):
-        dt, _ = orc.cfl_time_step(0.4, h, np.stack([fields["vx"], fields["vy"], fields["vz"]], 1))
+        dt = dt_steps
         g = orc.grid_build(geom, fields["xx"], fields["xy"], fields["xz"])
         for k in fields:
             fields[k] = orc.gather(g.sorted, fields[k])
@@ -132,6 +132,10 @@ def main() -> int:
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="time without per-launch events")
+    ap.add_argument("--dt-policy", choices=("controller", "cfl"), default="controller",
+                    help="controller: the reference main loop's simple_dt_controller from dt0 = 5e-4 "
+                         "(lambda 1, 10 target iterations, 0.1%% change per step); cfl: cfl_time_step(lambda)")
+    ap.add_argument("--cfl-lambda", type=float, default=None)
     args = ap.parse_args()
 
     def log(msg):
@@ -154,8 +158,10 @@ def main() -> int:
     bdy = BoundarySamples(ctx, sc.xb, sc.spacing, sc.rho_0)
     solver = IISPH(ctx, "B3", sc.n, 0.001, sc.spacing, sc.rho_0, omega=0.5, eta_rho_max=0.001, min_iter=3,
                    max_iter=100, stop_rule=capi.IISPH_STOP_MEAN, warm_start=True, check_every=4)
-    sim = Simulation(ctx, solver, state, bdy, sc.spacing, sc.bound_min, sc.bound_max, cfl_lambda=0.4,
-                     cell_ratio=0.5, extra_reorder=("p",))
+    lam = args.cfl_lambda if args.cfl_lambda is not None else (1.0 if args.dt_policy == "controller" else 0.4)
+    sim = Simulation(ctx, solver, state, bdy, sc.spacing, sc.bound_min, sc.bound_max, cfl_lambda=lam,
+                     cell_ratio=0.5, extra_reorder=("p",), dt_policy=args.dt_policy, dt0=0.0005, iter_target=10,
+                     change_factor=0.001)
     ncells = sim.builder.geometry().ncells
     log(f"setup {time.perf_counter() - t_setup:.1f} s; fluid grid {ncells} cells")
 
@@ -167,12 +173,14 @@ def main() -> int:
     ctx._lib.reyn_profile_enable(ctx.h, 0 if args.no_profile else 1)
     iters = []
     errs = []
+    dts = []
     t0 = time.perf_counter()
     ctx.timer_start()
     for k in range(args.steps):
         s = sim.step()
         iters.append(s.iterations)
         errs.append(s.mean_rel_error)
+        dts.append(s.dt)
     gpu_ms = ctx.timer_stop()
     ctx.synchronize()
     wall = time.perf_counter() - t0
@@ -219,7 +227,7 @@ def main() -> int:
         sc.xb_sorted = bdy.positions()
         sc.mb = bdy.m.numpy()
         sc.bdy_min, sc.bdy_max = bdy.bound_min, bdy.bound_max
-        cpu = cpu_baseline(sc, host, solver.p.numpy(), None, args, log)
+        cpu = cpu_baseline(sc, host, solver.p.numpy(), sim.next_dt(), args, log)
 
     out = {
         "metric": METRIC,
@@ -237,15 +245,19 @@ def main() -> int:
         "config": {
             "workload": f"IISPH dam break, {sc.n} FP32 fluid particles (128^3 block at s=1/128) + {sc.n_bdy} "
                         "boundary samples (2 wall layers of box [0,2]x[0,1.25]x[0,2] + sphere obstacle), B3 "
-                        "kernel, omega 0.5, stop at mean error <= 0.1% or 100 iterations, CFL lambda 0.4, "
-                        "grid rebuild + reorder every step",
+                        "kernel, omega 0.5, stop at mean error <= 0.1% or 100 iterations, grid rebuild + "
+                        f"reorder every step, dt policy {args.dt_policy}",
             "fluid_particles": sc.n,
             "boundary_particles": sc.n_bdy,
             "grid_cells": ncells,
             "cell_ratio": 0.5,
             "jacobi_iterations_mean": round(mean_iters, 2),
             "jacobi_iterations": iters,
-            "mean_density_error_last": errs[-1] if errs else None,
+            "mean_density_error_max": max(errs) if errs else None,
+            "dt_policy": args.dt_policy,
+            "cfl_lambda": lam,
+            "dt_first_last": [dts[0], dts[-1]] if dts else None,
+            "simulated_time_timed": float(sum(dts)),
             "density_mean_rel_dev": rho_err,
             "step_hbm_frac": round(step_frac, 5),
             "step_bytes": sbytes,

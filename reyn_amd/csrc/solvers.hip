@@ -56,14 +56,20 @@ struct Scratch {
     double* part_d; // per-block FP64 partials
     uint32_t* part_u; // per-block counts
     SolveCtl* ctl;
+    NbrDev L; // the step's neighbour list (cnt == NULL when not requested)
 };
 
-int get_scratch(reyn_ctx* ctx, uint32_t n, Scratch* s)
+// Neighbours kept per particle; a fluid particle at rest density has about
+// (4/3) pi 2^3 = 33.5 neighbours within the support 2h.
+constexpr uint32_t kListCap = 64;
+
+int get_scratch(reyn_ctx* ctx, uint32_t n, Scratch* s, bool lists = false)
 {
     const size_t nb = blocks_for(n);
     auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
     const size_t f = al(size_t(n) * 4);
-    const size_t bytes = 4 * f + al(nb * 8) + al(nb * 4) + al(sizeof(SolveCtl));
+    const size_t head = 4 * f + al(nb * 8) + al(nb * 4) + al(sizeof(SolveCtl));
+    const size_t bytes = head + (lists ? f + al(size_t(n) * kListCap * 4) : 0);
     char* p = nullptr;
     REYN_TRY(ctx_scratch(ctx, bytes, reinterpret_cast<void**>(&p)));
     s->gbx = reinterpret_cast<float*>(p);
@@ -73,6 +79,11 @@ int get_scratch(reyn_ctx* ctx, uint32_t n, Scratch* s)
     s->part_d = reinterpret_cast<double*>(p + 4 * f);
     s->part_u = reinterpret_cast<uint32_t*>(p + 4 * f + al(nb * 8));
     s->ctl = reinterpret_cast<SolveCtl*>(p + 4 * f + al(nb * 8) + al(nb * 4));
+    s->L = NbrDev { nullptr, nullptr, n, kListCap };
+    if (lists) {
+        s->L.cnt = reinterpret_cast<uint32_t*>(p + head);
+        s->L.idx = reinterpret_cast<uint32_t*>(p + head + f);
+    }
     return REYN_OK;
 }
 
@@ -80,10 +91,12 @@ __device__ __forceinline__ bool loop_done(const SolveCtl* ctl) { return ctl && c
 
 // ---- density ------------------------------------------------------------------
 
+// Densities; with GB also the boundary sums, and with a list in L the
+// neighbour list of the step (every later fluid loop reads it).
 template <int K, bool IND, bool GB>
 __global__ __launch_bounds__(kBlock) void k_density(FluidDev f, GridDev g, BdyDev b, KParams kp,
     float* __restrict__ rho, float* __restrict__ gbx, float* __restrict__ gby,
-    float* __restrict__ gbz, float* __restrict__ rb)
+    float* __restrict__ gbz, float* __restrict__ rb, NbrDev L)
 {
     const uint32_t t = global_index();
     if (t >= f.n)
@@ -91,7 +104,18 @@ __global__ __launch_bounds__(kBlock) void k_density(FluidDev f, GridDev g, BdyDe
     const uint32_t i = particle_of<IND>(g, t);
     const f3 xi = ld3(f.x, f.y, f.z, i);
     ksum acc;
-    walk<IND>(g, xi, f.x, f.y, f.z, [&](uint32_t j, f3, float d2) { acc.add(f.m[j] * kW<K>(kp, d2)); });
+    uint32_t cnt = 0;
+    uint32_t* __restrict__ col = L.cnt ? L.idx + t : nullptr;
+    walk<IND>(g, xi, f.x, f.y, f.z, [&](uint32_t j, f3, float d2) {
+        acc.add(f.m[j] * kW<K>(kp, d2));
+        if (col && j != i) {
+            if (cnt < L.cap)
+                col[size_t(cnt) * L.n] = j;
+            ++cnt;
+        }
+    });
+    if (col)
+        L.cnt[t] = cnt;
     ksum accb;
     ksum3 grad;
     if (b.present) {
@@ -118,7 +142,7 @@ __global__ __launch_bounds__(kBlock) void k_density(FluidDev f, GridDev g, BdyDe
 template <int K, bool IND>
 __global__ __launch_bounds__(kBlock) void k_nonpressure(FluidDev f, GridDev g, SphK c,
     const float* __restrict__ rho, float* __restrict__ ax, float* __restrict__ ay,
-    float* __restrict__ az)
+    float* __restrict__ az, NbrDev L)
 {
     const uint32_t t = global_index();
     if (t >= f.n)
@@ -127,7 +151,7 @@ __global__ __launch_bounds__(kBlock) void k_nonpressure(FluidDev f, GridDev g, S
     const f3 xi = ld3(f.x, f.y, f.z, i);
     const f3 vi = ld3(f.vx, f.vy, f.vz, i);
     f3 acc = c.g;
-    walk<IND>(g, xi, f.x, f.y, f.z, [&](uint32_t j, f3 xij, float d2) {
+    nbrs<IND>(g, L, t, xi, f.x, f.y, f.z, [&](uint32_t j, f3 xij, float d2) {
         const f3 vij = vi - ld3(f.vx, f.vy, f.vz, j);
         const float s = c.visc_c * f.m[j] / rho[j] * dot(vij, xij) / (d2 + c.visc_eps);
         acc += s * kdW<K>(c.kp, xij, d2);
@@ -163,7 +187,7 @@ template <int K, bool IND>
 __global__ __launch_bounds__(kBlock) void k_sesph_acc(FluidDev f, GridDev g, int bdy, SphK c,
     float k, const float* __restrict__ rho, const float* __restrict__ gbx,
     const float* __restrict__ gby, const float* __restrict__ gbz, float* __restrict__ ax,
-    float* __restrict__ ay, float* __restrict__ az)
+    float* __restrict__ ay, float* __restrict__ az, NbrDev L)
 {
     const uint32_t t = global_index();
     if (t >= f.n)
@@ -174,7 +198,7 @@ __global__ __launch_bounds__(kBlock) void k_sesph_acc(FluidDev f, GridDev g, int
     const float rhoi = rho[i];
     const float pri = eos_sesph(rhoi, k, c.rho0) / (rhoi * rhoi);
     f3 acc = c.g;
-    walk<IND>(g, xi, f.x, f.y, f.z, [&](uint32_t j, f3 xij, float d2) {
+    nbrs<IND>(g, L, t, xi, f.x, f.y, f.z, [&](uint32_t j, f3 xij, float d2) {
         const float rhoj = rho[j];
         const float mj = f.m[j];
         const f3 vij = vi - ld3(f.vx, f.vy, f.vz, j);
@@ -195,7 +219,7 @@ template <int K, bool IND>
 __global__ __launch_bounds__(kBlock) void k_predict_rho(FluidDev f, GridDev g, int bdy, KParams kp,
     float dt, const float* __restrict__ gbx, const float* __restrict__ gby,
     const float* __restrict__ gbz, const float* __restrict__ rb, float* __restrict__ rho,
-    double* __restrict__ part, const SolveCtl* ctl)
+    double* __restrict__ part, const SolveCtl* ctl, NbrDev L)
 {
     __shared__ double lds[kBlock / kWave];
     if (loop_done(ctl))
@@ -207,10 +231,12 @@ __global__ __launch_bounds__(kBlock) void k_predict_rho(FluidDev f, GridDev g, i
         const f3 xi = ld3(f.x, f.y, f.z, i);
         const f3 vi = ld3(f.vx, f.vy, f.vz, i);
         ksum acc;
-        walk<IND>(g, xi, f.x, f.y, f.z, [&](uint32_t j, f3 xij, float d2) {
+        const bool listed = nbrs<IND>(g, L, t, xi, f.x, f.y, f.z, [&](uint32_t j, f3 xij, float d2) {
             const f3 vij = vi - ld3(f.vx, f.vy, f.vz, j);
             acc.add(f.m[j] * (kW<K>(kp, d2) + dt * dot(kdW<K>(kp, xij, d2), vij)));
         });
+        if (listed) // the list skips the particle itself: m_i (W(0) + dt grad W(0) . 0)
+            acc.add(f.m[i] * kW<K>(kp, 0.f));
         if (bdy)
             acc.add(rb[i] + dt * dot(ld3(gbx, gby, gbz, i), vi));
         const float r = acc.value();
@@ -226,7 +252,7 @@ template <int K, bool IND>
 __global__ __launch_bounds__(kBlock) void k_pcisph_prs_acc(FluidDev f, GridDev g, int bdy,
     KParams kp, float k, float rho0, float rho0_sq_inv, const float* __restrict__ rho,
     const float* __restrict__ gbx, const float* __restrict__ gby, const float* __restrict__ gbz,
-    float* __restrict__ ax, float* __restrict__ ay, float* __restrict__ az, const SolveCtl* ctl)
+    float* __restrict__ ax, float* __restrict__ ay, float* __restrict__ az, const SolveCtl* ctl, NbrDev L)
 {
     if (loop_done(ctl))
         return;
@@ -240,7 +266,7 @@ __global__ __launch_bounds__(kBlock) void k_pcisph_prs_acc(FluidDev f, GridDev g
     const float pri = pi / (rhoi * rhoi);
     const float pr0 = pi * rho0_sq_inv;
     f3 acc { 0.f, 0.f, 0.f };
-    walk<IND>(g, xi, f.x, f.y, f.z, [&](uint32_t j, f3 xij, float d2) {
+    nbrs<IND>(g, L, t, xi, f.x, f.y, f.z, [&](uint32_t j, f3 xij, float d2) {
         const float rhoj = rho[j];
         const float pj = fmaxf(0.f, k * (rhoj - rho0));
         acc += (f.m[j] * (pri + pj / (rhoj * rhoj))) * kdW<K>(kp, xij, d2);
@@ -285,7 +311,7 @@ template <int K, bool IND>
 __global__ __launch_bounds__(kBlock) void k_iisph_aii_s(FluidDev f, GridDev g, int bdy, KParams kp,
     float rho0, float dt, float omega, int warm, const float* __restrict__ rho,
     const float* __restrict__ gbx, const float* __restrict__ gby, const float* __restrict__ gbz,
-    float* __restrict__ aii, float* __restrict__ si, float* __restrict__ p)
+    float* __restrict__ aii, float* __restrict__ si, float* __restrict__ p, NbrDev L)
 {
     const uint32_t t = global_index();
     if (t >= f.n)
@@ -295,7 +321,7 @@ __global__ __launch_bounds__(kBlock) void k_iisph_aii_s(FluidDev f, GridDev g, i
     const f3 vi = ld3(f.vx, f.vy, f.vz, i);
     ksum3 grad;
     ksum sq, div;
-    walk<IND>(g, xi, f.x, f.y, f.z, [&](uint32_t j, f3 xij, float d2) {
+    nbrs<IND>(g, L, t, xi, f.x, f.y, f.z, [&](uint32_t j, f3 xij, float d2) {
         const f3 dW = kdW<K>(kp, xij, d2);
         const f3 mdW = f.m[j] * dW;
         div.add(dot(mdW, vi - ld3(f.vx, f.vy, f.vz, j)));
@@ -317,7 +343,7 @@ template <int K, bool IND>
 __global__ __launch_bounds__(kBlock) void k_iisph_pacc(FluidDev f, GridDev g, int bdy, KParams kp,
     float rho0_sq_inv, const float* __restrict__ p, const float* __restrict__ gbx,
     const float* __restrict__ gby, const float* __restrict__ gbz, float* __restrict__ ax,
-    float* __restrict__ ay, float* __restrict__ az, const SolveCtl* ctl)
+    float* __restrict__ ay, float* __restrict__ az, const SolveCtl* ctl, NbrDev L)
 {
     if (loop_done(ctl))
         return;
@@ -328,7 +354,7 @@ __global__ __launch_bounds__(kBlock) void k_iisph_pacc(FluidDev f, GridDev g, in
     const f3 xi = ld3(f.x, f.y, f.z, i);
     const float pi = p[i];
     f3 acc { 0.f, 0.f, 0.f };
-    walk<IND>(g, xi, f.x, f.y, f.z,
+    nbrs<IND>(g, L, t, xi, f.x, f.y, f.z,
         [&](uint32_t j, f3 xij, float d2) { acc += (f.m[j] * (pi + p[j])) * kdW<K>(kp, xij, d2); });
     if (bdy)
         acc += pi * ld3(gbx, gby, gbz, i);
@@ -341,7 +367,7 @@ __global__ __launch_bounds__(kBlock) void k_iisph_jacobi(FluidDev f, GridDev g, 
     const float* __restrict__ az, const float* __restrict__ aii, const float* __restrict__ si,
     const float* __restrict__ gbx, const float* __restrict__ gby, const float* __restrict__ gbz,
     float* __restrict__ p, double* __restrict__ part_err, uint32_t* __restrict__ part_cnt,
-    const SolveCtl* ctl)
+    const SolveCtl* ctl, NbrDev L)
 {
     __shared__ double lds[kBlock / kWave];
     if (loop_done(ctl))
@@ -354,7 +380,7 @@ __global__ __launch_bounds__(kBlock) void k_iisph_jacobi(FluidDev f, GridDev g, 
         const f3 xi = ld3(f.x, f.y, f.z, i);
         const f3 ai = ld3(ax, ay, az, i);
         float lap = 0.f;
-        walk<IND>(g, xi, f.x, f.y, f.z, [&](uint32_t j, f3 xij, float d2) {
+        nbrs<IND>(g, L, t, xi, f.x, f.y, f.z, [&](uint32_t j, f3 xij, float d2) {
             lap += f.m[j] * dot(ai - ld3(ax, ay, az, j), kdW<K>(kp, xij, d2));
         });
         if (bdy)
@@ -458,7 +484,8 @@ int launch_density(reyn_ctx* ctx, int kernel, const Common& c, const KParams& kp
         {
             ProfScope _prof(ctx, "k_density");
             k_density<K, I, GB><<<c.nb, kBlock, 0, ctx->stream>>>(c.f, c.g, c.b, kp, rho,
-                GB ? s->gbx : nullptr, GB ? s->gby : nullptr, GB ? s->gbz : nullptr, GB ? s->rb : nullptr);
+                GB ? s->gbx : nullptr, GB ? s->gby : nullptr, GB ? s->gbz : nullptr, GB ? s->rb : nullptr,
+                s ? s->L : NbrDev {});
         }
         REYN_LAUNCHED(ctx, "k_density");
         return REYN_OK;
@@ -466,14 +493,14 @@ int launch_density(reyn_ctx* ctx, int kernel, const Common& c, const KParams& kp
 }
 
 int launch_nonpressure(reyn_ctx* ctx, int kernel, const Common& c, const SphK& k, const float* rho,
-    float* ax, float* ay, float* az)
+    float* ax, float* ay, float* az, const NbrDev& L)
 {
     return with_kernel(ctx, kernel, c.ind, [&](auto KC, auto IC) -> int {
         constexpr int K = decltype(KC)::value;
         constexpr bool I = decltype(IC)::value;
         {
             ProfScope _prof(ctx, "k_nonpressure");
-            k_nonpressure<K, I><<<c.nb, kBlock, 0, ctx->stream>>>(c.f, c.g, k, rho, ax, ay, az);
+            k_nonpressure<K, I><<<c.nb, kBlock, 0, ctx->stream>>>(c.f, c.g, k, rho, ax, ay, az, L);
         }
         REYN_LAUNCHED(ctx, "k_nonpressure");
         return REYN_OK;
@@ -553,7 +580,7 @@ int reyn_nonpressure_acc(reyn_ctx* ctx, const reyn_sph_params* prm, const reyn_p
         return REYN_OK;
     if (!rho || !ax || !ay || !az)
         return set_error(ctx, REYN_ERR_INVALID_ARGUMENT, "nonpressure_acc: NULL buffer");
-    return launch_nonpressure(ctx, prm->kernel, c, make_sphk(*prm), rho, ax, ay, az);
+    return launch_nonpressure(ctx, prm->kernel, c, make_sphk(*prm), rho, ax, ay, az, NbrDev {});
 }
 
 int reyn_predict_rho(reyn_ctx* ctx, const reyn_sph_params* prm, const reyn_particles* fluid,
@@ -578,7 +605,7 @@ int reyn_predict_rho(reyn_ctx* ctx, const reyn_sph_params* prm, const reyn_parti
         {
             ProfScope _prof(ctx, "k_predict_rho");
             k_predict_rho<K, I><<<c.nb, kBlock, 0, ctx->stream>>>(c.f, c.g, c.b.present, kp, dt, s.gbx,
-                s.gby, s.gbz, s.rb, rho, nullptr, nullptr);
+                s.gby, s.gbz, s.rb, rho, nullptr, nullptr, NbrDev {});
         }
         REYN_LAUNCHED(ctx, "k_predict_rho");
         return REYN_OK;
@@ -625,7 +652,7 @@ int reyn_sesph_step(reyn_ctx* ctx, const reyn_sesph_params* prm, const reyn_part
     if (!work->rho || !work->ax || !work->ay || !work->az)
         return set_error(ctx, REYN_ERR_INVALID_ARGUMENT, "sesph: NULL work buffer");
     Scratch s;
-    REYN_TRY(get_scratch(ctx, fluid->n, &s));
+    REYN_TRY(get_scratch(ctx, fluid->n, &s, true));
     const SphK k = make_sphk(prm->sph);
     REYN_TRY(launch_density<true>(ctx, prm->sph.kernel, c, k.kp, work->rho, &s));
     REYN_TRY(with_kernel(ctx, prm->sph.kernel, c.ind, [&](auto KC, auto IC) -> int {
@@ -634,7 +661,7 @@ int reyn_sesph_step(reyn_ctx* ctx, const reyn_sesph_params* prm, const reyn_part
         {
             ProfScope _prof(ctx, "k_sesph_acc");
             k_sesph_acc<K, I><<<c.nb, kBlock, 0, ctx->stream>>>(c.f, c.g, c.b.present, k, prm->k,
-                work->rho, s.gbx, s.gby, s.gbz, work->ax, work->ay, work->az);
+                work->rho, s.gbx, s.gby, s.gbz, work->ax, work->ay, work->az, s.L);
         }
         REYN_LAUNCHED(ctx, "k_sesph_acc");
         return REYN_OK;
@@ -717,13 +744,13 @@ int reyn_pcisph_step(reyn_ctx* ctx, const reyn_pcisph_params* prm, const reyn_pa
     if (!(dt > 0.f))
         return set_error(ctx, REYN_ERR_INVALID_ARGUMENT, "pcisph: dt must be positive");
     Scratch s;
-    REYN_TRY(get_scratch(ctx, fluid->n, &s));
+    REYN_TRY(get_scratch(ctx, fluid->n, &s, true));
     const SphK k = make_sphk(prm->sph);
     const float kk = static_cast<float>((prm->delta / dt) / dt);
     const float rho0 = prm->sph.rho_0;
     const float thr = (prm->eta_rho_max + 1.f) * rho0;
     REYN_TRY(launch_density<true>(ctx, prm->sph.kernel, c, k.kp, work->rho, &s));
-    REYN_TRY(launch_nonpressure(ctx, prm->sph.kernel, c, k, work->rho, work->ax, work->ay, work->az));
+    REYN_TRY(launch_nonpressure(ctx, prm->sph.kernel, c, k, work->rho, work->ax, work->ay, work->az, s.L));
     SolveCtl res;
     REYN_TRY(run_loop(
         ctx, s.ctl, prm->min_iter, prm->max_iter, 4,
@@ -735,14 +762,14 @@ int reyn_pcisph_step(reyn_ctx* ctx, const reyn_pcisph_params* prm, const reyn_pa
                 {
                     ProfScope _prof(ctx, "k_predict_rho");
                     k_predict_rho<K, I><<<c.nb, kBlock, 0, ctx->stream>>>(c.f, c.g, c.b.present, k.kp, dt,
-                        s.gbx, s.gby, s.gbz, s.rb, work->rho, s.part_d, s.ctl);
+                        s.gbx, s.gby, s.gbz, s.rb, work->rho, s.part_d, s.ctl, s.L);
                 }
                 REYN_LAUNCHED(ctx, "k_predict_rho");
                 {
                     ProfScope _prof(ctx, "k_pcisph_prs_acc");
                     k_pcisph_prs_acc<K, I><<<c.nb, kBlock, 0, ctx->stream>>>(c.f, c.g, c.b.present, k.kp, kk,
                         rho0, 1.f / (rho0 * rho0), work->rho, s.gbx, s.gby, s.gbz, work->ax, work->ay,
-                        work->az, s.ctl);
+                        work->az, s.ctl, s.L);
                 }
                 REYN_LAUNCHED(ctx, "k_pcisph_prs_acc");
                 return REYN_OK;
@@ -783,7 +810,7 @@ int reyn_iisph_step(reyn_ctx* ctx, const reyn_iisph_params* prm, const reyn_part
     if (prm->stop_rule != REYN_IISPH_STOP_ANY && prm->stop_rule != REYN_IISPH_STOP_MEAN)
         return set_error(ctx, REYN_ERR_INVALID_ARGUMENT, "iisph: unknown stop rule %d", prm->stop_rule);
     Scratch s;
-    REYN_TRY(get_scratch(ctx, fluid->n, &s));
+    REYN_TRY(get_scratch(ctx, fluid->n, &s, true));
     const SphK k = make_sphk(prm->sph);
     const float rho0 = prm->sph.rho_0;
     const float rho0_sq_inv = 1.f / (rho0 * rho0);
@@ -791,7 +818,7 @@ int reyn_iisph_step(reyn_ctx* ctx, const reyn_iisph_params* prm, const reyn_part
     const int kind = prm->sph.kernel;
     const int bd = c.b.present;
     REYN_TRY(launch_density<true>(ctx, kind, c, k.kp, work->rho, &s));
-    REYN_TRY(launch_nonpressure(ctx, kind, c, k, work->rho, work->ax, work->ay, work->az));
+    REYN_TRY(launch_nonpressure(ctx, kind, c, k, work->rho, work->ax, work->ay, work->az, s.L));
     REYN_TRY(launch_integrate(ctx, c, work->ax, work->ay, work->az, dt, nullptr, 0));
     REYN_TRY(with_kernel(ctx, kind, c.ind, [&](auto KC, auto IC) -> int {
         constexpr int K = decltype(KC)::value;
@@ -799,7 +826,7 @@ int reyn_iisph_step(reyn_ctx* ctx, const reyn_iisph_params* prm, const reyn_part
         {
             ProfScope _prof(ctx, "k_iisph_aii_s");
             k_iisph_aii_s<K, I><<<c.nb, kBlock, 0, ctx->stream>>>(c.f, c.g, bd, k.kp, rho0, dt, prm->omega,
-                prm->warm_start, work->rho, s.gbx, s.gby, s.gbz, work->a_ii, work->s_i, work->p);
+                prm->warm_start, work->rho, s.gbx, s.gby, s.gbz, work->a_ii, work->s_i, work->p, s.L);
         }
         REYN_LAUNCHED(ctx, "k_iisph_aii_s");
         return REYN_OK;
@@ -811,7 +838,7 @@ int reyn_iisph_step(reyn_ctx* ctx, const reyn_iisph_params* prm, const reyn_part
             {
                 ProfScope _prof(ctx, "k_iisph_pacc");
                 k_iisph_pacc<K, I><<<c.nb, kBlock, 0, ctx->stream>>>(c.f, c.g, bd, k.kp, rho0_sq_inv,
-                    work->p, s.gbx, s.gby, s.gbz, work->ax, work->ay, work->az, ctl);
+                    work->p, s.gbx, s.gby, s.gbz, work->ax, work->ay, work->az, ctl, s.L);
             }
             REYN_LAUNCHED(ctx, "k_iisph_pacc");
             return REYN_OK;
@@ -829,7 +856,7 @@ int reyn_iisph_step(reyn_ctx* ctx, const reyn_iisph_params* prm, const reyn_part
                     ProfScope _prof(ctx, "k_iisph_jacobi");
                     k_iisph_jacobi<K, I><<<c.nb, kBlock, 0, ctx->stream>>>(c.f, c.g, bd, k.kp, dt, prm->omega,
                         thr, work->ax, work->ay, work->az, work->a_ii, work->s_i, s.gbx, s.gby, s.gbz, work->p,
-                        s.part_d, s.part_u, s.ctl);
+                        s.part_d, s.part_u, s.ctl, s.L);
                 }
                 REYN_LAUNCHED(ctx, "k_iisph_jacobi");
                 return REYN_OK;

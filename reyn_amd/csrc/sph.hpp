@@ -213,6 +213,43 @@ __device__ __forceinline__ void walk(const GridDev& g, f3 xi, const float* __res
     }
 }
 
+// ---- per-step neighbour lists ------------------------------------------------
+// The density pass of a solver step walks the grid once and records, for the
+// particle in cell-order slot t, its fluid neighbours j != self in walk order:
+// idx[k * n + t] for k < cnt[t] (column-major, so a wavefront reads each k as
+// one contiguous row). A particle with more than `cap` neighbours keeps the
+// true count and is walked again instead.
+
+struct NbrDev {
+    uint32_t* cnt; // NULL: no list, every loop walks the grid
+    uint32_t* idx;
+    uint32_t n, cap;
+};
+
+// Calls f(j, x_ij, |x_ij|^2) for the neighbours of the particle in slot t at
+// xi. Through the list the particle itself is skipped (returns true); through
+// the grid it is included (returns false). Every list entry gives the same
+// x_ij and |x_ij|^2 bits as the walk.
+template <bool IND, class F>
+__device__ __forceinline__ bool nbrs(const GridDev& g, const NbrDev& L, uint32_t t, f3 xi,
+    const float* __restrict__ xs, const float* __restrict__ ys, const float* __restrict__ zs, F&& f)
+{
+    if (L.cnt) {
+        const uint32_t c = L.cnt[t];
+        if (c <= L.cap) {
+            const uint32_t* __restrict__ col = L.idx + t;
+            for (uint32_t k = 0; k < c; ++k) {
+                const uint32_t j = col[size_t(k) * L.n];
+                const f3 xij = xi - f3 { xs[j], ys[j], zs[j] };
+                f(j, xij, dot(xij, xij));
+            }
+            return true;
+        }
+    }
+    walk<IND>(g, xi, xs, ys, zs, f);
+    return false;
+}
+
 // ---- particle views ----------------------------------------------------------
 
 struct FluidDev {

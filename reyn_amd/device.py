@@ -8,6 +8,7 @@ storage role of the reference's ``DeviceBuffer<T>`` (src/buffer.cuh:15-269).
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 
 import numpy as np
 
@@ -23,6 +24,12 @@ class Context:
             raise capi.ReynError(rc, f"cannot create a context on HIP device {device}")
         self.h = h
         self.device = device
+        # objects holding library resources of this context; they are released
+        # before the context itself, whatever order Python finalises them in
+        self._children = weakref.WeakSet()
+
+    def adopt(self, obj) -> None:
+        self._children.add(obj)
 
     # -- errors ------------------------------------------------------------
     def last_error(self) -> str:
@@ -37,6 +44,8 @@ class Context:
     # -- lifetime ------------------------------------------------------------
     def close(self) -> None:
         if getattr(self, "h", None):
+            for child in list(self._children):
+                child.release()
             self._lib.reyn_ctx_destroy(self.h)
             self.h = None
 
@@ -90,6 +99,7 @@ class DeviceArray:
         p = C.c_void_p()
         ctx.check(ctx._lib.reyn_malloc(ctx.h, C.byref(p), max(self.nbytes, 4)), "malloc")
         self._ptr = p
+        ctx.adopt(self)
 
     @property
     def nbytes(self) -> int:
@@ -139,6 +149,8 @@ class DeviceArray:
         if getattr(self, "_ptr", None) is not None and self._ptr.value and self.ctx.h:
             self.ctx._lib.reyn_free(self.ctx.h, self._ptr)
         self._ptr = C.c_void_p()
+
+    release = free
 
     def __del__(self):
         try:

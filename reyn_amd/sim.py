@@ -73,6 +73,7 @@ class UniformGridBuilder:
         self.h = h
         self.search_radius = float(search_radius)
         self._tmp: dict[int, list[DeviceArray]] = {}
+        ctx.adopt(self)
 
     def geometry(self) -> capi.Grid:
         g = capi.Grid()
@@ -125,9 +126,11 @@ class UniformGridBuilder:
         return c, k
 
     def close(self) -> None:
-        if getattr(self, "h", None):
+        if getattr(self, "h", None) and self.ctx.h:
             self.ctx._lib.reyn_grid_builder_destroy(self.h)
-            self.h = None
+        self.h = None
+
+    release = close
 
     def __del__(self):
         try:
@@ -299,11 +302,20 @@ class StepStats:
 class Simulation:
     """The reference's per-step loop (src/main.cu:60-78): time step, grid
     build + reorder (every ``reorder_every`` steps, ``Resort::no`` between),
-    solver step, bounds."""
+    solver step, bounds.
+
+    Time step: ``dt_policy="cfl"`` takes ``cfl_time_step(cfl_lambda)`` every
+    step; ``dt_policy="controller"`` is the reference's main loop
+    (main.cu:59-65): dt starts at ``dt0`` and follows
+    ``simple_dt_controller(iters, dt, cfl_lambda, h, ..., iter_target,
+    change_factor)``."""
 
     def __init__(self, ctx: Context, solver, state: Particles, bdy: BoundarySamples | None, h: float,
                  bound_min, bound_max, cfl_lambda: float = 0.4, cell_ratio: float = 0.5, reorder_every: int = 1,
-                 extra_reorder: tuple[str, ...] = ()):
+                 extra_reorder: tuple[str, ...] = (), dt_policy: str = "cfl", dt0: float = 0.0005,
+                 iter_target: int = 10, change_factor: float = 0.001):
+        if dt_policy not in ("cfl", "controller"):
+            raise ValueError(f"unknown dt policy {dt_policy!r}")
         self.ctx = ctx
         self.solver = solver
         self.state = state
@@ -315,6 +327,11 @@ class Simulation:
         self.builder = UniformGridBuilder(ctx, self.bound_min, self.bound_max, 2.0 * h, cell_ratio)
         self.reorder_every = max(1, int(reorder_every))
         self.extra_reorder = extra_reorder
+        self.dt_policy = dt_policy
+        self.dt = float(dt0)
+        self.iter_target = int(iter_target)
+        self.change_factor = float(change_factor)
+        self.last_iters = 0
         self.steps = 0
         self.time = 0.0
 
@@ -329,13 +346,22 @@ class Simulation:
             return self.builder.construct_and_reorder(r, fields)
         return self.builder.construct(r, self.state.xx, self.state.xy, self.state.xz)
 
+    def next_dt(self) -> float:
+        if self.dt_policy == "cfl":
+            self.dt, _ = cfl_time_step(self.ctx, self.cfl_lambda, self.h, self.state)
+        else:
+            self.dt = simple_dt_controller(self.ctx, self.last_iters, self.dt, self.cfl_lambda, self.h, self.state,
+                                           GRAVITY, self.iter_target, self.change_factor)
+        return self.dt
+
     def step(self) -> StepStats:
-        dt, _ = cfl_time_step(self.ctx, self.cfl_lambda, self.h, self.state)
+        dt = self.next_dt()
         grid = self.get_grid()
         iters = self.solver.step(self.state, grid, self.bdy, dt) or 0
         hard_enforce_bounds(self.ctx, self.bound_min, self.bound_max, self.state)
         self.steps += 1
         self.time += dt
+        self.last_iters = int(iters)
         info = getattr(self.solver, "info", None)
         return StepStats(dt, int(iters), float(info.mean_rel_error) if info else 0.0,
                          int(info.n_violating) if info else 0)

@@ -193,6 +193,7 @@ int main()
             for (int step = 0; step < 5; ++step) {
                 const auto grid = builder.construct_and_reorder(2.f * h, tmp, state);
                 sesph.step(state, grid, bdy, 1e-3f);
+                hard_enforce_bounds(bdy.bound_min, bdy.bound_max, state);
             }
             CHECK(all_finite(state.xy) && all_finite(state.vy));
             CHECK(state.vy.avg() < 0.f);
@@ -205,6 +206,7 @@ int main()
             for (int step = 0; step < 5; ++step) {
                 const auto grid = builder.construct(2.f * h, state.xx, state.xy, state.xz);
                 iters = pcisph.step(state, grid, bdy, 1e-3f);
+                hard_enforce_bounds(bdy.bound_min, bdy.bound_max, state);
             }
             CHECK(iters >= 3);
             CHECK(all_finite(state.xy) && all_finite(rho));
