@@ -136,10 +136,32 @@ def main() -> int:
                     help="controller: the reference main loop's simple_dt_controller from dt0 = 5e-4 "
                          "(lambda 1, 10 target iterations, 0.1%% change per step); cfl: cfl_time_step(lambda)")
     ap.add_argument("--cfl-lambda", type=float, default=None)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run an identity gather of 2^26 floats (reads 512 MiB, writes 256 MiB), the "
+                         "plain copy kernel that scales rocprofv3 FETCH_SIZE/WRITE_SIZE to bytes")
     args = ap.parse_args()
 
     def log(msg):
         print(msg, file=sys.stderr, flush=True)
+
+    if args.copy_calibration:
+        import ctypes as C
+
+        from reyn_amd.device import Context
+        n = 1 << 26
+        with Context(0) as ctx:
+            src = ctx.array(np.arange(n, dtype=np.float32))
+            dst = ctx.empty(n)
+            idx = ctx.array(np.arange(n, dtype=np.uint32))
+            sp = (C.c_void_p * 1)(src.ptr)
+            dp = (C.c_void_p * 1)(dst.ptr)
+            for _ in range(3):
+                ctx.check(ctx._lib.reyn_gather_f32(ctx.h, idx._ptr, n, sp, dp, 1))
+            ctx.synchronize()
+            ok = bool(np.array_equal(dst.numpy()[:: 1 << 16], np.arange(n, dtype=np.float32)[:: 1 << 16]))
+        print(json.dumps({"copy_calibration": {"kernel": "k_gather", "launches": 3, "read_bytes": 8 * n,
+                                               "write_bytes": 4 * n, "ok": ok}}))
+        return 0
 
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); this build's path is "
