@@ -57,6 +57,10 @@ struct Scratch {
     uint32_t* part_u; // per-block counts
     SolveCtl* ctl;
     NbrDev L; // the step's neighbour list (cnt == NULL when not requested)
+    // packed records gathered by the listed loops (with the list only):
+    float4* P4; // {x, y, z, m}, written by the density pass
+    float4* V4; // {vx, vy, vz, 0} after a velocity update
+    float4* A4; // {ax, ay, az, 0} pressure accelerations inside a solver loop
 };
 
 // Neighbours kept per particle; a fluid particle at rest density has about
@@ -69,7 +73,8 @@ int get_scratch(reyn_ctx* ctx, uint32_t n, Scratch* s, bool lists = false)
     auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
     const size_t f = al(size_t(n) * 4);
     const size_t head = 4 * f + al(nb * 8) + al(nb * 4) + al(sizeof(SolveCtl));
-    const size_t bytes = head + (lists ? f + al(size_t(n) * kListCap * 4) : 0);
+    const size_t f4 = al(size_t(n) * 16);
+    const size_t bytes = head + (lists ? f + al(size_t(n) * kListCap * 4) + 3 * f4 : 0);
     char* p = nullptr;
     REYN_TRY(ctx_scratch(ctx, bytes, reinterpret_cast<void**>(&p)));
     s->gbx = reinterpret_cast<float*>(p);
@@ -80,14 +85,29 @@ int get_scratch(reyn_ctx* ctx, uint32_t n, Scratch* s, bool lists = false)
     s->part_u = reinterpret_cast<uint32_t*>(p + 4 * f + al(nb * 8));
     s->ctl = reinterpret_cast<SolveCtl*>(p + 4 * f + al(nb * 8) + al(nb * 4));
     s->L = NbrDev { nullptr, nullptr, n, kListCap };
+    s->P4 = s->V4 = s->A4 = nullptr;
     if (lists) {
         s->L.cnt = reinterpret_cast<uint32_t*>(p + head);
         s->L.idx = reinterpret_cast<uint32_t*>(p + head + f);
+        char* q = p + head + f + al(size_t(n) * kListCap * 4);
+        s->P4 = reinterpret_cast<float4*>(q);
+        s->V4 = reinterpret_cast<float4*>(q + f4);
+        s->A4 = reinterpret_cast<float4*>(q + 2 * f4);
     }
     return REYN_OK;
 }
 
 __device__ __forceinline__ bool loop_done(const SolveCtl* ctl) { return ctl && ctl->done; }
+
+// velocity of particle j from the packed copy when there is one
+__device__ __forceinline__ f3 ldv(const float4* __restrict__ V4, const FluidDev& f, uint32_t j)
+{
+    if (V4) {
+        const float4 q = V4[j];
+        return { q.x, q.y, q.z };
+    }
+    return ld3(f.vx, f.vy, f.vz, j);
+}
 
 // ---- density ------------------------------------------------------------------
 
@@ -96,13 +116,15 @@ __device__ __forceinline__ bool loop_done(const SolveCtl* ctl) { return ctl && c
 template <int K, bool IND, bool GB>
 __global__ __launch_bounds__(kBlock) void k_density(FluidDev f, GridDev g, BdyDev b, KParams kp,
     float* __restrict__ rho, float* __restrict__ gbx, float* __restrict__ gby,
-    float* __restrict__ gbz, float* __restrict__ rb, NbrDev L)
+    float* __restrict__ gbz, float* __restrict__ rb, NbrDev L, float4* __restrict__ P4)
 {
     const uint32_t t = global_index();
     if (t >= f.n)
         return;
     const uint32_t i = particle_of<IND>(g, t);
     const f3 xi = ld3(f.x, f.y, f.z, i);
+    if (P4)
+        P4[i] = make_float4(xi.x, xi.y, xi.z, f.m[i]);
     ksum acc;
     uint32_t cnt = 0;
     uint32_t* __restrict__ col = L.cnt ? L.idx + t : nullptr;
@@ -142,7 +164,7 @@ __global__ __launch_bounds__(kBlock) void k_density(FluidDev f, GridDev g, BdyDe
 template <int K, bool IND>
 __global__ __launch_bounds__(kBlock) void k_nonpressure(FluidDev f, GridDev g, SphK c,
     const float* __restrict__ rho, float* __restrict__ ax, float* __restrict__ ay,
-    float* __restrict__ az, NbrDev L)
+    float* __restrict__ az, NbrDev L, const float4* __restrict__ P4)
 {
     const uint32_t t = global_index();
     if (t >= f.n)
@@ -151,9 +173,9 @@ __global__ __launch_bounds__(kBlock) void k_nonpressure(FluidDev f, GridDev g, S
     const f3 xi = ld3(f.x, f.y, f.z, i);
     const f3 vi = ld3(f.vx, f.vy, f.vz, i);
     f3 acc = c.g;
-    nbrs<IND>(g, L, t, xi, f.x, f.y, f.z, [&](uint32_t j, f3 xij, float d2) {
+    nbrs4<IND>(g, L, t, xi, P4, f.x, f.y, f.z, f.m, [&](uint32_t j, f3 xij, float d2, float mj) {
         const f3 vij = vi - ld3(f.vx, f.vy, f.vz, j);
-        const float s = c.visc_c * f.m[j] / rho[j] * dot(vij, xij) / (d2 + c.visc_eps);
+        const float s = c.visc_c * mj / rho[j] * dot(vij, xij) / (d2 + c.visc_eps);
         acc += s * kdW<K>(c.kp, xij, d2);
     });
     st3(ax, ay, az, i, acc);
@@ -163,7 +185,7 @@ __global__ __launch_bounds__(kBlock) void k_nonpressure(FluidDev f, GridDev g, S
 
 __global__ __launch_bounds__(kBlock) void k_integrate(FluidDev f, const float* __restrict__ ax,
     const float* __restrict__ ay, const float* __restrict__ az, float dt, const SolveCtl* ctl,
-    int positions)
+    int positions, float4* __restrict__ V4)
 {
     if (loop_done(ctl))
         return;
@@ -172,6 +194,8 @@ __global__ __launch_bounds__(kBlock) void k_integrate(FluidDev f, const float* _
         return;
     const f3 v = ld3(f.vx, f.vy, f.vz, i) + dt * ld3(ax, ay, az, i);
     st3(f.vx, f.vy, f.vz, i, v);
+    if (V4)
+        V4[i] = make_float4(v.x, v.y, v.z, 0.f);
     if (positions)
         st3(f.x, f.y, f.z, i, ld3(f.x, f.y, f.z, i) + dt * v);
 }
@@ -219,7 +243,8 @@ template <int K, bool IND>
 __global__ __launch_bounds__(kBlock) void k_predict_rho(FluidDev f, GridDev g, int bdy, KParams kp,
     float dt, const float* __restrict__ gbx, const float* __restrict__ gby,
     const float* __restrict__ gbz, const float* __restrict__ rb, float* __restrict__ rho,
-    double* __restrict__ part, const SolveCtl* ctl, NbrDev L)
+    double* __restrict__ part, const SolveCtl* ctl, NbrDev L, const float4* __restrict__ P4,
+    const float4* __restrict__ V4)
 {
     __shared__ double lds[kBlock / kWave];
     if (loop_done(ctl))
@@ -231,9 +256,9 @@ __global__ __launch_bounds__(kBlock) void k_predict_rho(FluidDev f, GridDev g, i
         const f3 xi = ld3(f.x, f.y, f.z, i);
         const f3 vi = ld3(f.vx, f.vy, f.vz, i);
         ksum acc;
-        const bool listed = nbrs<IND>(g, L, t, xi, f.x, f.y, f.z, [&](uint32_t j, f3 xij, float d2) {
-            const f3 vij = vi - ld3(f.vx, f.vy, f.vz, j);
-            acc.add(f.m[j] * (kW<K>(kp, d2) + dt * dot(kdW<K>(kp, xij, d2), vij)));
+        const bool listed = nbrs4<IND>(g, L, t, xi, P4, f.x, f.y, f.z, f.m, [&](uint32_t j, f3 xij, float d2, float mj) {
+            const f3 vij = vi - ldv(V4, f, j);
+            acc.add(mj * (kW<K>(kp, d2) + dt * dot(kdW<K>(kp, xij, d2), vij)));
         });
         if (listed) // the list skips the particle itself: m_i (W(0) + dt grad W(0) . 0)
             acc.add(f.m[i] * kW<K>(kp, 0.f));
@@ -252,7 +277,8 @@ template <int K, bool IND>
 __global__ __launch_bounds__(kBlock) void k_pcisph_prs_acc(FluidDev f, GridDev g, int bdy,
     KParams kp, float k, float rho0, float rho0_sq_inv, const float* __restrict__ rho,
     const float* __restrict__ gbx, const float* __restrict__ gby, const float* __restrict__ gbz,
-    float* __restrict__ ax, float* __restrict__ ay, float* __restrict__ az, const SolveCtl* ctl, NbrDev L)
+    float* __restrict__ ax, float* __restrict__ ay, float* __restrict__ az, const SolveCtl* ctl, NbrDev L,
+    const float4* __restrict__ P4)
 {
     if (loop_done(ctl))
         return;
@@ -266,10 +292,10 @@ __global__ __launch_bounds__(kBlock) void k_pcisph_prs_acc(FluidDev f, GridDev g
     const float pri = pi / (rhoi * rhoi);
     const float pr0 = pi * rho0_sq_inv;
     f3 acc { 0.f, 0.f, 0.f };
-    nbrs<IND>(g, L, t, xi, f.x, f.y, f.z, [&](uint32_t j, f3 xij, float d2) {
+    nbrs4<IND>(g, L, t, xi, P4, f.x, f.y, f.z, f.m, [&](uint32_t j, f3 xij, float d2, float mj) {
         const float rhoj = rho[j];
         const float pj = fmaxf(0.f, k * (rhoj - rho0));
-        acc += (f.m[j] * (pri + pj / (rhoj * rhoj))) * kdW<K>(kp, xij, d2);
+        acc += (mj * (pri + pj / (rhoj * rhoj))) * kdW<K>(kp, xij, d2);
     });
     f3 a = -acc;
     if (bdy)
@@ -311,7 +337,8 @@ template <int K, bool IND>
 __global__ __launch_bounds__(kBlock) void k_iisph_aii_s(FluidDev f, GridDev g, int bdy, KParams kp,
     float rho0, float dt, float omega, int warm, const float* __restrict__ rho,
     const float* __restrict__ gbx, const float* __restrict__ gby, const float* __restrict__ gbz,
-    float* __restrict__ aii, float* __restrict__ si, float* __restrict__ p, NbrDev L)
+    float* __restrict__ aii, float* __restrict__ si, float* __restrict__ p, NbrDev L,
+    const float4* __restrict__ P4, const float4* __restrict__ V4)
 {
     const uint32_t t = global_index();
     if (t >= f.n)
@@ -321,10 +348,10 @@ __global__ __launch_bounds__(kBlock) void k_iisph_aii_s(FluidDev f, GridDev g, i
     const f3 vi = ld3(f.vx, f.vy, f.vz, i);
     ksum3 grad;
     ksum sq, div;
-    nbrs<IND>(g, L, t, xi, f.x, f.y, f.z, [&](uint32_t j, f3 xij, float d2) {
+    nbrs4<IND>(g, L, t, xi, P4, f.x, f.y, f.z, f.m, [&](uint32_t j, f3 xij, float d2, float mj) {
         const f3 dW = kdW<K>(kp, xij, d2);
-        const f3 mdW = f.m[j] * dW;
-        div.add(dot(mdW, vi - ld3(f.vx, f.vy, f.vz, j)));
+        const f3 mdW = mj * dW;
+        div.add(dot(mdW, vi - ldv(V4, f, j)));
         sq.add(dot(mdW, dW));
         grad.add(mdW);
     });
@@ -339,11 +366,14 @@ __global__ __launch_bounds__(kBlock) void k_iisph_aii_s(FluidDev f, GridDev g, i
     p[i] = warm ? p[i] * 0.5f : jacobi_update(omega, a, 0.f, s);
 }
 
+// Pressure accelerations; into the packed A4 inside the loop, into the SoA
+// work arrays for the final pass (A4 == NULL).
 template <int K, bool IND>
 __global__ __launch_bounds__(kBlock) void k_iisph_pacc(FluidDev f, GridDev g, int bdy, KParams kp,
     float rho0_sq_inv, const float* __restrict__ p, const float* __restrict__ gbx,
     const float* __restrict__ gby, const float* __restrict__ gbz, float* __restrict__ ax,
-    float* __restrict__ ay, float* __restrict__ az, const SolveCtl* ctl, NbrDev L)
+    float* __restrict__ ay, float* __restrict__ az, const SolveCtl* ctl, NbrDev L,
+    const float4* __restrict__ P4, float4* __restrict__ A4)
 {
     if (loop_done(ctl))
         return;
@@ -354,20 +384,24 @@ __global__ __launch_bounds__(kBlock) void k_iisph_pacc(FluidDev f, GridDev g, in
     const f3 xi = ld3(f.x, f.y, f.z, i);
     const float pi = p[i];
     f3 acc { 0.f, 0.f, 0.f };
-    nbrs<IND>(g, L, t, xi, f.x, f.y, f.z,
-        [&](uint32_t j, f3 xij, float d2) { acc += (f.m[j] * (pi + p[j])) * kdW<K>(kp, xij, d2); });
+    nbrs4<IND>(g, L, t, xi, P4, f.x, f.y, f.z, f.m, [&](uint32_t j, f3 xij, float d2, float mj) {
+        acc += (mj * (pi + p[j])) * kdW<K>(kp, xij, d2);
+    });
     if (bdy)
         acc += pi * ld3(gbx, gby, gbz, i);
-    st3(ax, ay, az, i, -rho0_sq_inv * acc);
+    const f3 a = -rho0_sq_inv * acc;
+    if (A4)
+        A4[i] = make_float4(a.x, a.y, a.z, 0.f);
+    else
+        st3(ax, ay, az, i, a);
 }
 
 template <int K, bool IND>
 __global__ __launch_bounds__(kBlock) void k_iisph_jacobi(FluidDev f, GridDev g, int bdy, KParams kp,
-    float dt, float omega, float thr, const float* __restrict__ ax, const float* __restrict__ ay,
-    const float* __restrict__ az, const float* __restrict__ aii, const float* __restrict__ si,
-    const float* __restrict__ gbx, const float* __restrict__ gby, const float* __restrict__ gbz,
-    float* __restrict__ p, double* __restrict__ part_err, uint32_t* __restrict__ part_cnt,
-    const SolveCtl* ctl, NbrDev L)
+    float dt, float omega, float thr, const float4* __restrict__ A4, const float* __restrict__ aii,
+    const float* __restrict__ si, const float* __restrict__ gbx, const float* __restrict__ gby,
+    const float* __restrict__ gbz, float* __restrict__ p, double* __restrict__ part_err,
+    uint32_t* __restrict__ part_cnt, const SolveCtl* ctl, NbrDev L, const float4* __restrict__ P4)
 {
     __shared__ double lds[kBlock / kWave];
     if (loop_done(ctl))
@@ -378,10 +412,12 @@ __global__ __launch_bounds__(kBlock) void k_iisph_jacobi(FluidDev f, GridDev g, 
     if (t < f.n) {
         const uint32_t i = particle_of<IND>(g, t);
         const f3 xi = ld3(f.x, f.y, f.z, i);
-        const f3 ai = ld3(ax, ay, az, i);
+        const float4 a4 = A4[i];
+        const f3 ai { a4.x, a4.y, a4.z };
         float lap = 0.f;
-        nbrs<IND>(g, L, t, xi, f.x, f.y, f.z, [&](uint32_t j, f3 xij, float d2) {
-            lap += f.m[j] * dot(ai - ld3(ax, ay, az, j), kdW<K>(kp, xij, d2));
+        nbrs4<IND>(g, L, t, xi, P4, f.x, f.y, f.z, f.m, [&](uint32_t j, f3 xij, float d2, float mj) {
+            const float4 aj = A4[j];
+            lap += mj * dot(ai - f3 { aj.x, aj.y, aj.z }, kdW<K>(kp, xij, d2));
         });
         if (bdy)
             lap += dot(ai, ld3(gbx, gby, gbz, i));
@@ -485,7 +521,7 @@ int launch_density(reyn_ctx* ctx, int kernel, const Common& c, const KParams& kp
             ProfScope _prof(ctx, "k_density");
             k_density<K, I, GB><<<c.nb, kBlock, 0, ctx->stream>>>(c.f, c.g, c.b, kp, rho,
                 GB ? s->gbx : nullptr, GB ? s->gby : nullptr, GB ? s->gbz : nullptr, GB ? s->rb : nullptr,
-                s ? s->L : NbrDev {});
+                s ? s->L : NbrDev {}, s ? s->P4 : nullptr);
         }
         REYN_LAUNCHED(ctx, "k_density");
         return REYN_OK;
@@ -493,14 +529,14 @@ int launch_density(reyn_ctx* ctx, int kernel, const Common& c, const KParams& kp
 }
 
 int launch_nonpressure(reyn_ctx* ctx, int kernel, const Common& c, const SphK& k, const float* rho,
-    float* ax, float* ay, float* az, const NbrDev& L)
+    float* ax, float* ay, float* az, const NbrDev& L, const float4* P4)
 {
     return with_kernel(ctx, kernel, c.ind, [&](auto KC, auto IC) -> int {
         constexpr int K = decltype(KC)::value;
         constexpr bool I = decltype(IC)::value;
         {
             ProfScope _prof(ctx, "k_nonpressure");
-            k_nonpressure<K, I><<<c.nb, kBlock, 0, ctx->stream>>>(c.f, c.g, k, rho, ax, ay, az, L);
+            k_nonpressure<K, I><<<c.nb, kBlock, 0, ctx->stream>>>(c.f, c.g, k, rho, ax, ay, az, L, P4);
         }
         REYN_LAUNCHED(ctx, "k_nonpressure");
         return REYN_OK;
@@ -508,11 +544,11 @@ int launch_nonpressure(reyn_ctx* ctx, int kernel, const Common& c, const SphK& k
 }
 
 int launch_integrate(reyn_ctx* ctx, const Common& c, const float* ax, const float* ay,
-    const float* az, float dt, const SolveCtl* ctl, int positions)
+    const float* az, float dt, const SolveCtl* ctl, int positions, float4* V4 = nullptr)
 {
     {
         ProfScope _prof(ctx, positions ? "k_integrate" : "k_integrate_v");
-        k_integrate<<<c.nb, kBlock, 0, ctx->stream>>>(c.f, ax, ay, az, dt, ctl, positions);
+        k_integrate<<<c.nb, kBlock, 0, ctx->stream>>>(c.f, ax, ay, az, dt, ctl, positions, V4);
     }
     REYN_LAUNCHED(ctx, "k_integrate");
     return REYN_OK;
@@ -580,7 +616,7 @@ int reyn_nonpressure_acc(reyn_ctx* ctx, const reyn_sph_params* prm, const reyn_p
         return REYN_OK;
     if (!rho || !ax || !ay || !az)
         return set_error(ctx, REYN_ERR_INVALID_ARGUMENT, "nonpressure_acc: NULL buffer");
-    return launch_nonpressure(ctx, prm->kernel, c, make_sphk(*prm), rho, ax, ay, az, NbrDev {});
+    return launch_nonpressure(ctx, prm->kernel, c, make_sphk(*prm), rho, ax, ay, az, NbrDev {}, nullptr);
 }
 
 int reyn_predict_rho(reyn_ctx* ctx, const reyn_sph_params* prm, const reyn_particles* fluid,
@@ -605,7 +641,7 @@ int reyn_predict_rho(reyn_ctx* ctx, const reyn_sph_params* prm, const reyn_parti
         {
             ProfScope _prof(ctx, "k_predict_rho");
             k_predict_rho<K, I><<<c.nb, kBlock, 0, ctx->stream>>>(c.f, c.g, c.b.present, kp, dt, s.gbx,
-                s.gby, s.gbz, s.rb, rho, nullptr, nullptr, NbrDev {});
+                s.gby, s.gbz, s.rb, rho, nullptr, nullptr, NbrDev {}, nullptr, nullptr);
         }
         REYN_LAUNCHED(ctx, "k_predict_rho");
         return REYN_OK;
@@ -750,26 +786,26 @@ int reyn_pcisph_step(reyn_ctx* ctx, const reyn_pcisph_params* prm, const reyn_pa
     const float rho0 = prm->sph.rho_0;
     const float thr = (prm->eta_rho_max + 1.f) * rho0;
     REYN_TRY(launch_density<true>(ctx, prm->sph.kernel, c, k.kp, work->rho, &s));
-    REYN_TRY(launch_nonpressure(ctx, prm->sph.kernel, c, k, work->rho, work->ax, work->ay, work->az, s.L));
+    REYN_TRY(launch_nonpressure(ctx, prm->sph.kernel, c, k, work->rho, work->ax, work->ay, work->az, s.L, s.P4));
     SolveCtl res;
     REYN_TRY(run_loop(
         ctx, s.ctl, prm->min_iter, prm->max_iter, 4,
         [&]() -> int {
-            REYN_TRY(launch_integrate(ctx, c, work->ax, work->ay, work->az, dt, s.ctl, 0));
+            REYN_TRY(launch_integrate(ctx, c, work->ax, work->ay, work->az, dt, s.ctl, 0, s.V4));
             REYN_TRY(with_kernel(ctx, prm->sph.kernel, c.ind, [&](auto KC, auto IC) -> int {
                 constexpr int K = decltype(KC)::value;
                 constexpr bool I = decltype(IC)::value;
                 {
                     ProfScope _prof(ctx, "k_predict_rho");
                     k_predict_rho<K, I><<<c.nb, kBlock, 0, ctx->stream>>>(c.f, c.g, c.b.present, k.kp, dt,
-                        s.gbx, s.gby, s.gbz, s.rb, work->rho, s.part_d, s.ctl, s.L);
+                        s.gbx, s.gby, s.gbz, s.rb, work->rho, s.part_d, s.ctl, s.L, s.P4, s.V4);
                 }
                 REYN_LAUNCHED(ctx, "k_predict_rho");
                 {
                     ProfScope _prof(ctx, "k_pcisph_prs_acc");
                     k_pcisph_prs_acc<K, I><<<c.nb, kBlock, 0, ctx->stream>>>(c.f, c.g, c.b.present, k.kp, kk,
                         rho0, 1.f / (rho0 * rho0), work->rho, s.gbx, s.gby, s.gbz, work->ax, work->ay,
-                        work->az, s.ctl, s.L);
+                        work->az, s.ctl, s.L, s.P4);
                 }
                 REYN_LAUNCHED(ctx, "k_pcisph_prs_acc");
                 return REYN_OK;
@@ -818,15 +854,15 @@ int reyn_iisph_step(reyn_ctx* ctx, const reyn_iisph_params* prm, const reyn_part
     const int kind = prm->sph.kernel;
     const int bd = c.b.present;
     REYN_TRY(launch_density<true>(ctx, kind, c, k.kp, work->rho, &s));
-    REYN_TRY(launch_nonpressure(ctx, kind, c, k, work->rho, work->ax, work->ay, work->az, s.L));
-    REYN_TRY(launch_integrate(ctx, c, work->ax, work->ay, work->az, dt, nullptr, 0));
+    REYN_TRY(launch_nonpressure(ctx, kind, c, k, work->rho, work->ax, work->ay, work->az, s.L, s.P4));
+    REYN_TRY(launch_integrate(ctx, c, work->ax, work->ay, work->az, dt, nullptr, 0, s.V4));
     REYN_TRY(with_kernel(ctx, kind, c.ind, [&](auto KC, auto IC) -> int {
         constexpr int K = decltype(KC)::value;
         constexpr bool I = decltype(IC)::value;
         {
             ProfScope _prof(ctx, "k_iisph_aii_s");
             k_iisph_aii_s<K, I><<<c.nb, kBlock, 0, ctx->stream>>>(c.f, c.g, bd, k.kp, rho0, dt, prm->omega,
-                prm->warm_start, work->rho, s.gbx, s.gby, s.gbz, work->a_ii, work->s_i, work->p, s.L);
+                prm->warm_start, work->rho, s.gbx, s.gby, s.gbz, work->a_ii, work->s_i, work->p, s.L, s.P4, s.V4);
         }
         REYN_LAUNCHED(ctx, "k_iisph_aii_s");
         return REYN_OK;
@@ -838,7 +874,7 @@ int reyn_iisph_step(reyn_ctx* ctx, const reyn_iisph_params* prm, const reyn_part
             {
                 ProfScope _prof(ctx, "k_iisph_pacc");
                 k_iisph_pacc<K, I><<<c.nb, kBlock, 0, ctx->stream>>>(c.f, c.g, bd, k.kp, rho0_sq_inv,
-                    work->p, s.gbx, s.gby, s.gbz, work->ax, work->ay, work->az, ctl, s.L);
+                    work->p, s.gbx, s.gby, s.gbz, work->ax, work->ay, work->az, ctl, s.L, s.P4, ctl ? s.A4 : nullptr);
             }
             REYN_LAUNCHED(ctx, "k_iisph_pacc");
             return REYN_OK;
@@ -855,8 +891,8 @@ int reyn_iisph_step(reyn_ctx* ctx, const reyn_iisph_params* prm, const reyn_part
                 {
                     ProfScope _prof(ctx, "k_iisph_jacobi");
                     k_iisph_jacobi<K, I><<<c.nb, kBlock, 0, ctx->stream>>>(c.f, c.g, bd, k.kp, dt, prm->omega,
-                        thr, work->ax, work->ay, work->az, work->a_ii, work->s_i, s.gbx, s.gby, s.gbz, work->p,
-                        s.part_d, s.part_u, s.ctl, s.L);
+                        thr, s.A4, work->a_ii, work->s_i, s.gbx, s.gby, s.gbz, work->p,
+                        s.part_d, s.part_u, s.ctl, s.L, s.P4);
                 }
                 REYN_LAUNCHED(ctx, "k_iisph_jacobi");
                 return REYN_OK;

@@ -250,6 +250,32 @@ __device__ __forceinline__ bool nbrs(const GridDev& g, const NbrDev& L, uint32_t
     return false;
 }
 
+// As `nbrs`, over packed records P[j] = {x, y, z, m}: one 16-byte load per
+// listed neighbour instead of one load per SoA field; f(j, x_ij, d2, m_j).
+// The walk fallback reads the SoA arrays, which hold the same values.
+template <bool IND, class F>
+__device__ __forceinline__ bool nbrs4(const GridDev& g, const NbrDev& L, uint32_t t, f3 xi,
+    const float4* __restrict__ P, const float* __restrict__ xs, const float* __restrict__ ys,
+    const float* __restrict__ zs, const float* __restrict__ ms, F&& f)
+{
+    if (L.cnt) {
+        const uint32_t c = L.cnt[t];
+        if (c <= L.cap) {
+            const uint32_t* __restrict__ col = L.idx + t;
+#pragma unroll 2
+            for (uint32_t k = 0; k < c; ++k) {
+                const uint32_t j = col[size_t(k) * L.n];
+                const float4 q = P[j];
+                const f3 xij = xi - f3 { q.x, q.y, q.z };
+                f(j, xij, dot(xij, xij), q.w);
+            }
+            return true;
+        }
+    }
+    walk<IND>(g, xi, xs, ys, zs, [&](uint32_t j, f3 xij, float d2) { f(j, xij, d2, ms[j]); });
+    return false;
+}
+
 // ---- particle views ----------------------------------------------------------
 
 struct FluidDev {
