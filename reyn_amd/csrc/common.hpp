@@ -44,6 +44,20 @@ __device__ __forceinline__ f3 ld3(const float* __restrict__ x, const float* __re
 {
     return { x[i], y[i], z[i] };
 }
+
+// Gathers through a 32-bit byte offset from a uniform base: the compiler
+// emits the scalar-base form of global loads (one offset VGPR, no 64-bit
+// address arithmetic). Valid while the byte offset fits in 32 bits, which the
+// entry points check (arrays below 4 GiB).
+template <class T> __device__ __forceinline__ T gat(const T* __restrict__ base, uint32_t i)
+{
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + static_cast<uint32_t>(i * sizeof(T)));
+}
+__device__ __forceinline__ f3 gat3(const float* __restrict__ x, const float* __restrict__ y,
+    const float* __restrict__ z, uint32_t i)
+{
+    return { gat(x, i), gat(y, i), gat(z, i) };
+}
 __device__ __forceinline__ void st3(float* __restrict__ x, float* __restrict__ y,
     float* __restrict__ z, uint32_t i, f3 v)
 {

@@ -470,8 +470,8 @@ int reyn_grid_builder_create(reyn_ctx* ctx, reyn_float3 bmin, reyn_float3 bmax,
         n[a] = static_cast<int>(e);
     }
     const double cells = double(n[0]) * double(n[1]) * double(n[2]);
-    if (cells > 2.0e9)
-        return set_error(ctx, REYN_ERR_CAPACITY, "grid: %.0f cells exceed the 32-bit key range", cells);
+    if (cells >= 1073741824.0)
+        return set_error(ctx, REYN_ERR_CAPACITY, "grid: %.0f cells exceed the 2^30 cell limit", cells);
     // cells searched on each side: ceil(radius / cell)
     const int range = static_cast<int>(std::ceil(1.0 / double(cell_ratio) - 1e-6));
     reyn_grid_builder* b = new (std::nothrow) reyn_grid_builder();

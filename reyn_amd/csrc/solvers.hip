@@ -74,6 +74,8 @@ int get_scratch(reyn_ctx* ctx, uint32_t n, Scratch* s, bool lists = false)
     const size_t f = al(size_t(n) * 4);
     const size_t head = 4 * f + al(nb * 8) + al(nb * 4) + al(sizeof(SolveCtl));
     const size_t f4 = al(size_t(n) * 16);
+    // list entries are addressed with 32-bit byte offsets
+    const bool keep_list = lists && size_t(n) * kListCap * 4 < (size_t(1) << 32);
     const size_t bytes = head + (lists ? f + al(size_t(n) * kListCap * 4) + 3 * f4 : 0);
     char* p = nullptr;
     REYN_TRY(ctx_scratch(ctx, bytes, reinterpret_cast<void**>(&p)));
@@ -87,8 +89,10 @@ int get_scratch(reyn_ctx* ctx, uint32_t n, Scratch* s, bool lists = false)
     s->L = NbrDev { nullptr, nullptr, n, kListCap };
     s->P4 = s->V4 = s->A4 = nullptr;
     if (lists) {
-        s->L.cnt = reinterpret_cast<uint32_t*>(p + head);
-        s->L.idx = reinterpret_cast<uint32_t*>(p + head + f);
+        if (keep_list) {
+            s->L.cnt = reinterpret_cast<uint32_t*>(p + head);
+            s->L.idx = reinterpret_cast<uint32_t*>(p + head + f);
+        }
         char* q = p + head + f + al(size_t(n) * kListCap * 4);
         s->P4 = reinterpret_cast<float4*>(q);
         s->V4 = reinterpret_cast<float4*>(q + f4);
@@ -103,13 +107,23 @@ __device__ __forceinline__ bool loop_done(const SolveCtl* ctl) { return ctl && c
 __device__ __forceinline__ f3 ldv(const float4* __restrict__ V4, const FluidDev& f, uint32_t j)
 {
     if (V4) {
-        const float4 q = V4[j];
+        const float4 q = gat(V4, j);
         return { q.x, q.y, q.z };
     }
-    return ld3(f.vx, f.vy, f.vz, j);
+    return gat3(f.vx, f.vy, f.vz, j);
 }
 
 // ---- density ------------------------------------------------------------------
+
+// P4[i] = {x_i, y_i, z_i, m_i}: the packed records every walk and list loop of
+// the step gathers.
+__global__ __launch_bounds__(kBlock) void k_pack(FluidDev f, float4* __restrict__ P4)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= f.n)
+        return;
+    P4[i] = make_float4(f.x[i], f.y[i], f.z[i], f.m[i]);
+}
 
 // Densities; with GB also the boundary sums, and with a list in L the
 // neighbour list of the step (every later fluid loop reads it).
@@ -123,20 +137,23 @@ __global__ __launch_bounds__(kBlock) void k_density(FluidDev f, GridDev g, BdyDe
         return;
     const uint32_t i = particle_of<IND>(g, t);
     const f3 xi = ld3(f.x, f.y, f.z, i);
-    if (P4)
-        P4[i] = make_float4(xi.x, xi.y, xi.z, f.m[i]);
     ksum acc;
     uint32_t cnt = 0;
-    uint32_t* __restrict__ col = L.cnt ? L.idx + t : nullptr;
-    walk<IND>(g, xi, f.x, f.y, f.z, [&](uint32_t j, f3, float d2) {
-        acc.add(f.m[j] * kW<K>(kp, d2));
-        if (col && j != i) {
+    uint32_t off = t;
+    auto visit = [&](uint32_t j, float d2, float mj) {
+        acc.add(mj * kW<K>(kp, d2));
+        if (L.cnt && j != i) {
             if (cnt < L.cap)
-                col[size_t(cnt) * L.n] = j;
+                L.idx[off] = j;
+            off += L.n;
             ++cnt;
         }
-    });
-    if (col)
+    };
+    if (P4)
+        walk4<IND>(g, xi, P4, [&](uint32_t j, f3, float d2, float mj) { visit(j, d2, mj); });
+    else
+        walk<IND>(g, xi, f.x, f.y, f.z, [&](uint32_t j, f3, float d2) { visit(j, d2, gat(f.m, j)); });
+    if (L.cnt)
         L.cnt[t] = cnt;
     ksum accb;
     ksum3 grad;
@@ -174,8 +191,8 @@ __global__ __launch_bounds__(kBlock) void k_nonpressure(FluidDev f, GridDev g, S
     const f3 vi = ld3(f.vx, f.vy, f.vz, i);
     f3 acc = c.g;
     nbrs4<IND>(g, L, t, xi, P4, f.x, f.y, f.z, f.m, [&](uint32_t j, f3 xij, float d2, float mj) {
-        const f3 vij = vi - ld3(f.vx, f.vy, f.vz, j);
-        const float s = c.visc_c * mj / rho[j] * dot(vij, xij) / (d2 + c.visc_eps);
+        const f3 vij = vi - gat3(f.vx, f.vy, f.vz, j);
+        const float s = c.visc_c * mj / gat(rho, j) * dot(vij, xij) / (d2 + c.visc_eps);
         acc += s * kdW<K>(c.kp, xij, d2);
     });
     st3(ax, ay, az, i, acc);
@@ -211,7 +228,8 @@ template <int K, bool IND>
 __global__ __launch_bounds__(kBlock) void k_sesph_acc(FluidDev f, GridDev g, int bdy, SphK c,
     float k, const float* __restrict__ rho, const float* __restrict__ gbx,
     const float* __restrict__ gby, const float* __restrict__ gbz, float* __restrict__ ax,
-    float* __restrict__ ay, float* __restrict__ az, NbrDev L)
+    float* __restrict__ ay, float* __restrict__ az, NbrDev L,
+    const float4* __restrict__ P4)
 {
     const uint32_t t = global_index();
     if (t >= f.n)
@@ -222,10 +240,9 @@ __global__ __launch_bounds__(kBlock) void k_sesph_acc(FluidDev f, GridDev g, int
     const float rhoi = rho[i];
     const float pri = eos_sesph(rhoi, k, c.rho0) / (rhoi * rhoi);
     f3 acc = c.g;
-    nbrs<IND>(g, L, t, xi, f.x, f.y, f.z, [&](uint32_t j, f3 xij, float d2) {
-        const float rhoj = rho[j];
-        const float mj = f.m[j];
-        const f3 vij = vi - ld3(f.vx, f.vy, f.vz, j);
+    nbrs4<IND>(g, L, t, xi, P4, f.x, f.y, f.z, f.m, [&](uint32_t j, f3 xij, float d2, float mj) {
+        const float rhoj = gat(rho, j);
+        const f3 vij = vi - gat3(f.vx, f.vy, f.vz, j);
         const f3 dW = kdW<K>(c.kp, xij, d2);
         const float pj = eos_sesph(rhoj, k, c.rho0);
         const float visc = c.visc_c * mj / rhoj * dot(vij, xij) / (d2 + c.visc_eps);
@@ -293,7 +310,7 @@ __global__ __launch_bounds__(kBlock) void k_pcisph_prs_acc(FluidDev f, GridDev g
     const float pr0 = pi * rho0_sq_inv;
     f3 acc { 0.f, 0.f, 0.f };
     nbrs4<IND>(g, L, t, xi, P4, f.x, f.y, f.z, f.m, [&](uint32_t j, f3 xij, float d2, float mj) {
-        const float rhoj = rho[j];
+        const float rhoj = gat(rho, j);
         const float pj = fmaxf(0.f, k * (rhoj - rho0));
         acc += (mj * (pri + pj / (rhoj * rhoj))) * kdW<K>(kp, xij, d2);
     });
@@ -385,7 +402,7 @@ __global__ __launch_bounds__(kBlock) void k_iisph_pacc(FluidDev f, GridDev g, in
     const float pi = p[i];
     f3 acc { 0.f, 0.f, 0.f };
     nbrs4<IND>(g, L, t, xi, P4, f.x, f.y, f.z, f.m, [&](uint32_t j, f3 xij, float d2, float mj) {
-        acc += (mj * (pi + p[j])) * kdW<K>(kp, xij, d2);
+        acc += (mj * (pi + gat(p, j))) * kdW<K>(kp, xij, d2);
     });
     if (bdy)
         acc += pi * ld3(gbx, gby, gbz, i);
@@ -416,7 +433,7 @@ __global__ __launch_bounds__(kBlock) void k_iisph_jacobi(FluidDev f, GridDev g, 
         const f3 ai { a4.x, a4.y, a4.z };
         float lap = 0.f;
         nbrs4<IND>(g, L, t, xi, P4, f.x, f.y, f.z, f.m, [&](uint32_t j, f3 xij, float d2, float mj) {
-            const float4 aj = A4[j];
+            const float4 aj = gat(A4, j);
             lap += mj * dot(ai - f3 { aj.x, aj.y, aj.z }, kdW<K>(kp, xij, d2));
         });
         if (bdy)
@@ -501,6 +518,8 @@ int prepare(reyn_ctx* ctx, const reyn_sph_params* prm, const reyn_particles* flu
     REYN_TRY(check_fluid(ctx, fluid));
     REYN_TRY(check_grid(ctx, grid, fluid->n, "fluid"));
     REYN_TRY(check_bdy(ctx, bdy));
+    if (fluid->n >= (1u << 28))
+        return set_error(ctx, REYN_ERR_CAPACITY, "at most 2^28 - 1 fluid particles per call (got %u)", fluid->n);
     REYN_HIP(ctx, hipSetDevice(ctx->device));
     c->f = to_dev(*fluid);
     c->g = to_dev(*grid);
@@ -514,6 +533,13 @@ template <bool GB>
 int launch_density(reyn_ctx* ctx, int kernel, const Common& c, const KParams& kp, float* rho,
     const Scratch* s)
 {
+    if (s && s->P4) {
+        {
+            ProfScope _prof(ctx, "k_pack");
+            k_pack<<<c.nb, kBlock, 0, ctx->stream>>>(c.f, s->P4);
+        }
+        REYN_LAUNCHED(ctx, "k_pack");
+    }
     return with_kernel(ctx, kernel, c.ind, [&](auto KC, auto IC) -> int {
         constexpr int K = decltype(KC)::value;
         constexpr bool I = decltype(IC)::value;
@@ -697,7 +723,7 @@ int reyn_sesph_step(reyn_ctx* ctx, const reyn_sesph_params* prm, const reyn_part
         {
             ProfScope _prof(ctx, "k_sesph_acc");
             k_sesph_acc<K, I><<<c.nb, kBlock, 0, ctx->stream>>>(c.f, c.g, c.b.present, k, prm->k,
-                work->rho, s.gbx, s.gby, s.gbz, work->ax, work->ay, work->az, s.L);
+                work->rho, s.gbx, s.gby, s.gbz, work->ax, work->ay, work->az, s.L, s.P4);
         }
         REYN_LAUNCHED(ctx, "k_sesph_acc");
         return REYN_OK;

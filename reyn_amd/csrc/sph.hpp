@@ -111,19 +111,23 @@ template <> struct Shape<KCOS> {
     }
 };
 
-// W from the squared distance (the neighbour walk already formed it).
+// W from the squared distance (the neighbour walk already formed it). |x| and
+// 1/|x| come from one hardware reciprocal square root (v_rsq_f32, about 1 ulp)
+// instead of a correctly rounded sqrt and divide; the results stay within the
+// FP32 rounding the solver parity bar allows.
 template <int K> __device__ __forceinline__ float kW(const KParams& p, float d2)
 {
-    const float len = __builtin_sqrtf(d2);
+    const float len = d2 > 0.f ? d2 * __builtin_amdgcn_rsqf(d2) : 0.f;
     return p.ws * Shape<K>::w(sat(len * p.hinv));
 }
 
-// grad W from the offset and its squared length.
+// grad W from the offset and its squared length; the zero vector at x = 0
+// (and wherever the scale is not finite), as in the reference.
 template <int K> __device__ __forceinline__ f3 kdW(const KParams& p, f3 x, float d2)
 {
-    const float len = __builtin_sqrtf(d2);
-    const float scale = p.dws * Shape<K>::dw(len * p.hinv) / len;
-    return (__builtin_isfinite(scale) ? scale : 0.f) * x;
+    const float inv = __builtin_amdgcn_rsqf(d2);
+    const float scale = p.dws * Shape<K>::dw(d2 * inv * p.hinv) * inv;
+    return ((d2 > 0.f && __builtin_isfinite(scale)) ? scale : 0.f) * x;
 }
 
 // ---- grid view and neighbour walk -----------------------------------------
@@ -177,14 +181,11 @@ __device__ __forceinline__ int3c cell_of(const GridDev& g, f3 x)
 
 constexpr uint32_t kWalkBudget = 1u << 15;
 
-// Calls f(j, x_ij, |x_ij|^2) for every indexed point j with
-// |x_ij|^2 = fmaf-dot <= r_c_2 (the point itself included), walking
-// (2R+1)^2 x-strips of 2R+1 cells, each one contiguous range of the
-// cell-ordered arrays. Strips are clipped to the grid, which leaves every
-// point whose cell lies R or more cells inside the grid unchanged.
-template <bool IND, class F>
-__device__ __forceinline__ void walk(const GridDev& g, f3 xi, const float* __restrict__ xs,
-    const float* __restrict__ ys, const float* __restrict__ zs, F&& f)
+// Calls r(s, e) for the candidate range [s, e) (cell order) of each of the
+// (2R+1)^2 x-strips of 2R+1 cells around the cell of xi, clipped to the grid.
+// Clipping leaves every point whose cell lies R or more cells inside the grid
+// unchanged.
+template <class Rg> __device__ __forceinline__ void strips(const GridDev& g, f3 xi, Rg&& r)
 {
     const int3c c = cell_of(g, xi);
     const int R = g.range;
@@ -196,21 +197,49 @@ __device__ __forceinline__ void walk(const GridDev& g, f3 xi, const float* __res
     uint32_t budget = kWalkBudget;
     for (int z = z0; z <= z1; ++z) {
         for (int y = y0; y <= y1; ++y) {
-            const int base = z * g.nxny + y * g.nx;
-            const uint32_t s = g.prefix[base + x0];
-            uint32_t e = g.prefix[base + x1 + 1];
+            const uint32_t base = static_cast<uint32_t>(z * g.nxny + y * g.nx);
+            const uint32_t s = gat(g.prefix, base + x0);
+            uint32_t e = gat(g.prefix, base + x1 + 1);
             if (e - s > budget)
                 e = s + budget;
             budget -= e - s;
-            for (uint32_t q = s; q < e; ++q) {
-                const uint32_t j = IND ? g.sorted[q] : q;
-                const f3 xij = xi - f3 { xs[j], ys[j], zs[j] };
-                const float d2 = dot(xij, xij);
-                if (d2 <= g.r2)
-                    f(j, xij, d2);
-            }
+            r(s, e);
         }
     }
+}
+
+// Calls f(j, x_ij, |x_ij|^2) for every indexed point j with
+// |x_ij|^2 = fmaf-dot <= r_c_2 (the point itself included), in walk order
+// (src/datastructure/uniformgrid.cuh:113-178).
+template <bool IND, class F>
+__device__ __forceinline__ void walk(const GridDev& g, f3 xi, const float* __restrict__ xs,
+    const float* __restrict__ ys, const float* __restrict__ zs, F&& f)
+{
+    strips(g, xi, [&](uint32_t s, uint32_t e) {
+        for (uint32_t q = s; q < e; ++q) {
+            const uint32_t j = IND ? gat(g.sorted, q) : q;
+            const f3 xij = xi - gat3(xs, ys, zs, j);
+            const float d2 = dot(xij, xij);
+            if (d2 <= g.r2)
+                f(j, xij, d2);
+        }
+    });
+}
+
+// The same walk over packed records P[j] = {x, y, z, m}; f(j, x_ij, d2, m_j).
+template <bool IND, class F>
+__device__ __forceinline__ void walk4(const GridDev& g, f3 xi, const float4* __restrict__ P, F&& f)
+{
+    strips(g, xi, [&](uint32_t s, uint32_t e) {
+        for (uint32_t q = s; q < e; ++q) {
+            const uint32_t j = IND ? gat(g.sorted, q) : q;
+            const float4 pj = gat(P, j);
+            const f3 xij = xi - f3 { pj.x, pj.y, pj.z };
+            const float d2 = dot(xij, xij);
+            if (d2 <= g.r2)
+                f(j, xij, d2, pj.w);
+        }
+    });
 }
 
 // ---- per-step neighbour lists ------------------------------------------------
@@ -237,10 +266,10 @@ __device__ __forceinline__ bool nbrs(const GridDev& g, const NbrDev& L, uint32_t
     if (L.cnt) {
         const uint32_t c = L.cnt[t];
         if (c <= L.cap) {
-            const uint32_t* __restrict__ col = L.idx + t;
-            for (uint32_t k = 0; k < c; ++k) {
-                const uint32_t j = col[size_t(k) * L.n];
-                const f3 xij = xi - f3 { xs[j], ys[j], zs[j] };
+            uint32_t off = t;
+            for (uint32_t k = 0; k < c; ++k, off += L.n) {
+                const uint32_t j = gat(L.idx, off);
+                const f3 xij = xi - gat3(xs, ys, zs, j);
                 f(j, xij, dot(xij, xij));
             }
             return true;
@@ -261,18 +290,18 @@ __device__ __forceinline__ bool nbrs4(const GridDev& g, const NbrDev& L, uint32_
     if (L.cnt) {
         const uint32_t c = L.cnt[t];
         if (c <= L.cap) {
-            const uint32_t* __restrict__ col = L.idx + t;
+            uint32_t off = t;
 #pragma unroll 2
-            for (uint32_t k = 0; k < c; ++k) {
-                const uint32_t j = col[size_t(k) * L.n];
-                const float4 q = P[j];
+            for (uint32_t k = 0; k < c; ++k, off += L.n) {
+                const uint32_t j = gat(L.idx, off);
+                const float4 q = gat(P, j);
                 const f3 xij = xi - f3 { q.x, q.y, q.z };
                 f(j, xij, dot(xij, xij), q.w);
             }
             return true;
         }
     }
-    walk<IND>(g, xi, xs, ys, zs, [&](uint32_t j, f3 xij, float d2) { f(j, xij, d2, ms[j]); });
+    walk<IND>(g, xi, xs, ys, zs, [&](uint32_t j, f3 xij, float d2) { f(j, xij, d2, gat(ms, j)); });
     return false;
 }
 
