@@ -117,7 +117,9 @@ template <> struct Shape<KCOS> {
 // FP32 rounding the solver parity bar allows.
 template <int K> __device__ __forceinline__ float kW(const KParams& p, float d2)
 {
-    const float len = d2 > 0.f ? d2 * __builtin_amdgcn_rsqf(d2) : 0.f;
+    float len = d2 * __builtin_amdgcn_rsqf(d2);
+    if (!(len == len)) // 0 * inf at d2 = 0 and inf * 0 at d2 = inf: |x| is d2 itself
+        len = d2;
     return p.ws * Shape<K>::w(sat(len * p.hinv));
 }
 
