@@ -118,8 +118,10 @@ template <> struct Shape<KCOS> {
 template <int K> __device__ __forceinline__ float kW(const KParams& p, float d2)
 {
     float len = d2 * __builtin_amdgcn_rsqf(d2);
-    if (!(len == len)) // 0 * inf at d2 = 0 and inf * 0 at d2 = inf: |x| is d2 itself
-        len = d2;
+    // not finite: d2 = 0 or below the normal range (the hardware rsqrt flushes
+    // it to 0) gives |x| ~ 0; d2 = inf gives |x| = inf
+    if (!(len <= 3.0e38f))
+        len = d2 > 1.f ? d2 : 0.f;
     return p.ws * Shape<K>::w(sat(len * p.hinv));
 }
 
