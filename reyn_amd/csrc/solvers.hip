@@ -138,23 +138,34 @@ __global__ __launch_bounds__(kBlock) void k_density(FluidDev f, GridDev g, BdyDe
     const uint32_t i = particle_of<IND>(g, t);
     const f3 xi = ld3(f.x, f.y, f.z, i);
     ksum acc;
-    uint32_t cnt = 0;
-    uint32_t off = t;
-    auto visit = [&](uint32_t j, float d2, float mj) {
-        acc.add(mj * kW<K>(kp, d2));
-        if (L.cnt && j != i) {
-            if (cnt < L.cap)
-                L.idx[off] = j;
-            off += L.n;
-            ++cnt;
-        }
-    };
-    if (P4)
-        walk4<IND>(g, xi, P4, [&](uint32_t j, f3, float d2, float mj) { visit(j, d2, mj); });
-    else
-        walk<IND>(g, xi, f.x, f.y, f.z, [&](uint32_t j, f3, float d2) { visit(j, d2, gat(f.m, j)); });
-    if (L.cnt)
+    bool summed = false;
+    if (L.cnt && P4) {
+        // the walk only tests candidates and records neighbours; the sum runs
+        // over the compact list afterwards, so the kernel body is not executed
+        // once per candidate of every lane of the wavefront
+        uint32_t cnt = 0, off = t;
+        walk4<IND>(g, xi, P4, [&](uint32_t j, f3, float, float) {
+            if (j != i) {
+                if (cnt < L.cap)
+                    L.idx[off] = j;
+                off += L.n;
+                ++cnt;
+            }
+        });
         L.cnt[t] = cnt;
+        if (cnt <= L.cap) {
+            acc.add(gat(f.m, i) * kW<K>(kp, 0.f)); // the particle itself
+            nbrs4<IND>(g, L, t, xi, P4, f.x, f.y, f.z, f.m,
+                [&](uint32_t, f3, float d2, float mj) { acc.add(mj * kW<K>(kp, d2)); });
+            summed = true;
+        }
+    }
+    if (!summed) {
+        if (P4)
+            walk4<IND>(g, xi, P4, [&](uint32_t, f3, float d2, float mj) { acc.add(mj * kW<K>(kp, d2)); });
+        else
+            walk<IND>(g, xi, f.x, f.y, f.z, [&](uint32_t j, f3, float d2) { acc.add(gat(f.m, j) * kW<K>(kp, d2)); });
+    }
     ksum accb;
     ksum3 grad;
     if (b.present) {
@@ -598,7 +609,8 @@ int run_loop(reyn_ctx* ctx, SolveCtl* ctl, uint32_t min_iter, uint32_t max_iter,
     const uint32_t hard_cap = max_iter ? max_iter : 100000u;
     uint32_t launched = 0;
     for (;;) {
-        uint32_t m = (launched == 0) ? std::max(min_iter, chunk) : chunk;
+        // the first chunk is exactly the iterations that must run anyway
+        uint32_t m = (launched == 0) ? std::max(min_iter, 1u) : chunk;
         m = std::min(m, hard_cap - launched);
         for (uint32_t k = 0; k < m; ++k)
             REYN_TRY(iter());

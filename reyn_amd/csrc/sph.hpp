@@ -294,13 +294,26 @@ __device__ __forceinline__ bool nbrs4(const GridDev& g, const NbrDev& L, uint32_
     if (L.cnt) {
         const uint32_t c = L.cnt[t];
         if (c <= L.cap) {
+            // four neighbours per round: their index and record loads are all
+            // issued before the first use, which hides gather latency
+            constexpr uint32_t B = 4;
             uint32_t off = t;
-#pragma unroll 2
-            for (uint32_t k = 0; k < c; ++k, off += L.n) {
-                const uint32_t j = gat(L.idx, off);
-                const float4 q = gat(P, j);
-                const f3 xij = xi - f3 { q.x, q.y, q.z };
-                f(j, xij, dot(xij, xij), q.w);
+            for (uint32_t k = 0; k < c; k += B, off += B * L.n) {
+                uint32_t j[B];
+                float4 q[B];
+#pragma unroll
+                for (uint32_t u = 0; u < B; ++u) // past the count: reload row k (valid)
+                    j[u] = gat(L.idx, (k + u < c) ? off + u * L.n : off);
+#pragma unroll
+                for (uint32_t u = 0; u < B; ++u)
+                    q[u] = gat(P, j[u]);
+#pragma unroll
+                for (uint32_t u = 0; u < B; ++u) {
+                    if (k + u < c) {
+                        const f3 xij = xi - f3 { q[u].x, q[u].y, q[u].z };
+                        f(j[u], xij, dot(xij, xij), q[u].w);
+                    }
+                }
             }
             return true;
         }
