@@ -53,6 +53,11 @@ template <class T> __device__ __forceinline__ T gat(const T* __restrict__ base, 
 {
     return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + static_cast<uint32_t>(i * sizeof(T)));
 }
+// The same through an explicit byte offset (for records of mixed layout).
+template <class T> __device__ __forceinline__ T gatb(const void* __restrict__ base, uint32_t byte_off)
+{
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + byte_off);
+}
 __device__ __forceinline__ f3 gat3(const float* __restrict__ x, const float* __restrict__ y,
     const float* __restrict__ z, uint32_t i)
 {

@@ -57,28 +57,31 @@ struct Scratch {
     uint32_t* part_u; // per-block counts
     SolveCtl* ctl;
     NbrDev L; // the step's neighbour list (cnt == NULL when not requested)
-    // packed records gathered by the listed loops (with the list only):
-    float4* P4; // {x, y, z, m}, written by the density pass
-    float4* V4; // {vx, vy, vz, 0} after a velocity update
-    float4* A4; // {ax, ay, az, 0} pressure accelerations inside a solver loop
+    // packed records (NULL when the step has none):
+    float4* P4; // {x, y, z, m}: walked and summed by the density pass
+    float4* X; // interleaved records of the later loops, see `aux4`
 };
 
 // Neighbours kept per particle; a fluid particle at rest density has about
 // (4/3) pi 2^3 = 33.5 neighbours within the support 2h.
 constexpr uint32_t kListCap = 64;
 
+// lists: the solver-step scratch (packed records, and the neighbour list
+// while its entries fit 32-bit byte offsets); otherwise only the boundary
+// sums and the loop control.
 int get_scratch(reyn_ctx* ctx, uint32_t n, Scratch* s, bool lists = false)
 {
     const size_t nb = blocks_for(n);
     auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
     const size_t f = al(size_t(n) * 4);
     const size_t head = 4 * f + al(nb * 8) + al(nb * 4) + al(sizeof(SolveCtl));
-    const size_t f4 = al(size_t(n) * 16);
-    // list entries are addressed with 32-bit byte offsets
-    const bool keep_list = lists && size_t(n) * kListCap * 4 < (size_t(1) << 32);
-    const size_t bytes = head + (lists ? f + al(size_t(n) * kListCap * 4) + 3 * f4 : 0);
+    // records and list entries are addressed with 32-bit byte offsets
+    const bool records = lists && size_t(n) * 32 < (size_t(1) << 32);
+    const bool keep_list = records && size_t(n) * kListCap * 4 < (size_t(1) << 32);
+    const size_t rec_bytes = records ? al(size_t(n) * 16) + al(size_t(n) * 32) : 0;
+    const size_t list_bytes = keep_list ? f + al(size_t(n) * kListCap * 4) : 0;
     char* p = nullptr;
-    REYN_TRY(ctx_scratch(ctx, bytes, reinterpret_cast<void**>(&p)));
+    REYN_TRY(ctx_scratch(ctx, head + rec_bytes + list_bytes, reinterpret_cast<void**>(&p)));
     s->gbx = reinterpret_cast<float*>(p);
     s->gby = reinterpret_cast<float*>(p + f);
     s->gbz = reinterpret_cast<float*>(p + 2 * f);
@@ -87,42 +90,59 @@ int get_scratch(reyn_ctx* ctx, uint32_t n, Scratch* s, bool lists = false)
     s->part_u = reinterpret_cast<uint32_t*>(p + 4 * f + al(nb * 8));
     s->ctl = reinterpret_cast<SolveCtl*>(p + 4 * f + al(nb * 8) + al(nb * 4));
     s->L = NbrDev { nullptr, nullptr, n, kListCap };
-    s->P4 = s->V4 = s->A4 = nullptr;
-    if (lists) {
-        if (keep_list) {
-            s->L.cnt = reinterpret_cast<uint32_t*>(p + head);
-            s->L.idx = reinterpret_cast<uint32_t*>(p + head + f);
-        }
-        char* q = p + head + f + al(size_t(n) * kListCap * 4);
-        s->P4 = reinterpret_cast<float4*>(q);
-        s->V4 = reinterpret_cast<float4*>(q + f4);
-        s->A4 = reinterpret_cast<float4*>(q + 2 * f4);
+    s->P4 = s->X = nullptr;
+    if (records) {
+        s->P4 = reinterpret_cast<float4*>(p + head);
+        s->X = reinterpret_cast<float4*>(p + head + al(size_t(n) * 16));
+    }
+    if (keep_list) {
+        s->L.cnt = reinterpret_cast<uint32_t*>(p + head + rec_bytes);
+        s->L.idx = reinterpret_cast<uint32_t*>(p + head + rec_bytes + f);
     }
     return REYN_OK;
 }
 
 __device__ __forceinline__ bool loop_done(const SolveCtl* ctl) { return ctl && ctl->done; }
 
-// velocity of particle j from the packed copy when there is one
-__device__ __forceinline__ f3 ldv(const float4* __restrict__ V4, const FluidDev& f, uint32_t j)
+// Interleaved records X of the loops after the density pass: X[2j] holds
+// {x, y, z, m} of particle j and X[2j+1] an auxiliary slot {u, w} whose
+// meaning follows the phase:
+//   k_pack                      u = v,  w = 0
+//   k_density                   w = rho
+//   k_integrate (velocity only) u = the updated velocity
+//   k_predict_rho (PCISPH)      w = the predicted density
+//   k_iisph_pacc (loop)         u = the pressure acceleration
+// A neighbour's record and auxiliary slot share one 32-byte block, so the two
+// gathers of a neighbour touch one cache line. A phase writes only the
+// component that no concurrent thread of the same phase reads.
+__device__ __forceinline__ float4 aux4(const float4* __restrict__ X, uint32_t j) { return gat(X, 2 * j + 1); }
+__device__ __forceinline__ f3 aux3(const float4* __restrict__ X, uint32_t j) { return gatb<f3>(X, 32 * j + 16); }
+__device__ __forceinline__ float auxw(const float4* __restrict__ X, uint32_t j) { return gatb<float>(X, 32 * j + 28); }
+__device__ __forceinline__ void put_aux3(float4* __restrict__ X, uint32_t i, f3 u)
 {
-    if (V4) {
-        const float4 q = gat(V4, j);
-        return { q.x, q.y, q.z };
-    }
-    return gat3(f.vx, f.vy, f.vz, j);
+    float* q = reinterpret_cast<float*>(X + 2 * size_t(i) + 1);
+    q[0] = u.x;
+    q[1] = u.y;
+    q[2] = u.z;
+}
+__device__ __forceinline__ void put_auxw(float4* __restrict__ X, uint32_t i, float w)
+{
+    reinterpret_cast<float*>(X + 2 * size_t(i) + 1)[3] = w;
 }
 
 // ---- density ------------------------------------------------------------------
 
-// P4[i] = {x_i, y_i, z_i, m_i}: the packed records every walk and list loop of
-// the step gathers.
-__global__ __launch_bounds__(kBlock) void k_pack(FluidDev f, float4* __restrict__ P4)
+// P4[i] = {x_i, y_i, z_i, m_i}, the records the density pass walks, and the
+// interleaved records X of the later loops with u = v_i (see `aux4`).
+__global__ __launch_bounds__(kBlock) void k_pack(FluidDev f, float4* __restrict__ P4, float4* __restrict__ X)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= f.n)
         return;
-    P4[i] = make_float4(f.x[i], f.y[i], f.z[i], f.m[i]);
+    const float4 r = make_float4(f.x[i], f.y[i], f.z[i], f.m[i]);
+    P4[i] = r;
+    X[2 * size_t(i)] = r;
+    X[2 * size_t(i) + 1] = make_float4(f.vx[i], f.vy[i], f.vz[i], 0.f);
 }
 
 // Densities; with GB also the boundary sums, and with a list in L the
@@ -130,7 +150,8 @@ __global__ __launch_bounds__(kBlock) void k_pack(FluidDev f, float4* __restrict_
 template <int K, bool IND, bool GB>
 __global__ __launch_bounds__(kBlock) void k_density(FluidDev f, GridDev g, BdyDev b, KParams kp,
     float* __restrict__ rho, float* __restrict__ gbx, float* __restrict__ gby,
-    float* __restrict__ gbz, float* __restrict__ rb, NbrDev L, float4* __restrict__ P4)
+    float* __restrict__ gbz, float* __restrict__ rb, NbrDev L, const float4* __restrict__ P4,
+    float4* __restrict__ X)
 {
     const uint32_t t = global_index();
     if (t >= f.n)
@@ -155,7 +176,7 @@ __global__ __launch_bounds__(kBlock) void k_density(FluidDev f, GridDev g, BdyDe
         L.cnt[t] = cnt;
         if (cnt <= L.cap) {
             acc.add(gat(f.m, i) * kW<K>(kp, 0.f)); // the particle itself
-            nbrs4<IND>(g, L, t, xi, P4, f.x, f.y, f.z, f.m,
+            nbrs4<IND, kNbrBatch<K>>(g, L, t, xi, P4, f.x, f.y, f.z, f.m,
                 [&](uint32_t, f3, float d2, float mj) { acc.add(mj * kW<K>(kp, d2)); });
             summed = true;
         }
@@ -177,7 +198,10 @@ __global__ __launch_bounds__(kBlock) void k_density(FluidDev f, GridDev g, BdyDe
         });
     }
     const float rbi = accb.value();
-    rho[i] = acc.value() + rbi;
+    const float r = acc.value() + rbi;
+    rho[i] = r;
+    if (X)
+        put_auxw(X, i, r);
     if (GB) {
         const f3 gb = grad.value();
         gbx[i] = gb.x;
@@ -189,10 +213,12 @@ __global__ __launch_bounds__(kBlock) void k_density(FluidDev f, GridDev g, BdyDe
 
 // ---- non-pressure accelerations (gravity + fluid-fluid viscosity) -----------
 
-template <int K, bool IND>
+// PK: the interleaved records X hold {x, m} and {v, rho} of every particle
+// (u = v, w = rho); otherwise the loop walks the SoA arrays.
+template <int K, bool IND, bool PK>
 __global__ __launch_bounds__(kBlock) void k_nonpressure(FluidDev f, GridDev g, SphK c,
     const float* __restrict__ rho, float* __restrict__ ax, float* __restrict__ ay,
-    float* __restrict__ az, NbrDev L, const float4* __restrict__ P4)
+    float* __restrict__ az, NbrDev L, const float4* __restrict__ X)
 {
     const uint32_t t = global_index();
     if (t >= f.n)
@@ -201,9 +227,14 @@ __global__ __launch_bounds__(kBlock) void k_nonpressure(FluidDev f, GridDev g, S
     const f3 xi = ld3(f.x, f.y, f.z, i);
     const f3 vi = ld3(f.vx, f.vy, f.vz, i);
     f3 acc = c.g;
-    nbrs4<IND>(g, L, t, xi, P4, f.x, f.y, f.z, f.m, [&](uint32_t j, f3 xij, float d2, float mj) {
-        const f3 vij = vi - gat3(f.vx, f.vy, f.vz, j);
-        const float s = c.visc_c * mj / gat(rho, j) * dot(vij, xij) / (d2 + c.visc_eps);
+    nbrs4<IND, kNbrBatch<K>, 2>(g, L, t, xi, X, f.x, f.y, f.z, f.m, [&](uint32_t j, f3 xij, float d2, float mj) {
+        float4 u;
+        if constexpr (PK)
+            u = aux4(X, j);
+        else
+            u = make_float4(gat(f.vx, j), gat(f.vy, j), gat(f.vz, j), gat(rho, j));
+        const f3 vij = vi - f3 { u.x, u.y, u.z };
+        const float s = c.visc_c * mj / u.w * dot(vij, xij) / (d2 + c.visc_eps);
         acc += s * kdW<K>(c.kp, xij, d2);
     });
     st3(ax, ay, az, i, acc);
@@ -213,7 +244,7 @@ __global__ __launch_bounds__(kBlock) void k_nonpressure(FluidDev f, GridDev g, S
 
 __global__ __launch_bounds__(kBlock) void k_integrate(FluidDev f, const float* __restrict__ ax,
     const float* __restrict__ ay, const float* __restrict__ az, float dt, const SolveCtl* ctl,
-    int positions, float4* __restrict__ V4)
+    int positions, float4* __restrict__ X)
 {
     if (loop_done(ctl))
         return;
@@ -222,8 +253,8 @@ __global__ __launch_bounds__(kBlock) void k_integrate(FluidDev f, const float* _
         return;
     const f3 v = ld3(f.vx, f.vy, f.vz, i) + dt * ld3(ax, ay, az, i);
     st3(f.vx, f.vy, f.vz, i, v);
-    if (V4)
-        V4[i] = make_float4(v.x, v.y, v.z, 0.f);
+    if (X)
+        put_aux3(X, i, v);
     if (positions)
         st3(f.x, f.y, f.z, i, ld3(f.x, f.y, f.z, i) + dt * v);
 }
@@ -235,12 +266,11 @@ __device__ __forceinline__ float eos_sesph(float rho, float k, float rho0)
     return fmaxf(0.f, k * (rho / rho0 - 1.f));
 }
 
-template <int K, bool IND>
+template <int K, bool IND, bool PK>
 __global__ __launch_bounds__(kBlock) void k_sesph_acc(FluidDev f, GridDev g, int bdy, SphK c,
     float k, const float* __restrict__ rho, const float* __restrict__ gbx,
     const float* __restrict__ gby, const float* __restrict__ gbz, float* __restrict__ ax,
-    float* __restrict__ ay, float* __restrict__ az, NbrDev L,
-    const float4* __restrict__ P4)
+    float* __restrict__ ay, float* __restrict__ az, NbrDev L, const float4* __restrict__ X)
 {
     const uint32_t t = global_index();
     if (t >= f.n)
@@ -251,9 +281,14 @@ __global__ __launch_bounds__(kBlock) void k_sesph_acc(FluidDev f, GridDev g, int
     const float rhoi = rho[i];
     const float pri = eos_sesph(rhoi, k, c.rho0) / (rhoi * rhoi);
     f3 acc = c.g;
-    nbrs4<IND>(g, L, t, xi, P4, f.x, f.y, f.z, f.m, [&](uint32_t j, f3 xij, float d2, float mj) {
-        const float rhoj = gat(rho, j);
-        const f3 vij = vi - gat3(f.vx, f.vy, f.vz, j);
+    nbrs4<IND, kNbrBatch<K>, 2>(g, L, t, xi, X, f.x, f.y, f.z, f.m, [&](uint32_t j, f3 xij, float d2, float mj) {
+        float4 u; // {v_j, rho_j}
+        if constexpr (PK)
+            u = aux4(X, j);
+        else
+            u = make_float4(gat(f.vx, j), gat(f.vy, j), gat(f.vz, j), gat(rho, j));
+        const float rhoj = u.w;
+        const f3 vij = vi - f3 { u.x, u.y, u.z };
         const f3 dW = kdW<K>(c.kp, xij, d2);
         const float pj = eos_sesph(rhoj, k, c.rho0);
         const float visc = c.visc_c * mj / rhoj * dot(vij, xij) / (d2 + c.visc_eps);
@@ -267,12 +302,12 @@ __global__ __launch_bounds__(kBlock) void k_sesph_acc(FluidDev f, GridDev g, int
 // ---- PCISPH ----------------------------------------------------------------------
 
 // predicted density; per-block FP64 partial sums feed the mean-density test
-template <int K, bool IND>
+// PK: u of X holds the updated velocities and w receives the prediction.
+template <int K, bool IND, bool PK>
 __global__ __launch_bounds__(kBlock) void k_predict_rho(FluidDev f, GridDev g, int bdy, KParams kp,
     float dt, const float* __restrict__ gbx, const float* __restrict__ gby,
     const float* __restrict__ gbz, const float* __restrict__ rb, float* __restrict__ rho,
-    double* __restrict__ part, const SolveCtl* ctl, NbrDev L, const float4* __restrict__ P4,
-    const float4* __restrict__ V4)
+    double* __restrict__ part, const SolveCtl* ctl, NbrDev L, float4* __restrict__ X)
 {
     __shared__ double lds[kBlock / kWave];
     if (loop_done(ctl))
@@ -284,8 +319,13 @@ __global__ __launch_bounds__(kBlock) void k_predict_rho(FluidDev f, GridDev g, i
         const f3 xi = ld3(f.x, f.y, f.z, i);
         const f3 vi = ld3(f.vx, f.vy, f.vz, i);
         ksum acc;
-        const bool listed = nbrs4<IND>(g, L, t, xi, P4, f.x, f.y, f.z, f.m, [&](uint32_t j, f3 xij, float d2, float mj) {
-            const f3 vij = vi - ldv(V4, f, j);
+        const bool listed = nbrs4<IND, kNbrBatch<K>, 2>(g, L, t, xi, X, f.x, f.y, f.z, f.m, [&](uint32_t j, f3 xij, float d2, float mj) {
+            f3 vj;
+            if constexpr (PK)
+                vj = aux3(X, j);
+            else
+                vj = gat3(f.vx, f.vy, f.vz, j);
+            const f3 vij = vi - vj;
             acc.add(mj * (kW<K>(kp, d2) + dt * dot(kdW<K>(kp, xij, d2), vij)));
         });
         if (listed) // the list skips the particle itself: m_i (W(0) + dt grad W(0) . 0)
@@ -294,6 +334,8 @@ __global__ __launch_bounds__(kBlock) void k_predict_rho(FluidDev f, GridDev g, i
             acc.add(rb[i] + dt * dot(ld3(gbx, gby, gbz, i), vi));
         const float r = acc.value();
         rho[i] = r;
+        if constexpr (PK)
+            put_auxw(X, i, r);
         mine = r;
     }
     const double s = block_reduce<0>(mine, lds);
@@ -301,12 +343,13 @@ __global__ __launch_bounds__(kBlock) void k_predict_rho(FluidDev f, GridDev g, i
         part[xcd_block(blockIdx.x, gridDim.x)] = s;
 }
 
-template <int K, bool IND>
+// PK: w of X holds the predicted densities.
+template <int K, bool IND, bool PK>
 __global__ __launch_bounds__(kBlock) void k_pcisph_prs_acc(FluidDev f, GridDev g, int bdy,
     KParams kp, float k, float rho0, float rho0_sq_inv, const float* __restrict__ rho,
     const float* __restrict__ gbx, const float* __restrict__ gby, const float* __restrict__ gbz,
     float* __restrict__ ax, float* __restrict__ ay, float* __restrict__ az, const SolveCtl* ctl, NbrDev L,
-    const float4* __restrict__ P4)
+    const float4* __restrict__ X)
 {
     if (loop_done(ctl))
         return;
@@ -320,8 +363,12 @@ __global__ __launch_bounds__(kBlock) void k_pcisph_prs_acc(FluidDev f, GridDev g
     const float pri = pi / (rhoi * rhoi);
     const float pr0 = pi * rho0_sq_inv;
     f3 acc { 0.f, 0.f, 0.f };
-    nbrs4<IND>(g, L, t, xi, P4, f.x, f.y, f.z, f.m, [&](uint32_t j, f3 xij, float d2, float mj) {
-        const float rhoj = gat(rho, j);
+    nbrs4<IND, kNbrBatch<K>, 2>(g, L, t, xi, X, f.x, f.y, f.z, f.m, [&](uint32_t j, f3 xij, float d2, float mj) {
+        float rhoj;
+        if constexpr (PK)
+            rhoj = auxw(X, j);
+        else
+            rhoj = gat(rho, j);
         const float pj = fmaxf(0.f, k * (rhoj - rho0));
         acc += (mj * (pri + pj / (rhoj * rhoj))) * kdW<K>(kp, xij, d2);
     });
@@ -361,12 +408,13 @@ __device__ __forceinline__ float jacobi_update(float omega, float aii, float Ap,
     return (aii <= lim && aii >= -lim) ? 0.f : fmaxf(0.f, omega * (s - Ap) / aii);
 }
 
-template <int K, bool IND>
+// PK: u of X holds the updated velocities.
+template <int K, bool IND, bool PK>
 __global__ __launch_bounds__(kBlock) void k_iisph_aii_s(FluidDev f, GridDev g, int bdy, KParams kp,
     float rho0, float dt, float omega, int warm, const float* __restrict__ rho,
     const float* __restrict__ gbx, const float* __restrict__ gby, const float* __restrict__ gbz,
     float* __restrict__ aii, float* __restrict__ si, float* __restrict__ p, NbrDev L,
-    const float4* __restrict__ P4, const float4* __restrict__ V4)
+    float4* __restrict__ X)
 {
     const uint32_t t = global_index();
     if (t >= f.n)
@@ -376,10 +424,15 @@ __global__ __launch_bounds__(kBlock) void k_iisph_aii_s(FluidDev f, GridDev g, i
     const f3 vi = ld3(f.vx, f.vy, f.vz, i);
     ksum3 grad;
     ksum sq, div;
-    nbrs4<IND>(g, L, t, xi, P4, f.x, f.y, f.z, f.m, [&](uint32_t j, f3 xij, float d2, float mj) {
+    nbrs4<IND, kNbrBatch<K>, 2>(g, L, t, xi, X, f.x, f.y, f.z, f.m, [&](uint32_t j, f3 xij, float d2, float mj) {
+        f3 vj;
+        if constexpr (PK)
+            vj = aux3(X, j);
+        else
+            vj = gat3(f.vx, f.vy, f.vz, j);
         const f3 dW = kdW<K>(kp, xij, d2);
         const f3 mdW = mj * dW;
-        div.add(dot(mdW, vi - ldv(V4, f, j)));
+        div.add(dot(mdW, vi - vj));
         sq.add(dot(mdW, dW));
         grad.add(mdW);
     });
@@ -394,14 +447,17 @@ __global__ __launch_bounds__(kBlock) void k_iisph_aii_s(FluidDev f, GridDev g, i
     p[i] = warm ? p[i] * 0.5f : jacobi_update(omega, a, 0.f, s);
 }
 
-// Pressure accelerations; into the packed A4 inside the loop, into the SoA
-// work arrays for the final pass (A4 == NULL).
-template <int K, bool IND>
+// Pressure accelerations. PK: the loop reads the dense records P4 and the
+// pressure array (4 + 16 bytes per neighbour in two dense arrays touch fewer
+// cache lines than the interleaved records) and, inside the solver loop
+// (ctl != NULL), writes the result to u of X for the Jacobi pass; the final
+// pass and the SoA mode write the work arrays.
+template <int K, bool IND, bool PK>
 __global__ __launch_bounds__(kBlock) void k_iisph_pacc(FluidDev f, GridDev g, int bdy, KParams kp,
     float rho0_sq_inv, const float* __restrict__ p, const float* __restrict__ gbx,
     const float* __restrict__ gby, const float* __restrict__ gbz, float* __restrict__ ax,
     float* __restrict__ ay, float* __restrict__ az, const SolveCtl* ctl, NbrDev L,
-    const float4* __restrict__ P4, float4* __restrict__ A4)
+    const float4* __restrict__ P4, float4* __restrict__ X)
 {
     if (loop_done(ctl))
         return;
@@ -412,24 +468,26 @@ __global__ __launch_bounds__(kBlock) void k_iisph_pacc(FluidDev f, GridDev g, in
     const f3 xi = ld3(f.x, f.y, f.z, i);
     const float pi = p[i];
     f3 acc { 0.f, 0.f, 0.f };
-    nbrs4<IND>(g, L, t, xi, P4, f.x, f.y, f.z, f.m, [&](uint32_t j, f3 xij, float d2, float mj) {
+    nbrs4<IND, kNbrBatch<K>>(g, L, t, xi, P4, f.x, f.y, f.z, f.m, [&](uint32_t j, f3 xij, float d2, float mj) {
         acc += (mj * (pi + gat(p, j))) * kdW<K>(kp, xij, d2);
     });
     if (bdy)
         acc += pi * ld3(gbx, gby, gbz, i);
     const f3 a = -rho0_sq_inv * acc;
-    if (A4)
-        A4[i] = make_float4(a.x, a.y, a.z, 0.f);
+    if (PK && ctl)
+        put_aux3(X, i, a);
     else
         st3(ax, ay, az, i, a);
 }
 
-template <int K, bool IND>
+// PK: the accelerations come from u of X; otherwise from the SoA work arrays.
+template <int K, bool IND, bool PK>
 __global__ __launch_bounds__(kBlock) void k_iisph_jacobi(FluidDev f, GridDev g, int bdy, KParams kp,
-    float dt, float omega, float thr, const float4* __restrict__ A4, const float* __restrict__ aii,
+    float dt, float omega, float thr, const float* __restrict__ ax, const float* __restrict__ ay,
+    const float* __restrict__ az, const float* __restrict__ aii,
     const float* __restrict__ si, const float* __restrict__ gbx, const float* __restrict__ gby,
     const float* __restrict__ gbz, float* __restrict__ p, double* __restrict__ part_err,
-    uint32_t* __restrict__ part_cnt, const SolveCtl* ctl, NbrDev L, const float4* __restrict__ P4)
+    uint32_t* __restrict__ part_cnt, const SolveCtl* ctl, NbrDev L, float4* __restrict__ X)
 {
     __shared__ double lds[kBlock / kWave];
     if (loop_done(ctl))
@@ -440,12 +498,19 @@ __global__ __launch_bounds__(kBlock) void k_iisph_jacobi(FluidDev f, GridDev g, 
     if (t < f.n) {
         const uint32_t i = particle_of<IND>(g, t);
         const f3 xi = ld3(f.x, f.y, f.z, i);
-        const float4 a4 = A4[i];
-        const f3 ai { a4.x, a4.y, a4.z };
+        f3 ai;
+        if constexpr (PK)
+            ai = aux3(X, i);
+        else
+            ai = ld3(ax, ay, az, i);
         float lap = 0.f;
-        nbrs4<IND>(g, L, t, xi, P4, f.x, f.y, f.z, f.m, [&](uint32_t j, f3 xij, float d2, float mj) {
-            const float4 aj = gat(A4, j);
-            lap += mj * dot(ai - f3 { aj.x, aj.y, aj.z }, kdW<K>(kp, xij, d2));
+        nbrs4<IND, kNbrBatch<K>, 2>(g, L, t, xi, X, f.x, f.y, f.z, f.m, [&](uint32_t j, f3 xij, float d2, float mj) {
+            f3 aj;
+            if constexpr (PK)
+                aj = aux3(X, j);
+            else
+                aj = gat3(ax, ay, az, j);
+            lap += mj * dot(ai - aj, kdW<K>(kp, xij, d2));
         });
         if (bdy)
             lap += dot(ai, ld3(gbx, gby, gbz, i));
@@ -540,6 +605,12 @@ int prepare(reyn_ctx* ctx, const reyn_sph_params* prm, const reyn_particles* flu
     return REYN_OK;
 }
 
+// Dispatches on whether the step has interleaved records.
+template <class F> int with_pk(const float4* X, F&& f)
+{
+    return X ? f(std::true_type {}) : f(std::false_type {});
+}
+
 template <bool GB>
 int launch_density(reyn_ctx* ctx, int kernel, const Common& c, const KParams& kp, float* rho,
     const Scratch* s)
@@ -547,7 +618,7 @@ int launch_density(reyn_ctx* ctx, int kernel, const Common& c, const KParams& kp
     if (s && s->P4) {
         {
             ProfScope _prof(ctx, "k_pack");
-            k_pack<<<c.nb, kBlock, 0, ctx->stream>>>(c.f, s->P4);
+            k_pack<<<c.nb, kBlock, 0, ctx->stream>>>(c.f, s->P4, s->X);
         }
         REYN_LAUNCHED(ctx, "k_pack");
     }
@@ -558,7 +629,7 @@ int launch_density(reyn_ctx* ctx, int kernel, const Common& c, const KParams& kp
             ProfScope _prof(ctx, "k_density");
             k_density<K, I, GB><<<c.nb, kBlock, 0, ctx->stream>>>(c.f, c.g, c.b, kp, rho,
                 GB ? s->gbx : nullptr, GB ? s->gby : nullptr, GB ? s->gbz : nullptr, GB ? s->rb : nullptr,
-                s ? s->L : NbrDev {}, s ? s->P4 : nullptr);
+                s ? s->L : NbrDev {}, s ? s->P4 : nullptr, s ? s->X : nullptr);
         }
         REYN_LAUNCHED(ctx, "k_density");
         return REYN_OK;
@@ -566,26 +637,29 @@ int launch_density(reyn_ctx* ctx, int kernel, const Common& c, const KParams& kp
 }
 
 int launch_nonpressure(reyn_ctx* ctx, int kernel, const Common& c, const SphK& k, const float* rho,
-    float* ax, float* ay, float* az, const NbrDev& L, const float4* P4)
+    float* ax, float* ay, float* az, const NbrDev& L, const float4* X)
 {
     return with_kernel(ctx, kernel, c.ind, [&](auto KC, auto IC) -> int {
-        constexpr int K = decltype(KC)::value;
-        constexpr bool I = decltype(IC)::value;
-        {
-            ProfScope _prof(ctx, "k_nonpressure");
-            k_nonpressure<K, I><<<c.nb, kBlock, 0, ctx->stream>>>(c.f, c.g, k, rho, ax, ay, az, L, P4);
-        }
-        REYN_LAUNCHED(ctx, "k_nonpressure");
-        return REYN_OK;
+        return with_pk(X, [&](auto PC) -> int {
+            constexpr int K = decltype(KC)::value;
+            constexpr bool I = decltype(IC)::value;
+            constexpr bool PK = decltype(PC)::value;
+            {
+                ProfScope _prof(ctx, "k_nonpressure");
+                k_nonpressure<K, I, PK><<<c.nb, kBlock, 0, ctx->stream>>>(c.f, c.g, k, rho, ax, ay, az, L, X);
+            }
+            REYN_LAUNCHED(ctx, "k_nonpressure");
+            return REYN_OK;
+        });
     });
 }
 
 int launch_integrate(reyn_ctx* ctx, const Common& c, const float* ax, const float* ay,
-    const float* az, float dt, const SolveCtl* ctl, int positions, float4* V4 = nullptr)
+    const float* az, float dt, const SolveCtl* ctl, int positions, float4* X = nullptr)
 {
     {
         ProfScope _prof(ctx, positions ? "k_integrate" : "k_integrate_v");
-        k_integrate<<<c.nb, kBlock, 0, ctx->stream>>>(c.f, ax, ay, az, dt, ctl, positions, V4);
+        k_integrate<<<c.nb, kBlock, 0, ctx->stream>>>(c.f, ax, ay, az, dt, ctl, positions, X);
     }
     REYN_LAUNCHED(ctx, "k_integrate");
     return REYN_OK;
@@ -678,8 +752,8 @@ int reyn_predict_rho(reyn_ctx* ctx, const reyn_sph_params* prm, const reyn_parti
         constexpr bool I = decltype(IC)::value;
         {
             ProfScope _prof(ctx, "k_predict_rho");
-            k_predict_rho<K, I><<<c.nb, kBlock, 0, ctx->stream>>>(c.f, c.g, c.b.present, kp, dt, s.gbx,
-                s.gby, s.gbz, s.rb, rho, nullptr, nullptr, NbrDev {}, nullptr, nullptr);
+            k_predict_rho<K, I, false><<<c.nb, kBlock, 0, ctx->stream>>>(c.f, c.g, c.b.present, kp, dt,
+                s.gbx, s.gby, s.gbz, s.rb, rho, nullptr, nullptr, NbrDev {}, nullptr);
         }
         REYN_LAUNCHED(ctx, "k_predict_rho");
         return REYN_OK;
@@ -730,15 +804,18 @@ int reyn_sesph_step(reyn_ctx* ctx, const reyn_sesph_params* prm, const reyn_part
     const SphK k = make_sphk(prm->sph);
     REYN_TRY(launch_density<true>(ctx, prm->sph.kernel, c, k.kp, work->rho, &s));
     REYN_TRY(with_kernel(ctx, prm->sph.kernel, c.ind, [&](auto KC, auto IC) -> int {
-        constexpr int K = decltype(KC)::value;
-        constexpr bool I = decltype(IC)::value;
-        {
-            ProfScope _prof(ctx, "k_sesph_acc");
-            k_sesph_acc<K, I><<<c.nb, kBlock, 0, ctx->stream>>>(c.f, c.g, c.b.present, k, prm->k,
-                work->rho, s.gbx, s.gby, s.gbz, work->ax, work->ay, work->az, s.L, s.P4);
-        }
-        REYN_LAUNCHED(ctx, "k_sesph_acc");
-        return REYN_OK;
+        return with_pk(s.X, [&](auto PC) -> int {
+            constexpr int K = decltype(KC)::value;
+            constexpr bool I = decltype(IC)::value;
+            constexpr bool PK = decltype(PC)::value;
+            {
+                ProfScope _prof(ctx, "k_sesph_acc");
+                k_sesph_acc<K, I, PK><<<c.nb, kBlock, 0, ctx->stream>>>(c.f, c.g, c.b.present, k, prm->k,
+                    work->rho, s.gbx, s.gby, s.gbz, work->ax, work->ay, work->az, s.L, s.X);
+            }
+            REYN_LAUNCHED(ctx, "k_sesph_acc");
+            return REYN_OK;
+        });
     }));
     return launch_integrate(ctx, c, work->ax, work->ay, work->az, dt, nullptr, 1);
 }
@@ -824,29 +901,32 @@ int reyn_pcisph_step(reyn_ctx* ctx, const reyn_pcisph_params* prm, const reyn_pa
     const float rho0 = prm->sph.rho_0;
     const float thr = (prm->eta_rho_max + 1.f) * rho0;
     REYN_TRY(launch_density<true>(ctx, prm->sph.kernel, c, k.kp, work->rho, &s));
-    REYN_TRY(launch_nonpressure(ctx, prm->sph.kernel, c, k, work->rho, work->ax, work->ay, work->az, s.L, s.P4));
+    REYN_TRY(launch_nonpressure(ctx, prm->sph.kernel, c, k, work->rho, work->ax, work->ay, work->az, s.L, s.X));
     SolveCtl res;
     REYN_TRY(run_loop(
         ctx, s.ctl, prm->min_iter, prm->max_iter, 4,
         [&]() -> int {
-            REYN_TRY(launch_integrate(ctx, c, work->ax, work->ay, work->az, dt, s.ctl, 0, s.V4));
+            REYN_TRY(launch_integrate(ctx, c, work->ax, work->ay, work->az, dt, s.ctl, 0, s.X));
             REYN_TRY(with_kernel(ctx, prm->sph.kernel, c.ind, [&](auto KC, auto IC) -> int {
-                constexpr int K = decltype(KC)::value;
-                constexpr bool I = decltype(IC)::value;
-                {
-                    ProfScope _prof(ctx, "k_predict_rho");
-                    k_predict_rho<K, I><<<c.nb, kBlock, 0, ctx->stream>>>(c.f, c.g, c.b.present, k.kp, dt,
-                        s.gbx, s.gby, s.gbz, s.rb, work->rho, s.part_d, s.ctl, s.L, s.P4, s.V4);
-                }
-                REYN_LAUNCHED(ctx, "k_predict_rho");
-                {
-                    ProfScope _prof(ctx, "k_pcisph_prs_acc");
-                    k_pcisph_prs_acc<K, I><<<c.nb, kBlock, 0, ctx->stream>>>(c.f, c.g, c.b.present, k.kp, kk,
-                        rho0, 1.f / (rho0 * rho0), work->rho, s.gbx, s.gby, s.gbz, work->ax, work->ay,
-                        work->az, s.ctl, s.L, s.P4);
-                }
-                REYN_LAUNCHED(ctx, "k_pcisph_prs_acc");
-                return REYN_OK;
+                return with_pk(s.X, [&](auto PC) -> int {
+                    constexpr int K = decltype(KC)::value;
+                    constexpr bool I = decltype(IC)::value;
+                    constexpr bool PK = decltype(PC)::value;
+                    {
+                        ProfScope _prof(ctx, "k_predict_rho");
+                        k_predict_rho<K, I, PK><<<c.nb, kBlock, 0, ctx->stream>>>(c.f, c.g, c.b.present, k.kp,
+                            dt, s.gbx, s.gby, s.gbz, s.rb, work->rho, s.part_d, s.ctl, s.L, s.X);
+                    }
+                    REYN_LAUNCHED(ctx, "k_predict_rho");
+                    {
+                        ProfScope _prof(ctx, "k_pcisph_prs_acc");
+                        k_pcisph_prs_acc<K, I, PK><<<c.nb, kBlock, 0, ctx->stream>>>(c.f, c.g, c.b.present,
+                            k.kp, kk, rho0, 1.f / (rho0 * rho0), work->rho, s.gbx, s.gby, s.gbz, work->ax,
+                            work->ay, work->az, s.ctl, s.L, s.X);
+                    }
+                    REYN_LAUNCHED(ctx, "k_pcisph_prs_acc");
+                    return REYN_OK;
+                });
             }));
             {
                 ProfScope _prof(ctx, "k_ctl_pcisph");
@@ -892,30 +972,37 @@ int reyn_iisph_step(reyn_ctx* ctx, const reyn_iisph_params* prm, const reyn_part
     const int kind = prm->sph.kernel;
     const int bd = c.b.present;
     REYN_TRY(launch_density<true>(ctx, kind, c, k.kp, work->rho, &s));
-    REYN_TRY(launch_nonpressure(ctx, kind, c, k, work->rho, work->ax, work->ay, work->az, s.L, s.P4));
-    REYN_TRY(launch_integrate(ctx, c, work->ax, work->ay, work->az, dt, nullptr, 0, s.V4));
+    REYN_TRY(launch_nonpressure(ctx, kind, c, k, work->rho, work->ax, work->ay, work->az, s.L, s.X));
+    REYN_TRY(launch_integrate(ctx, c, work->ax, work->ay, work->az, dt, nullptr, 0, s.X));
     REYN_TRY(with_kernel(ctx, kind, c.ind, [&](auto KC, auto IC) -> int {
-        constexpr int K = decltype(KC)::value;
-        constexpr bool I = decltype(IC)::value;
-        {
-            ProfScope _prof(ctx, "k_iisph_aii_s");
-            k_iisph_aii_s<K, I><<<c.nb, kBlock, 0, ctx->stream>>>(c.f, c.g, bd, k.kp, rho0, dt, prm->omega,
-                prm->warm_start, work->rho, s.gbx, s.gby, s.gbz, work->a_ii, work->s_i, work->p, s.L, s.P4, s.V4);
-        }
-        REYN_LAUNCHED(ctx, "k_iisph_aii_s");
-        return REYN_OK;
+        return with_pk(s.X, [&](auto PC) -> int {
+            constexpr int K = decltype(KC)::value;
+            constexpr bool I = decltype(IC)::value;
+            constexpr bool PK = decltype(PC)::value;
+            {
+                ProfScope _prof(ctx, "k_iisph_aii_s");
+                k_iisph_aii_s<K, I, PK><<<c.nb, kBlock, 0, ctx->stream>>>(c.f, c.g, bd, k.kp, rho0, dt,
+                    prm->omega, prm->warm_start, work->rho, s.gbx, s.gby, s.gbz, work->a_ii, work->s_i, work->p,
+                    s.L, s.X);
+            }
+            REYN_LAUNCHED(ctx, "k_iisph_aii_s");
+            return REYN_OK;
+        });
     }));
     auto pacc = [&](const SolveCtl* ctl) -> int {
         return with_kernel(ctx, kind, c.ind, [&](auto KC, auto IC) -> int {
-            constexpr int K = decltype(KC)::value;
-            constexpr bool I = decltype(IC)::value;
-            {
-                ProfScope _prof(ctx, "k_iisph_pacc");
-                k_iisph_pacc<K, I><<<c.nb, kBlock, 0, ctx->stream>>>(c.f, c.g, bd, k.kp, rho0_sq_inv,
-                    work->p, s.gbx, s.gby, s.gbz, work->ax, work->ay, work->az, ctl, s.L, s.P4, ctl ? s.A4 : nullptr);
-            }
-            REYN_LAUNCHED(ctx, "k_iisph_pacc");
-            return REYN_OK;
+            return with_pk(s.X, [&](auto PC) -> int {
+                constexpr int K = decltype(KC)::value;
+                constexpr bool I = decltype(IC)::value;
+                constexpr bool PK = decltype(PC)::value;
+                {
+                    ProfScope _prof(ctx, "k_iisph_pacc");
+                    k_iisph_pacc<K, I, PK><<<c.nb, kBlock, 0, ctx->stream>>>(c.f, c.g, bd, k.kp, rho0_sq_inv,
+                        work->p, s.gbx, s.gby, s.gbz, work->ax, work->ay, work->az, ctl, s.L, s.P4, s.X);
+                }
+                REYN_LAUNCHED(ctx, "k_iisph_pacc");
+                return REYN_OK;
+            });
         });
     };
     SolveCtl res;
@@ -924,16 +1011,19 @@ int reyn_iisph_step(reyn_ctx* ctx, const reyn_iisph_params* prm, const reyn_part
         [&]() -> int {
             REYN_TRY(pacc(s.ctl));
             REYN_TRY(with_kernel(ctx, kind, c.ind, [&](auto KC, auto IC) -> int {
-                constexpr int K = decltype(KC)::value;
-                constexpr bool I = decltype(IC)::value;
-                {
-                    ProfScope _prof(ctx, "k_iisph_jacobi");
-                    k_iisph_jacobi<K, I><<<c.nb, kBlock, 0, ctx->stream>>>(c.f, c.g, bd, k.kp, dt, prm->omega,
-                        thr, s.A4, work->a_ii, work->s_i, s.gbx, s.gby, s.gbz, work->p,
-                        s.part_d, s.part_u, s.ctl, s.L, s.P4);
-                }
-                REYN_LAUNCHED(ctx, "k_iisph_jacobi");
-                return REYN_OK;
+                return with_pk(s.X, [&](auto PC) -> int {
+                    constexpr int K = decltype(KC)::value;
+                    constexpr bool I = decltype(IC)::value;
+                    constexpr bool PK = decltype(PC)::value;
+                    {
+                        ProfScope _prof(ctx, "k_iisph_jacobi");
+                        k_iisph_jacobi<K, I, PK><<<c.nb, kBlock, 0, ctx->stream>>>(c.f, c.g, bd, k.kp, dt,
+                            prm->omega, thr, work->ax, work->ay, work->az, work->a_ii, work->s_i, s.gbx, s.gby,
+                            s.gbz, work->p, s.part_d, s.part_u, s.ctl, s.L, s.X);
+                    }
+                    REYN_LAUNCHED(ctx, "k_iisph_jacobi");
+                    return REYN_OK;
+                });
             }));
             {
                 ProfScope _prof(ctx, "k_ctl_iisph");

@@ -259,34 +259,27 @@ struct NbrDev {
     uint32_t n, cap;
 };
 
-// Calls f(j, x_ij, |x_ij|^2) for the neighbours of the particle in slot t at
-// xi. Through the list the particle itself is skipped (returns true); through
-// the grid it is included (returns false). Every list entry gives the same
-// x_ij and |x_ij|^2 bits as the walk.
-template <bool IND, class F>
-__device__ __forceinline__ bool nbrs(const GridDev& g, const NbrDev& L, uint32_t t, f3 xi,
-    const float* __restrict__ xs, const float* __restrict__ ys, const float* __restrict__ zs, F&& f)
-{
-    if (L.cnt) {
-        const uint32_t c = L.cnt[t];
-        if (c <= L.cap) {
-            uint32_t off = t;
-            for (uint32_t k = 0; k < c; ++k, off += L.n) {
-                const uint32_t j = gat(L.idx, off);
-                const f3 xij = xi - gat3(xs, ys, zs, j);
-                f(j, xij, dot(xij, xij));
-            }
-            return true;
-        }
-    }
-    walk<IND>(g, xi, xs, ys, zs, f);
-    return false;
-}
-
-// As `nbrs`, over packed records P[j] = {x, y, z, m}: one 16-byte load per
-// listed neighbour instead of one load per SoA field; f(j, x_ij, d2, m_j).
-// The walk fallback reads the SoA arrays, which hold the same values.
-template <bool IND, class F>
+// Calls f(j, x_ij, d2, m_j) for the neighbours of the particle in slot t at
+// xi, reading the packed record {x, y, z, m} of j at P[S j]: one 16-byte load
+// per listed neighbour instead of one load per SoA field. Through the list
+// the particle itself is skipped (returns true); through the grid (no list,
+// or a count above the cap) it is included (returns false). The walk reads
+// the SoA arrays, which hold the same values, and every list entry gives the
+// same x_ij and d2 bits as the walk.
+//
+// The list is read B entries per round with no branch inside a round: the
+// index loads, the record loads and whatever the bodies gather for j are all
+// in flight together. Rounds are padded past the count with the round's
+// first entry passed as m_j = 0, so every body must be linear in m_j (each
+// SPH pair term carries the factor m_j); a padding call then adds an exact
+// zero for finite particle states.
+#ifndef REYN_NBR_BATCH
+#define REYN_NBR_BATCH 8
+#endif
+// entries per round: the double-cosine kernel's sin/cos bodies need more
+// registers per entry, so it reads fewer entries per round
+template <int K> constexpr uint32_t kNbrBatch = (K == KCOS) ? 2u : uint32_t(REYN_NBR_BATCH);
+template <bool IND, uint32_t B, uint32_t S = 1, class F>
 __device__ __forceinline__ bool nbrs4(const GridDev& g, const NbrDev& L, uint32_t t, f3 xi,
     const float4* __restrict__ P, const float* __restrict__ xs, const float* __restrict__ ys,
     const float* __restrict__ zs, const float* __restrict__ ms, F&& f)
@@ -294,25 +287,20 @@ __device__ __forceinline__ bool nbrs4(const GridDev& g, const NbrDev& L, uint32_
     if (L.cnt) {
         const uint32_t c = L.cnt[t];
         if (c <= L.cap) {
-            // four neighbours per round: their index and record loads are all
-            // issued before the first use, which hides gather latency
-            constexpr uint32_t B = 4;
             uint32_t off = t;
             for (uint32_t k = 0; k < c; k += B, off += B * L.n) {
                 uint32_t j[B];
                 float4 q[B];
 #pragma unroll
-                for (uint32_t u = 0; u < B; ++u) // past the count: reload row k (valid)
+                for (uint32_t u = 0; u < B; ++u) // past the count: row k again (valid)
                     j[u] = gat(L.idx, (k + u < c) ? off + u * L.n : off);
 #pragma unroll
                 for (uint32_t u = 0; u < B; ++u)
-                    q[u] = gat(P, j[u]);
+                    q[u] = gat(P, S * j[u]);
 #pragma unroll
                 for (uint32_t u = 0; u < B; ++u) {
-                    if (k + u < c) {
-                        const f3 xij = xi - f3 { q[u].x, q[u].y, q[u].z };
-                        f(j[u], xij, dot(xij, xij), q[u].w);
-                    }
+                    const f3 xij = xi - f3 { q[u].x, q[u].y, q[u].z };
+                    f(j[u], xij, dot(xij, xij), (k + u < c) ? q[u].w : 0.f);
                 }
             }
             return true;
