@@ -186,31 +186,63 @@ __device__ __forceinline__ int3c cell_of(const GridDev& g, f3 x)
 constexpr uint32_t kWalkBudget = 1u << 15;
 
 // Calls r(s, e) for the candidate range [s, e) (cell order) of each of the
-// (2R+1)^2 x-strips of 2R+1 cells around the cell of xi, clipped to the grid.
-// Clipping leaves every point whose cell lies R or more cells inside the grid
-// unchanged.
+// (2R+1)^2 x-strips of 2R+1 cells around the cell of xi, clipped to the grid,
+// in the order z, then y ascending. Clipping leaves every point whose cell
+// lies R or more cells inside the grid unchanged. For R <= 2 the bounds of
+// the strips of one z-plane are loaded together (from clamped, valid rows),
+// so a walk waits for 2R+1 round trips of cell-start loads instead of
+// (2R+1)^2.
 template <class Rg> __device__ __forceinline__ void strips(const GridDev& g, f3 xi, Rg&& r)
 {
     const int3c c = cell_of(g, xi);
     const int R = g.range;
     const int x0 = max(c.x - R, 0), x1 = min(c.x + R, g.nx - 1);
-    const int y0 = max(c.y - R, 0), y1 = min(c.y + R, g.ny - 1);
-    const int z0 = max(c.z - R, 0), z1 = min(c.z + R, g.nz - 1);
     // candidate budget: only degenerate inputs (thousands of points per cell,
     // already reported by the build) reach it; it keeps every walk bounded
     uint32_t budget = kWalkBudget;
-    for (int z = z0; z <= z1; ++z) {
-        for (int y = y0; y <= y1; ++y) {
+    auto emit = [&](uint32_t s, uint32_t e) {
+        if (e - s > budget)
+            e = s + budget;
+        budget -= e - s;
+        r(s, e);
+    };
+    if (R <= 2) {
+        for (int dz = -R; dz <= R; ++dz) {
+            const int z = c.z + dz;
+            const int zc = min(max(z, 0), g.nz - 1);
+            uint32_t s[5], e[5];
+#pragma unroll
+            for (int u = 0; u < 5; ++u) {
+                if (u <= 2 * R) {
+                    const int yc = min(max(c.y - R + u, 0), g.ny - 1);
+                    const uint32_t base = static_cast<uint32_t>(zc * g.nxny + yc * g.nx);
+                    s[u] = gat(g.prefix, base + x0);
+                    e[u] = gat(g.prefix, base + x1 + 1);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 5; ++u) {
+                if (u <= 2 * R) {
+                    const int y = c.y - R + u;
+                    const bool in = z >= 0 && z < g.nz && y >= 0 && y < g.ny;
+                    emit(in ? s[u] : 0u, in ? e[u] : 0u);
+                }
+            }
+        }
+        return;
+    }
+    for (int z = max(c.z - R, 0); z <= min(c.z + R, g.nz - 1); ++z) {
+        for (int y = max(c.y - R, 0); y <= min(c.y + R, g.ny - 1); ++y) {
             const uint32_t base = static_cast<uint32_t>(z * g.nxny + y * g.nx);
-            const uint32_t s = gat(g.prefix, base + x0);
-            uint32_t e = gat(g.prefix, base + x1 + 1);
-            if (e - s > budget)
-                e = s + budget;
-            budget -= e - s;
-            r(s, e);
+            emit(gat(g.prefix, base + x0), gat(g.prefix, base + x1 + 1));
         }
     }
 }
+
+// Candidates of one strip are loaded kWalkBatch at a time, all in flight
+// before the first test (past the end of the strip the last candidate is
+// loaded again and not tested).
+constexpr uint32_t kWalkBatch = 4;
 
 // Calls f(j, x_ij, |x_ij|^2) for every indexed point j with
 // |x_ij|^2 = fmaf-dot <= r_c_2 (the point itself included), in walk order
@@ -220,12 +252,27 @@ __device__ __forceinline__ void walk(const GridDev& g, f3 xi, const float* __res
     const float* __restrict__ ys, const float* __restrict__ zs, F&& f)
 {
     strips(g, xi, [&](uint32_t s, uint32_t e) {
-        for (uint32_t q = s; q < e; ++q) {
-            const uint32_t j = IND ? gat(g.sorted, q) : q;
-            const f3 xij = xi - gat3(xs, ys, zs, j);
-            const float d2 = dot(xij, xij);
-            if (d2 <= g.r2)
-                f(j, xij, d2);
+        constexpr uint32_t B = kWalkBatch;
+        for (uint32_t q = s; q < e; q += B) {
+            uint32_t j[B];
+            f3 pj[B];
+#pragma unroll
+            for (uint32_t u = 0; u < B; ++u) {
+                const uint32_t qq = min(q + u, e - 1);
+                j[u] = IND ? gat(g.sorted, qq) : qq;
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < B; ++u)
+                pj[u] = gat3(xs, ys, zs, j[u]);
+#pragma unroll
+            for (uint32_t u = 0; u < B; ++u) {
+                if (q + u < e) {
+                    const f3 xij = xi - pj[u];
+                    const float d2 = dot(xij, xij);
+                    if (d2 <= g.r2)
+                        f(j[u], xij, d2);
+                }
+            }
         }
     });
 }
@@ -235,13 +282,27 @@ template <bool IND, class F>
 __device__ __forceinline__ void walk4(const GridDev& g, f3 xi, const float4* __restrict__ P, F&& f)
 {
     strips(g, xi, [&](uint32_t s, uint32_t e) {
-        for (uint32_t q = s; q < e; ++q) {
-            const uint32_t j = IND ? gat(g.sorted, q) : q;
-            const float4 pj = gat(P, j);
-            const f3 xij = xi - f3 { pj.x, pj.y, pj.z };
-            const float d2 = dot(xij, xij);
-            if (d2 <= g.r2)
-                f(j, xij, d2, pj.w);
+        constexpr uint32_t B = kWalkBatch;
+        for (uint32_t q = s; q < e; q += B) {
+            uint32_t j[B];
+            float4 pj[B];
+#pragma unroll
+            for (uint32_t u = 0; u < B; ++u) {
+                const uint32_t qq = min(q + u, e - 1);
+                j[u] = IND ? gat(g.sorted, qq) : qq;
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < B; ++u)
+                pj[u] = gat(P, j[u]);
+#pragma unroll
+            for (uint32_t u = 0; u < B; ++u) {
+                if (q + u < e) {
+                    const f3 xij = xi - f3 { pj[u].x, pj[u].y, pj[u].z };
+                    const float d2 = dot(xij, xij);
+                    if (d2 <= g.r2)
+                        f(j[u], xij, d2, pj[u].w);
+                }
+            }
         }
     });
 }
