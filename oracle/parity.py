@@ -86,9 +86,9 @@ def sph_oracle(kernel: int, h: float, rho0: float):
 
 
 def run_iisph_case(ctx, kernel="B3", dt=2e-3, side=14, warm=True, stop_rule=capi.IISPH_STOP_ANY, eta=0.005,
-                   min_iter=3, max_iter=0, seed=7, p0_scale=200.0):
-    sc, v = make_case(side=side, seed=seed)
-    case = GpuCase(ctx, sc, v)
+                   min_iter=3, max_iter=0, seed=7, p0_scale=200.0, cell_ratio=0.5, compress=0.97):
+    sc, v = make_case(side=side, seed=seed, compress=compress)
+    case = GpuCase(ctx, sc, v, cell_ratio)
     solver = IISPH(ctx, kernel, sc.n, NU, sc.spacing, sc.rho_0, eta_rho_max=eta, min_iter=min_iter,
                    max_iter=max_iter, stop_rule=stop_rule, warm_start=warm)
     p0 = np.abs(np.random.default_rng(seed + 2).normal(0.0, p0_scale, sc.n)).astype(np.float32)
@@ -112,9 +112,9 @@ def run_iisph_case(ctx, kernel="B3", dt=2e-3, side=14, warm=True, stop_rule=capi
     return errs, gpu, ref
 
 
-def run_sesph_case(ctx, kernel="C2", dt=1e-3, side=14, k=1000.0, seed=11):
-    sc, v = make_case(side=side, seed=seed)
-    case = GpuCase(ctx, sc, v)
+def run_sesph_case(ctx, kernel="C2", dt=1e-3, side=14, k=1000.0, seed=11, cell_ratio=0.5, compress=0.97):
+    sc, v = make_case(side=side, seed=seed, compress=compress)
+    case = GpuCase(ctx, sc, v, cell_ratio)
     solver = SESPH(ctx, kernel, sc.n, NU, sc.spacing, sc.rho_0, k=k)
     case.grid()
     fg, st, ob = case.oracle_inputs()
@@ -130,9 +130,10 @@ def run_sesph_case(ctx, kernel="C2", dt=1e-3, side=14, k=1000.0, seed=11):
     return errs, gpu, ref
 
 
-def run_pcisph_case(ctx, kernel="W6", dt=1e-3, side=14, seed=13, min_iter=3, max_iter=0):
-    sc, v = make_case(side=side, seed=seed)
-    case = GpuCase(ctx, sc, v)
+def run_pcisph_case(ctx, kernel="W6", dt=1e-3, side=14, seed=13, min_iter=3, max_iter=0, cell_ratio=0.5,
+                    compress=0.97):
+    sc, v = make_case(side=side, seed=seed, compress=compress)
+    case = GpuCase(ctx, sc, v, cell_ratio)
     solver = PCISPH(ctx, kernel, sc.n, NU, sc.spacing, sc.rho_0, min_iter=min_iter, max_iter=max_iter)
     case.grid()
     fg, st, ob = case.oracle_inputs()

@@ -65,6 +65,26 @@ struct Scratch {
 // Neighbours kept per particle; a fluid particle at rest density has about
 // (4/3) pi 2^3 = 33.5 neighbours within the support 2h.
 constexpr uint32_t kListCap = 64;
+// Minimum waves per SIMD each neighbour kernel is compiled for (the register
+// budget follows: 512 / waves VGPRs per lane).
+#ifndef REYN_OCC_DENSITY
+#define REYN_OCC_DENSITY 8
+#endif
+#ifndef REYN_OCC_LOOP
+#define REYN_OCC_LOOP 4
+#endif
+#ifndef REYN_OCC_PACC
+#define REYN_OCC_PACC 6
+#endif
+#define REYN_OCC(w) __attribute__((amdgpu_waves_per_eu(w)))
+
+// LDS staging of the list rows in the density pass: entries per lane (the
+// spread of neighbour counts inside one wavefront reaches about 24 at a free
+// surface), and the fewest rows worth a flush.
+#ifndef REYN_LIST_RING
+#define REYN_LIST_RING 16
+#endif
+constexpr uint32_t kRing = REYN_LIST_RING, kFlush = 4;
 
 // lists: the solver-step scratch (packed records, and the neighbour list
 // while its entries fit 32-bit byte offsets); otherwise only the boundary
@@ -148,31 +168,72 @@ __global__ __launch_bounds__(kBlock) void k_pack(FluidDev f, float4* __restrict_
 // Densities; with GB also the boundary sums, and with a list in L the
 // neighbour list of the step (every later fluid loop reads it).
 template <int K, bool IND, bool GB>
-__global__ __launch_bounds__(kBlock) void k_density(FluidDev f, GridDev g, BdyDev b, KParams kp,
+__global__ __launch_bounds__(kBlock) REYN_OCC(REYN_OCC_DENSITY) void k_density(FluidDev f, GridDev g, BdyDev b, KParams kp,
     float* __restrict__ rho, float* __restrict__ gbx, float* __restrict__ gby,
     float* __restrict__ gbz, float* __restrict__ rb, NbrDev L, const float4* __restrict__ P4,
     float4* __restrict__ X)
 {
     const uint32_t t = global_index();
-    if (t >= f.n)
+    const bool live = t < f.n;
+    const bool listing = L.cnt && P4;
+    if (!live && !listing)
         return;
-    const uint32_t i = particle_of<IND>(g, t);
-    const f3 xi = ld3(f.x, f.y, f.z, i);
+    // lanes past the end stay for the list staging below, whose row flushes
+    // take minima over the whole wavefront; they record nothing
+    const uint32_t i = live ? particle_of<IND>(g, t) : 0u;
+    const f3 xi = live ? ld3(f.x, f.y, f.z, i) : g.bmin;
     ksum acc;
     bool summed = false;
-    if (L.cnt && P4) {
+    if (listing) {
         // the walk only tests candidates and records neighbours; the sum runs
         // over the compact list afterwards, so the kernel body is not executed
         // once per candidate of every lane of the wavefront
-        uint32_t cnt = 0, off = t;
-        walk4<IND>(g, xi, P4, [&](uint32_t j, f3, float, float) {
-            if (j != i) {
-                if (cnt < L.cap)
-                    L.idx[off] = j;
-                off += L.n;
+        //
+        // Lanes find their k-th neighbour at different times, so storing
+        // entries as found would write each list row in many partial pieces.
+        // Each lane instead keeps its entries in its own column of an LDS
+        // ring, and after every strip the wavefront writes the rows that all
+        // its lanes have filled, one contiguous row per store; the last rows
+        // are written at the end. A lane whose ring is full stores its later
+        // entries directly.
+        __shared__ uint32_t ring[kBlock / kWave][kRing][kWave];
+        const uint32_t wv = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+        uint32_t cnt = 0, flushed = 0, direct = L.cap; // `flushed` is uniform
+        auto flush = [&](uint32_t end) { // rows [flushed, end) from the ring
+            const uint32_t have = min(cnt, direct);
+            for (uint32_t r = flushed; r < end; ++r)
+                if (r < have)
+                    L.idx[t + r * L.n] = ring[wv][r % kRing][lane];
+            flushed = end;
+        };
+        walk4<IND>(
+            g, xi, P4,
+            [&](uint32_t j, f3, float, float) {
+                if (!live || j == i)
+                    return;
+                if (cnt < L.cap) {
+                    if (cnt >= direct) {
+                        L.idx[t + cnt * L.n] = j;
+                    } else if (cnt < flushed + kRing) {
+                        ring[wv][cnt % kRing][lane] = j;
+                    } else {
+                        direct = cnt;
+                        L.idx[t + cnt * L.n] = j;
+                    }
+                }
                 ++cnt;
-            }
-        });
+            },
+            [&] {
+                // write rows while every lane has filled the next kFlush (a
+                // lane that stores directly, or one past the end, holds back
+                // no row)
+                const uint32_t full = (!live || cnt >= direct) ? L.cap : min(cnt, L.cap);
+                while (__ballot(full < flushed + kFlush) == 0 && flushed + kFlush <= L.cap)
+                    flush(flushed + kFlush);
+            });
+        flush(wave_max_u32(live ? min(cnt, direct) : 0u));
+        if (!live)
+            return;
         L.cnt[t] = cnt;
         if (cnt <= L.cap) {
             acc.add(gat(f.m, i) * kW<K>(kp, 0.f)); // the particle itself
@@ -216,7 +277,7 @@ __global__ __launch_bounds__(kBlock) void k_density(FluidDev f, GridDev g, BdyDe
 // PK: the interleaved records X hold {x, m} and {v, rho} of every particle
 // (u = v, w = rho); otherwise the loop walks the SoA arrays.
 template <int K, bool IND, bool PK>
-__global__ __launch_bounds__(kBlock) void k_nonpressure(FluidDev f, GridDev g, SphK c,
+__global__ __launch_bounds__(kBlock) REYN_OCC(REYN_OCC_LOOP) void k_nonpressure(FluidDev f, GridDev g, SphK c,
     const float* __restrict__ rho, float* __restrict__ ax, float* __restrict__ ay,
     float* __restrict__ az, NbrDev L, const float4* __restrict__ X)
 {
@@ -410,7 +471,7 @@ __device__ __forceinline__ float jacobi_update(float omega, float aii, float Ap,
 
 // PK: u of X holds the updated velocities.
 template <int K, bool IND, bool PK>
-__global__ __launch_bounds__(kBlock) void k_iisph_aii_s(FluidDev f, GridDev g, int bdy, KParams kp,
+__global__ __launch_bounds__(kBlock) REYN_OCC(REYN_OCC_LOOP) void k_iisph_aii_s(FluidDev f, GridDev g, int bdy, KParams kp,
     float rho0, float dt, float omega, int warm, const float* __restrict__ rho,
     const float* __restrict__ gbx, const float* __restrict__ gby, const float* __restrict__ gbz,
     float* __restrict__ aii, float* __restrict__ si, float* __restrict__ p, NbrDev L,
@@ -453,7 +514,7 @@ __global__ __launch_bounds__(kBlock) void k_iisph_aii_s(FluidDev f, GridDev g, i
 // (ctl != NULL), writes the result to u of X for the Jacobi pass; the final
 // pass and the SoA mode write the work arrays.
 template <int K, bool IND, bool PK>
-__global__ __launch_bounds__(kBlock) void k_iisph_pacc(FluidDev f, GridDev g, int bdy, KParams kp,
+__global__ __launch_bounds__(kBlock) REYN_OCC(REYN_OCC_PACC) void k_iisph_pacc(FluidDev f, GridDev g, int bdy, KParams kp,
     float rho0_sq_inv, const float* __restrict__ p, const float* __restrict__ gbx,
     const float* __restrict__ gby, const float* __restrict__ gbz, float* __restrict__ ax,
     float* __restrict__ ay, float* __restrict__ az, const SolveCtl* ctl, NbrDev L,
@@ -482,7 +543,7 @@ __global__ __launch_bounds__(kBlock) void k_iisph_pacc(FluidDev f, GridDev g, in
 
 // PK: the accelerations come from u of X; otherwise from the SoA work arrays.
 template <int K, bool IND, bool PK>
-__global__ __launch_bounds__(kBlock) void k_iisph_jacobi(FluidDev f, GridDev g, int bdy, KParams kp,
+__global__ __launch_bounds__(kBlock) REYN_OCC(REYN_OCC_LOOP) void k_iisph_jacobi(FluidDev f, GridDev g, int bdy, KParams kp,
     float dt, float omega, float thr, const float* __restrict__ ax, const float* __restrict__ ay,
     const float* __restrict__ az, const float* __restrict__ aii,
     const float* __restrict__ si, const float* __restrict__ gbx, const float* __restrict__ gby,

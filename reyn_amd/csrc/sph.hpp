@@ -187,8 +187,10 @@ constexpr uint32_t kWalkBudget = 1u << 15;
 
 // Calls r(s, e) for the candidate range [s, e) (cell order) of each of the
 // (2R+1)^2 x-strips of 2R+1 cells around the cell of xi, clipped to the grid,
-// in the order z, then y ascending. Clipping leaves every point whose cell
-// lies R or more cells inside the grid unchanged. For R <= 2 the bounds of
+// in the order z, then y ascending. Every lane makes all (2R+1)^2 calls, with
+// an empty range for a strip outside the grid, so a call is a point the whole
+// wavefront reaches together. Clipping leaves every point whose cell lies R
+// or more cells inside the grid unchanged. For R <= 2 the bounds of
 // the strips of one z-plane are loaded together (from clamped, valid rows),
 // so a walk waits for 2R+1 round trips of cell-start loads instead of
 // (2R+1)^2.
@@ -231,10 +233,16 @@ template <class Rg> __device__ __forceinline__ void strips(const GridDev& g, f3 
         }
         return;
     }
-    for (int z = max(c.z - R, 0); z <= min(c.z + R, g.nz - 1); ++z) {
-        for (int y = max(c.y - R, 0); y <= min(c.y + R, g.ny - 1); ++y) {
-            const uint32_t base = static_cast<uint32_t>(z * g.nxny + y * g.nx);
-            emit(gat(g.prefix, base + x0), gat(g.prefix, base + x1 + 1));
+    for (int dz = -R; dz <= R; ++dz) {
+        for (int dy = -R; dy <= R; ++dy) {
+            const int z = c.z + dz, y = c.y + dy;
+            uint32_t s = 0, e = 0;
+            if (z >= 0 && z < g.nz && y >= 0 && y < g.ny) {
+                const uint32_t base = static_cast<uint32_t>(z * g.nxny + y * g.nx);
+                s = gat(g.prefix, base + x0);
+                e = gat(g.prefix, base + x1 + 1);
+            }
+            emit(s, e);
         }
     }
 }
@@ -278,8 +286,11 @@ __device__ __forceinline__ void walk(const GridDev& g, f3 xi, const float* __res
 }
 
 // The same walk over packed records P[j] = {x, y, z, m}; f(j, x_ij, d2, m_j).
-template <bool IND, class F>
-__device__ __forceinline__ void walk4(const GridDev& g, f3 xi, const float4* __restrict__ P, F&& f)
+// `strip_end()` runs after each strip, at a point every lane of the
+// wavefront that entered the walk reaches together.
+template <bool IND, class F, class E>
+__device__ __forceinline__ void walk4(const GridDev& g, f3 xi, const float4* __restrict__ P, F&& f,
+    E&& strip_end)
 {
     strips(g, xi, [&](uint32_t s, uint32_t e) {
         constexpr uint32_t B = kWalkBatch;
@@ -304,7 +315,13 @@ __device__ __forceinline__ void walk4(const GridDev& g, f3 xi, const float4* __r
                 }
             }
         }
+        strip_end();
     });
+}
+template <bool IND, class F>
+__device__ __forceinline__ void walk4(const GridDev& g, f3 xi, const float4* __restrict__ P, F&& f)
+{
+    walk4<IND>(g, xi, P, f, [] {});
 }
 
 // ---- per-step neighbour lists ------------------------------------------------

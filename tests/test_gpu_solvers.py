@@ -92,6 +92,33 @@ def test_iisph_step_parity(ctx, kernel, warm):
     assert errs["dx"] <= TOL_DIFF, errs
 
 
+@pytest.mark.parametrize("ratio", [0.25, 1.0, 2.0])
+def test_iisph_step_parity_cell_ratios(ctx, ratio):
+    # larger cells: longer strips, so the density pass stages many list
+    # entries per strip and exercises its direct-store paths; smaller cells:
+    # more strips than the plane-batched walk handles (range 4)
+    errs, gpu, ref = parity.run_iisph_case(ctx, cell_ratio=ratio)
+    assert gpu["iterations"] >= 3
+    assert errs["rho"] <= TOL and errs["p"] <= TOL and errs["x"] <= TOL, errs
+    assert errs["a_ii"] <= TOL_DIFF and errs["s_i"] <= TOL_DIFF and errs["v"] <= TOL_DIFF, errs
+
+
+def test_list_cap_overflow_parity(ctx):
+    # at 0.8 s spacing a particle has about 64 neighbours, around the list
+    # capacity: part of the particles keep a list, the rest walk the grid in
+    # every loop of the step
+    sc, _ = parity.make_case(compress=0.8)
+    d = sc.x[:, None, :].astype(np.float64) - sc.x[None, :, :]
+    counts = ((d * d).sum(-1) <= (2.0 * sc.spacing) ** 2).sum(1) - 1
+    assert (counts > 64).any() and (counts <= 64).any()
+    errs, gpu, ref = parity.run_iisph_case(ctx, compress=0.8, max_iter=12)
+    assert errs["rho"] <= TOL and errs["p"] <= TOL and errs["x"] <= TOL, errs
+    assert errs["a_ii"] <= TOL_DIFF and errs["s_i"] <= TOL_DIFF and errs["v"] <= TOL_DIFF, errs
+    errs, gpu, ref = parity.run_sesph_case(ctx, compress=0.8, cell_ratio=1.0)
+    assert errs["rho"] <= TOL and errs["x"] <= TOL, errs
+    assert errs["a"] <= TOL_DIFF and errs["v"] <= TOL_DIFF, errs
+
+
 def test_iisph_stop_rules(ctx):
     # the reference rule stops once no particle exceeds eta rho_0 (after >= 3)
     errs, gpu, ref = parity.run_iisph_case(ctx, stop_rule=capi.IISPH_STOP_ANY, eta=0.005)
