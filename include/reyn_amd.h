@@ -259,7 +259,7 @@ typedef struct reyn_sesph_work {
     float *ax, *ay, *az;
 } reyn_sesph_work;
 
-/* One `SESPH<K,R>::step` (SESPH.cu:138-166): densities, accelerations with
+/* One `SESPH<K,R>::step` (SESPH.cu:67-95): densities, accelerations with
  * the normalised EOS max(0, k(rho/rho_0 - 1)), integration. */
 int reyn_sesph_step(reyn_ctx* ctx, const reyn_sesph_params* prm,
     const reyn_particles* fluid, const reyn_grid* grid,
@@ -292,7 +292,7 @@ typedef struct reyn_solve_info {
     double avg_density;     /* PCISPH: FP64 mean predicted density */
 } reyn_solve_info;
 
-/* One `PCISPH<K,R>::step` (PCISPH.cu:171-224). The loop
+/* One `PCISPH<K,R>::step` (PCISPH.cu:58-111). The loop
  * `while (l < min_iter || avg(rho) > (1 + eta) rho_0)` runs on the device,
  * with the average formed in FP64. */
 int reyn_pcisph_step(reyn_ctx* ctx, const reyn_pcisph_params* prm,
@@ -304,7 +304,7 @@ int reyn_pcisph_step(reyn_ctx* ctx, const reyn_pcisph_params* prm,
 
 enum {
     /* the reference rule: stop once no particle has a predicted density error
-     * above eta * rho_0 (IISPH.cu:249-255, :340) */
+     * above eta * rho_0 (IISPH.cu:153-161, :246) */
     REYN_IISPH_STOP_ANY = 0,
     /* stop once mean_i max(0, err_i) / rho_0 <= eta, with the mean formed in
      * FP64 on the device */
@@ -318,7 +318,7 @@ typedef struct reyn_iisph_params {
     uint32_t min_iter; /* 3 (IISPH.cuh:43) */
     uint32_t max_iter; /* 0 = no cap, as in the reference */
     int32_t stop_rule; /* REYN_IISPH_STOP_* */
-    int32_t warm_start; /* 1: p *= 0.5 (IISPH.cuh:48, IISPH.cu:293-309) */
+    int32_t warm_start; /* 1: p *= 0.5 (IISPH.cuh:48, IISPH.cu:199-215) */
     uint32_t check_every; /* iterations launched between host checks; 0 = 4 */
 } reyn_iisph_params;
 
@@ -330,7 +330,7 @@ typedef struct reyn_iisph_work {
     float *ax, *ay, *az;
 } reyn_iisph_work;
 
-/* One `IISPH<K,R>::step` (IISPH.cu:258-356): densities, non-pressure
+/* One `IISPH<K,R>::step` (IISPH.cu:164-262): densities, non-pressure
  * accelerations, a_ii and s_i, the relaxed-Jacobi pressure loop with its
  * convergence test on the device, final pressure accelerations and
  * integration. */
