@@ -306,6 +306,79 @@ def test_iisph_float_port_tracks_fp64():
         assert np.abs(a - b).max() <= 1e-4 * np.abs(a).max(), k
 
 
+def test_iisph_pair_pressure_iteration_closed_form():
+    """A compressed pair at rest, no gravity, viscosity or boundary, cold
+    start: by symmetry p_0 = p_1 = p, a_0 = -a_1 = -(2 m p / rho0^2) grad W_01,
+    (Ap)_i = 2 a_ii p, so the relaxed Jacobi iterate is p <- p + max(0,
+    omega (s / a_ii - 2 p)) from p = omega s / a_ii (IISPH.cu:68-162,
+    :199-215), and the step ends with x_0 += dt^2 a_0 (solver.cu:15-32)."""
+    h, dt, m, rho0, omega = 0.05, 1e-3, 0.6, 1000.0, 0.3
+    x = np.array([[0.5, 0.5, 0.5], [0.56, 0.5, 0.5]], np.float32)
+    st = state_of(x, m=m)
+    g = small_grid(x, 2 * h, lo=np.zeros(3), hi=np.ones(3))
+    prm = orc.sph(1, 2 * h, h, 0.0, rho0, g=(0.0, 0.0, 0.0))
+    ip = orc.OrcIisph(omega, 0.005, 3, 0, 0, 0)
+    res = orc.iisph_step(g, st, None, prm, ip, np.zeros(2), dt, fixed_iters=3)
+    f32 = lambda v: float(np.float32(v))  # parameters cross the interface as floats
+    dt, omega = f32(dt), f32(omega)
+    d = float(np.float32(0.56) - np.float32(0.5))
+    W, G = orc.kernel_eval(1, 2 * h, np.array([[0.0, 0, 0], [-d, 0, 0]]))
+    gw2 = (G[1] ** 2).sum()
+    rho = m * (W[0] + W[1])
+    a_ii = -2 * dt * m * m * gw2 / rho0 ** 2
+    c = ((rho0 - rho) / dt) / a_ii
+    assert c > 0  # compressed: a positive pressure solves the pair
+    p = omega * c
+    for _ in range(3):
+        p = p + max(0.0, omega * (c - 2 * p))
+    np.testing.assert_allclose(res["rho"], [rho, rho], rtol=1e-12)
+    np.testing.assert_allclose(res["p"], [p, p], rtol=1e-10)
+    a0 = -(2 * m * p / rho0 ** 2) * G[1]
+    np.testing.assert_allclose(st.x[0] - 0.5, dt * dt * a0[0], rtol=1e-6)
+    np.testing.assert_allclose(st.x[1] - float(np.float32(0.56)), -dt * dt * a0[0], rtol=1e-6)
+
+
+def test_pcisph_free_fall_and_pair_momentum():
+    """PCISPH step (PCISPH.cu:58-111): gravity and viscosity enter the
+    velocity in the first loop pass and `a` ends as the last pressure
+    acceleration. An isolated particle is below rest density, so its pressure
+    is 0: after exactly min_iter passes a = 0, v = g dt, y += dt^2 g. A
+    compressed pair's pressure and viscosity forces are equal and opposite:
+    the final pressure accelerations cancel and over all passes the momentum
+    gains exactly 2 m g dt."""
+    h, dt = 0.05, 1e-3
+    prm = orc.sph(2, 2 * h, h, 0.001, 1000.0)
+    delta = orc.pcisph_delta_f64(2, 2 * h, h)
+    g32, dt32 = float(np.float32(-9.81)), float(np.float32(dt))
+    x = np.array([[0.5, 0.5, 0.5]], np.float32)
+    st = state_of(x, m=0.125)
+    g = small_grid(x, 2 * h, lo=np.zeros(3), hi=np.ones(3))
+    it, rho, a, avg = orc.pcisph_step(g, st, None, prm, delta, dt)
+    assert it == 3
+    np.testing.assert_array_equal(a[0], [0, 0, 0])
+    assert st.vy[0] == pytest.approx(g32 * dt32, rel=1e-15)
+    assert st.y[0] == pytest.approx(float(np.float32(0.5)) + dt32 * dt32 * g32, rel=1e-14)
+    x = np.array([[0.5, 0.5, 0.5], [0.53, 0.52, 0.49]], np.float32)
+    v = np.array([[0.3, 0.0, 0.1], [-0.2, 0.1, 0.0]], np.float32)
+    st = state_of(x, v, m=0.6)
+    g = small_grid(x, 2 * h, lo=np.zeros(3), hi=np.ones(3))
+    p0 = v.astype(np.float64).sum(axis=0)
+    it, rho, a, avg = orc.pcisph_step(g, st, None, prm, delta, dt, max_iter=20)
+    assert abs(st.vx[0] - st.vx[1]) > 1.0  # the pressure pushed the pair apart
+    np.testing.assert_allclose(a.sum(axis=0), 0.0, atol=1e-12)
+    p1 = np.array([st.vx.sum(), st.vy.sum(), st.vz.sum()])
+    np.testing.assert_allclose(p1 - p0, [0, 2 * g32 * dt32, 0], atol=1e-12)
+
+
+def test_gather_is_the_permutation():
+    """The reorder gather (buffer.cuh:184-209) is numpy's fancy indexing."""
+    rng = np.random.default_rng(5)
+    src = rng.normal(size=1000).astype(np.float32)
+    perm = rng.permutation(1000).astype(np.uint32)
+    np.testing.assert_array_equal(orc.gather(perm, src), src[perm])
+    assert orc.gather(np.zeros(0, np.uint32), np.zeros(0, np.float32)).size == 0
+
+
 def test_boundary_masses_fixed_point():
     """On a large flat 2-layer wall the refined mass of a central sample tends
     to rho0 / sum_j W_bj (the Akinci fixed point, sample_boundary.cu:251-297)."""
