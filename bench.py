@@ -23,6 +23,7 @@ import json
 import os
 import sys
 import time
+from pathlib import Path
 
 import numpy as np
 
@@ -64,6 +65,19 @@ def step_bytes(n: int, nb: int, ncells: int, iters: float) -> float:
     """Compulsory bytes of one IISPH step under the reference's phase
     structure: 372N + 48N_b + 28C + l (72N + 32N_b)."""
     return 372.0 * n + 48.0 * nb + 28.0 * ncells + iters * (72.0 * n + 32.0 * nb)
+
+
+def measured_traffic(kernel: str) -> float | None:
+    """HBM bytes per launch of `kernel` from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (scaled by the copy-kernel calibration it
+    records), or None when the summary does not cover the kernel."""
+    path = Path(__file__).resolve().parent / "profiles" / "hbm_traffic.json"
+    try:
+        table = json.loads(path.read_text())["bytes_per_launch"]
+    except (OSError, ValueError, KeyError):
+        return None
+    v = table.get(kernel)
+    return float(v) if v is not None else None
 
 
 def parse_profile(text: str) -> dict:
@@ -129,7 +143,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--side", type=int, default=128, help="fluid lattice side (128 -> 2,097,152 particles)")
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=10,
+                    help="IISPH steps of the CPU baseline sample (about 1 s each on 16 cores)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="time without per-launch events")
     ap.add_argument("--dt-policy", choices=("controller", "cfl"), default="controller",
@@ -234,7 +249,7 @@ def main() -> int:
                 best = (name, cnt, tot, by, avg_us, gbs)
         name, cnt, tot, by, avg_us, gbs = best
         roofline = {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(gbs / HBM_PEAK_GBS, 5), "traffic": None, "kernel": name,
+                    "frac": round(gbs / HBM_PEAK_GBS, 5), "traffic": measured_traffic(name), "kernel": name,
                     "bytes_per_launch": by, "avg_launch_us": round(avg_us, 2), "launches": cnt,
                     "share_of_step_time": round(tot / gpu_ms, 4)}
     mean_iters = float(np.mean(iters))
