@@ -1,0 +1,216 @@
+"""GPU parity at the sizes of the benchmark configurations, against the FP64
+oracle, through the C-ABI.
+
+- Configuration 2 (grid + SESPH, 10^6 particles, cell ratios 0.5/1/2, the
+  grid used without reordering as between the reorders of that workload):
+  neighbour counts bit-exact, one SESPH step with rho and x within 1e-5 and
+  a, v within 1e-4 (relative to the largest value of each field).
+- Configuration 1 (SESPH dam break, 32^3, CFL 0.4, 1000 steps, cell = support
+  radius): the same per-step bar at sampled steps of the GPU's own rollout,
+  each step from the GPU's state.
+- Configuration 3 (IISPH dam break, here at 16^3): a 500-step GPU trajectory
+  against the oracle's trajectory from the same initial state (each step in
+  FP64, state stored in FP32, the GPU's time steps and iteration counts); the
+  per-field distance is printed at checkpoints and bounded by 3x the distance
+  of the oracle's own FP32 port run the same way (the dynamics amplify
+  rounding differences along any trajectory).
+"""
+
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oracle import binding as orc
+from reyn_amd import capi, scenes
+from reyn_amd.sim import (IISPH, SESPH, BoundarySamples, Particles, Simulation, UniformGridBuilder,
+                          hard_enforce_bounds)
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+TOL_DIFF = 1e-4
+NU = 0.001
+
+
+def rel_err(gpu, ref):
+    gpu = np.asarray(gpu, np.float64)
+    ref = np.asarray(ref, np.float64)
+    return float(np.abs(gpu - ref).max() / np.abs(ref).max())
+
+
+def oracle_bdy(bdy, r, ratio):
+    xb = bdy.positions()
+    bgeom = orc.geometry(bdy.bound_min, bdy.bound_max, r, ratio)
+    bgrid = orc.grid_build(bgeom, xb[:, 0], xb[:, 1], xb[:, 2])
+    assert np.array_equal(bgrid.sorted, np.arange(xb.shape[0])), "boundary is not in cell order"
+    return orc.Bdy(bgrid.resorted(), xb[:, 0], xb[:, 1], xb[:, 2], bdy.m.numpy())
+
+
+def oracle_state(hs):
+    return orc.State({"x": hs["xx"], "y": hs["xy"], "z": hs["xz"], "vx": hs["vx"], "vy": hs["vy"],
+                      "vz": hs["vz"], "m": hs["m"]})
+
+
+def compare_sesph(gpu_solver, state, st, rho, a):
+    out = state.host()
+    got = {"rho": gpu_solver.rho.numpy(),
+           "a": np.stack([gpu_solver.ax.numpy(), gpu_solver.ay.numpy(), gpu_solver.az.numpy()], 1),
+           "x": np.stack([out["xx"], out["xy"], out["xz"]], 1), "v": np.stack([out["vx"], out["vy"], out["vz"]], 1)}
+    ref = {"rho": rho, "a": a, "x": np.stack([st.x, st.y, st.z], 1), "v": np.stack([st.vx, st.vy, st.vz], 1)}
+    return {k: rel_err(got[k], ref[k]) for k in got}
+
+
+@pytest.mark.parametrize("ratio", [0.5, 1.0, 2.0])
+def test_config2_counts_and_sesph_step_full_size(ctx, ratio):
+    sc = scenes.config2_grid(side=100)
+    h, r = sc.spacing, 2.0 * sc.spacing
+    n = sc.n
+    print(f"config 2: {n} fluid, {sc.xb.shape[0]} boundary, ratio {ratio}", flush=True)
+    v = np.random.default_rng(21).normal(0.0, 0.1, size=sc.x.shape).astype(np.float32)
+    state = Particles(ctx, sc.x, v, sc.mass, sc.rho_0)
+    bdy = BoundarySamples(ctx, sc.xb, h, sc.rho_0, ratio)
+    builder = UniformGridBuilder(ctx, sc.bound_min, sc.bound_max, r, ratio)
+    g = builder.construct(r, state.xx, state.xy, state.xz)  # Resort::no: the grid indexes the unsorted state
+    hs = state.host()
+    og = orc.grid_build(orc.geometry(sc.bound_min, sc.bound_max, r, ratio), hs["xx"], hs["xy"], hs["xz"])
+    cnt = ctx.empty(n, np.uint32)
+    l2, vs = ctx.empty(n), [ctx.empty(n) for _ in range(3)]
+    ctx.check(ctx._lib.reyn_grid_query_stats(ctx.h, C.byref(g), state.xx._ptr, state.xy._ptr, state.xz._ptr, n,
+                                             cnt._ptr, l2._ptr, vs[0]._ptr, vs[1]._ptr, vs[2]._ptr))
+    ref_cnt, _, _ = orc.query_stats(og, hs["xx"], hs["xy"], hs["xz"])
+    np.testing.assert_array_equal(cnt.numpy(), ref_cnt)
+    solver = SESPH(ctx, "C2", n, NU, h, sc.rho_0, k=1000.0)
+    st = oracle_state(hs)
+    ob = oracle_bdy(bdy, r, ratio)
+    dt = 5e-4
+    solver.step(state, g, bdy, dt)
+    rho, a = orc.sesph_step(og, st, ob, orc.sph(0, r, h, NU, sc.rho_0), 1000.0, dt)
+    errs = compare_sesph(solver, state, st, rho, a)
+    print(f"config 2 ratio {ratio}: {errs}", flush=True)
+    assert errs["rho"] <= TOL and errs["x"] <= TOL, errs
+    assert errs["a"] <= TOL_DIFF and errs["v"] <= TOL_DIFF, errs
+
+
+def test_config1_sesph_rollout_per_step_parity(ctx):
+    sc = scenes.config1_sesph()
+    h, r, ratio, k = sc.spacing, 2.0 * sc.spacing, 1.0, 1000.0
+    state = Particles(ctx, sc.x, None, sc.mass, sc.rho_0)
+    bdy = BoundarySamples(ctx, sc.xb, h, sc.rho_0, ratio)
+    solver = SESPH(ctx, "C2", sc.n, NU, h, sc.rho_0, k=k)
+    sim = Simulation(ctx, solver, state, bdy, h, sc.bound_min, sc.bound_max, cfl_lambda=0.4, cell_ratio=ratio)
+    ob = oracle_bdy(bdy, r, ratio)
+    prm = orc.sph(0, r, h, NU, sc.rho_0)
+    geom = orc.geometry(sc.bound_min, sc.bound_max, r, ratio)
+    checks = {0, 1, 10, 100, 500, 999}
+    for step in range(1000):
+        dt = sim.next_dt()
+        g = sim.get_grid()
+        if step in checks:
+            hs = state.host()
+            og = orc.grid_build(geom, hs["xx"], hs["xy"], hs["xz"])
+            assert np.array_equal(og.sorted, np.arange(sc.n)), "GPU reorder is not in cell order"
+            st = oracle_state(hs)
+        solver.step(state, g, bdy, dt)
+        if step in checks:
+            rho, a = orc.sesph_step(og.resorted(), st, ob, prm, k, dt)
+            errs = compare_sesph(solver, state, st, rho, a)
+            print(f"config 1 step {step}: dt {dt:.3e} mean rho {rho.mean():.1f} {errs}", flush=True)
+            assert errs["rho"] <= TOL and errs["x"] <= TOL, (step, errs)
+            assert errs["a"] <= TOL_DIFF and errs["v"] <= TOL_DIFF, (step, errs)
+        hard_enforce_bounds(ctx, sim.bound_min, sim.bound_max, state)
+        sim.steps += 1
+    x = state.positions()
+    assert np.all(np.isfinite(x)) and np.all(np.isfinite(state.velocities()))
+
+
+def oracle_trajectory(sc, bdy_args, geom, prm, dts, iters, dtype, checkpoints):
+    """The oracle's run of the main loop (grid + reorder, IISPH step, bounds)
+    from the scene's initial state with the given time steps and iteration
+    counts, arithmetic in `dtype`, state stored in FP32. Returns the density
+    and positions per particle id at the checkpoints."""
+    n = sc.n
+    r = 2.0 * sc.spacing
+    bg, xb, mb = bdy_args
+    ob = orc.Bdy(bg, xb[:, 0], xb[:, 1], xb[:, 2], mb, dtype=dtype)
+    ip = orc.OrcIisph(0.5, 0.001, 3, 100, capi.IISPH_STOP_MEAN, 1)
+    x = sc.x.astype(np.float32)
+    v = np.zeros_like(x)
+    m = np.full(n, sc.mass, np.float32)
+    p = np.zeros(n, np.float32)
+    ids = np.arange(n)
+    out = {}
+    for step, (dt, it) in enumerate(zip(dts, iters)):
+        g = orc.grid_build(geom, x[:, 0], x[:, 1], x[:, 2])
+        x, v, m, p, ids = x[g.sorted], v[g.sorted], m[g.sorted], p[g.sorted], ids[g.sorted]
+        st = orc.State({"x": x[:, 0], "y": x[:, 1], "z": x[:, 2], "vx": v[:, 0], "vy": v[:, 1], "vz": v[:, 2],
+                        "m": m}, dtype=dtype)
+        res = orc.iisph_step(g.resorted(), st, ob, prm, ip, p, dt, fixed_iters=it)
+        p = res["p"].astype(np.float32)
+        x, v = orc.hard_enforce_bounds(sc.bound_min, sc.bound_max, np.stack([st.x, st.y, st.z], 1),
+                                       np.stack([st.vx, st.vy, st.vz], 1))
+        if step + 1 in checkpoints:
+            rho = np.empty(n)
+            rho[ids] = res["rho"]
+            xo = np.empty((n, 3))
+            xo[ids] = x
+            out[step + 1] = (rho, xo)
+    return out
+
+
+def test_config3_small_rollout_error_after_500_steps(ctx):
+    """500 steps of the IISPH dam break at 16^3. Each step's pressure solve
+    stops at a 0.1% mean error, so rounding differences grow along any
+    trajectory: the yardstick is the oracle's own FP32 port, run with the same
+    time steps and iteration counts. The GPU must stay within 3x of the
+    port's distance to the FP64 oracle (99th percentile per particle), and
+    every step's solve must reach the 0.1% mean error (or its cap)."""
+    sc = scenes.config3_iisph(side=16)
+    h, r = sc.spacing, 2.0 * sc.spacing
+    n = sc.n
+    print(f"config 3 at 16^3: {n} fluid, {sc.xb.shape[0]} boundary", flush=True)
+    state = Particles(ctx, sc.x, None, sc.mass, sc.rho_0)
+    bdy = BoundarySamples(ctx, sc.xb, h, sc.rho_0)
+    solver = IISPH(ctx, "B3", n, NU, h, sc.rho_0, omega=0.5, eta_rho_max=0.001, min_iter=3, max_iter=100,
+                   stop_rule=capi.IISPH_STOP_MEAN, warm_start=True)
+    solver.pid = ctx.array(np.arange(n, dtype=np.float32))  # particle ids follow every reorder
+    sim = Simulation(ctx, solver, state, bdy, h, sc.bound_min, sc.bound_max, cfl_lambda=1.0,
+                     extra_reorder=("p", "pid"), dt_policy="controller", dt0=5e-4, iter_target=10,
+                     change_factor=0.001)
+    checkpoints = (25, 50, 100, 200, 500)
+    dts, iters, gpu = [], [], {}
+    m0 = float(state.m.numpy().astype(np.float64).sum())
+    for step in range(500):
+        s = sim.step()
+        assert s.iterations == 100 or s.mean_rel_error <= 1e-3, (step, s)
+        dts.append(s.dt)
+        iters.append(s.iterations)
+        if step + 1 in checkpoints:
+            gid = solver.pid.numpy().astype(np.int64)
+            rho = np.empty(n)
+            rho[gid] = solver.rho.numpy()
+            xg = np.empty((n, 3))
+            xg[gid] = state.positions()
+            gpu[step + 1] = (rho, xg)
+    assert np.array_equal(np.sort(solver.pid.numpy().astype(np.int64)), np.arange(n))
+    assert float(state.m.numpy().astype(np.float64).sum()) == m0
+    xb = bdy.positions()
+    bg = orc.grid_build(orc.geometry(bdy.bound_min, bdy.bound_max, r, 0.5), xb[:, 0], xb[:, 1], xb[:, 2])
+    bdy_args = (bg.resorted(), xb, bdy.m.numpy())
+    prm = orc.sph(1, r, h, NU, sc.rho_0)
+    geom = orc.geometry(sc.bound_min, sc.bound_max, r, 0.5)
+    f64 = oracle_trajectory(sc, bdy_args, geom, prm, dts, iters, np.float64, checkpoints)
+    f32 = oracle_trajectory(sc, bdy_args, geom, prm, dts, iters, np.float32, checkpoints)
+    box = float(np.max(sc.bound_max - sc.bound_min))
+    for k in checkpoints:
+        def dist(a):
+            drho = np.abs(a[0] - f64[k][0]) / sc.rho_0
+            dx = np.abs(a[1] - f64[k][1]).max(axis=1) / box
+            return np.percentile(drho, 99), drho.max(), np.percentile(dx, 99), dx.max()
+        g_rho99, g_rhomax, g_x99, g_xmax = dist(gpu[k])
+        c_rho99, c_rhomax, c_x99, c_xmax = dist(f32[k])
+        print(f"step {k:3d}: |rho - rho_fp64|/rho0 p99 gpu {g_rho99:.2e} port {c_rho99:.2e}, max gpu {g_rhomax:.2e} "
+              f"port {c_rhomax:.2e}; |x - x_fp64|/box p99 gpu {g_x99:.2e} port {c_x99:.2e}, max gpu {g_xmax:.2e} "
+              f"port {c_xmax:.2e}", flush=True)
+        assert g_rho99 <= 3.0 * c_rho99 + 1e-6, k
+        assert g_x99 <= 3.0 * c_x99 + 1e-6, k
