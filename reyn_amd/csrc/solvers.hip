@@ -19,14 +19,6 @@
 namespace reyn {
 namespace {
 
-struct SphK {
-    KParams kp;
-    float h, nu, rho0;
-    f3 g;
-    float visc_c; // 10 nu
-    float visc_eps; // 0.01 h^2
-};
-
 SphK make_sphk(const reyn_sph_params& p)
 {
     SphK c;
@@ -37,6 +29,7 @@ SphK make_sphk(const reyn_sph_params& p)
     c.g = { p.g.x, p.g.y, p.g.z };
     c.visc_c = 10.f * p.nu;
     c.visc_eps = static_cast<float>(0.01 * double(p.h) * double(p.h));
+    c.rho0_sq_inv = 1.f / (p.rho_0 * p.rho_0);
     return c;
 }
 
@@ -122,7 +115,10 @@ int get_scratch(reyn_ctx* ctx, uint32_t n, Scratch* s, bool lists = false)
     return REYN_OK;
 }
 
-__device__ __forceinline__ bool loop_done(const SolveCtl* ctl) { return ctl && ctl->done; }
+__device__ __forceinline__ bool loop_done(const SolveCtl* ctl, const SceneDev& sc)
+{
+    return ctl && ctl[sc.idx].done;
+}
 
 // Interleaved records X of the loops after the density pass: X[2j] holds
 // {x, y, z, m} of particle j and X[2j+1] an auxiliary slot {u, w} whose
@@ -154,11 +150,14 @@ __device__ __forceinline__ void put_auxw(float4* __restrict__ X, uint32_t i, flo
 
 // P4[i] = {x_i, y_i, z_i, m_i}, the records the density pass walks, and the
 // interleaved records X of the later loops with u = v_i (see `aux4`).
-__global__ __launch_bounds__(kBlock) void k_pack(FluidDev f, float4* __restrict__ P4, float4* __restrict__ X)
+template <bool BT>
+__global__ __launch_bounds__(kBlock) void k_pack(SceneArg sa, FluidDev f, float4* __restrict__ P4,
+    float4* __restrict__ X)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= f.n)
+    REYN_SCENE(BT, sa);
+    if (!live)
         return;
+    const uint32_t i = t;
     const float4 r = make_float4(f.x[i], f.y[i], f.z[i], f.m[i]);
     P4[i] = r;
     X[2 * size_t(i)] = r;
@@ -167,14 +166,16 @@ __global__ __launch_bounds__(kBlock) void k_pack(FluidDev f, float4* __restrict_
 
 // Densities; with GB also the boundary sums, and with a list in L the
 // neighbour list of the step (every later fluid loop reads it).
-template <int K, bool IND, bool GB>
-__global__ __launch_bounds__(kBlock) REYN_OCC(REYN_OCC_DENSITY) void k_density(FluidDev f, GridDev g, BdyDev b, KParams kp,
+template <int K, bool IND, bool GB, bool BT>
+__global__ __launch_bounds__(kBlock) REYN_OCC(REYN_OCC_DENSITY) void k_density(SceneArg sa, FluidDev f,
     float* __restrict__ rho, float* __restrict__ gbx, float* __restrict__ gby,
     float* __restrict__ gbz, float* __restrict__ rb, NbrDev L, const float4* __restrict__ P4,
     float4* __restrict__ X)
 {
-    const uint32_t t = global_index();
-    const bool live = t < f.n;
+    REYN_SCENE(BT, sa);
+    const GridDev& g = sc.g;
+    const BdyDev& b = sc.b;
+    const KParams& kp = sc.c.kp;
     const bool listing = L.cnt && P4;
     if (!live && !listing)
         return;
@@ -276,14 +277,16 @@ __global__ __launch_bounds__(kBlock) REYN_OCC(REYN_OCC_DENSITY) void k_density(F
 
 // PK: the interleaved records X hold {x, m} and {v, rho} of every particle
 // (u = v, w = rho); otherwise the loop walks the SoA arrays.
-template <int K, bool IND, bool PK>
-__global__ __launch_bounds__(kBlock) REYN_OCC(REYN_OCC_LOOP) void k_nonpressure(FluidDev f, GridDev g, SphK c,
+template <int K, bool IND, bool PK, bool BT>
+__global__ __launch_bounds__(kBlock) REYN_OCC(REYN_OCC_LOOP) void k_nonpressure(SceneArg sa, FluidDev f,
     const float* __restrict__ rho, float* __restrict__ ax, float* __restrict__ ay,
     float* __restrict__ az, NbrDev L, const float4* __restrict__ X)
 {
-    const uint32_t t = global_index();
-    if (t >= f.n)
+    REYN_SCENE(BT, sa);
+    if (!live)
         return;
+    const GridDev& g = sc.g;
+    const SphK& c = sc.c;
     const uint32_t i = particle_of<IND>(g, t);
     const f3 xi = ld3(f.x, f.y, f.z, i);
     const f3 vi = ld3(f.vx, f.vy, f.vz, i);
@@ -303,15 +306,16 @@ __global__ __launch_bounds__(kBlock) REYN_OCC(REYN_OCC_LOOP) void k_nonpressure(
 
 // ---- integration --------------------------------------------------------------
 
-__global__ __launch_bounds__(kBlock) void k_integrate(FluidDev f, const float* __restrict__ ax,
-    const float* __restrict__ ay, const float* __restrict__ az, float dt, const SolveCtl* ctl,
-    int positions, float4* __restrict__ X)
+template <bool BT>
+__global__ __launch_bounds__(kBlock) void k_integrate(SceneArg sa, FluidDev f, const float* __restrict__ ax,
+    const float* __restrict__ ay, const float* __restrict__ az, const SolveCtl* ctl, int positions,
+    float4* __restrict__ X)
 {
-    if (loop_done(ctl))
+    REYN_SCENE(BT, sa);
+    if (loop_done(ctl, sc) || !live)
         return;
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= f.n)
-        return;
+    const uint32_t i = t;
+    const float dt = sc.dt;
     const f3 v = ld3(f.vx, f.vy, f.vz, i) + dt * ld3(ax, ay, az, i);
     st3(f.vx, f.vy, f.vz, i, v);
     if (X)
@@ -327,15 +331,18 @@ __device__ __forceinline__ float eos_sesph(float rho, float k, float rho0)
     return fmaxf(0.f, k * (rho / rho0 - 1.f));
 }
 
-template <int K, bool IND, bool PK>
-__global__ __launch_bounds__(kBlock) void k_sesph_acc(FluidDev f, GridDev g, int bdy, SphK c,
-    float k, const float* __restrict__ rho, const float* __restrict__ gbx,
+template <int K, bool IND, bool PK, bool BT>
+__global__ __launch_bounds__(kBlock) void k_sesph_acc(SceneArg sa, FluidDev f, float k,
+    const float* __restrict__ rho, const float* __restrict__ gbx,
     const float* __restrict__ gby, const float* __restrict__ gbz, float* __restrict__ ax,
     float* __restrict__ ay, float* __restrict__ az, NbrDev L, const float4* __restrict__ X)
 {
-    const uint32_t t = global_index();
-    if (t >= f.n)
+    REYN_SCENE(BT, sa);
+    if (!live)
         return;
+    const GridDev& g = sc.g;
+    const SphK& c = sc.c;
+    const int bdy = sc.b.present;
     const uint32_t i = particle_of<IND>(g, t);
     const f3 xi = ld3(f.x, f.y, f.z, i);
     const f3 vi = ld3(f.vx, f.vy, f.vz, i);
@@ -364,18 +371,22 @@ __global__ __launch_bounds__(kBlock) void k_sesph_acc(FluidDev f, GridDev g, int
 
 // predicted density; per-block FP64 partial sums feed the mean-density test
 // PK: u of X holds the updated velocities and w receives the prediction.
-template <int K, bool IND, bool PK>
-__global__ __launch_bounds__(kBlock) void k_predict_rho(FluidDev f, GridDev g, int bdy, KParams kp,
-    float dt, const float* __restrict__ gbx, const float* __restrict__ gby,
+template <int K, bool IND, bool PK, bool BT>
+__global__ __launch_bounds__(kBlock) void k_predict_rho(SceneArg sa, FluidDev f,
+    const float* __restrict__ gbx, const float* __restrict__ gby,
     const float* __restrict__ gbz, const float* __restrict__ rb, float* __restrict__ rho,
     double* __restrict__ part, const SolveCtl* ctl, NbrDev L, float4* __restrict__ X)
 {
     __shared__ double lds[kBlock / kWave];
-    if (loop_done(ctl))
+    REYN_SCENE(BT, sa);
+    if (loop_done(ctl, sc))
         return;
-    const uint32_t t = global_index();
+    const GridDev& g = sc.g;
+    const KParams& kp = sc.c.kp;
+    const float dt = sc.dt;
+    const int bdy = sc.b.present;
     double mine = 0.0;
-    if (t < f.n) {
+    if (live) {
         const uint32_t i = particle_of<IND>(g, t);
         const f3 xi = ld3(f.x, f.y, f.z, i);
         const f3 vi = ld3(f.vx, f.vy, f.vz, i);
@@ -401,22 +412,23 @@ __global__ __launch_bounds__(kBlock) void k_predict_rho(FluidDev f, GridDev g, i
     }
     const double s = block_reduce<0>(mine, lds);
     if (part && threadIdx.x == 0)
-        part[xcd_block(blockIdx.x, gridDim.x)] = s;
+        part[lb_] = s;
 }
 
 // PK: w of X holds the predicted densities.
-template <int K, bool IND, bool PK>
-__global__ __launch_bounds__(kBlock) void k_pcisph_prs_acc(FluidDev f, GridDev g, int bdy,
-    KParams kp, float k, float rho0, float rho0_sq_inv, const float* __restrict__ rho,
-    const float* __restrict__ gbx, const float* __restrict__ gby, const float* __restrict__ gbz,
-    float* __restrict__ ax, float* __restrict__ ay, float* __restrict__ az, const SolveCtl* ctl, NbrDev L,
-    const float4* __restrict__ X)
+template <int K, bool IND, bool PK, bool BT>
+__global__ __launch_bounds__(kBlock) void k_pcisph_prs_acc(SceneArg sa, FluidDev f,
+    const float* __restrict__ rho, const float* __restrict__ gbx, const float* __restrict__ gby,
+    const float* __restrict__ gbz, float* __restrict__ ax, float* __restrict__ ay, float* __restrict__ az,
+    const SolveCtl* ctl, NbrDev L, const float4* __restrict__ X)
 {
-    if (loop_done(ctl))
+    REYN_SCENE(BT, sa);
+    if (loop_done(ctl, sc) || !live)
         return;
-    const uint32_t t = global_index();
-    if (t >= f.n)
-        return;
+    const GridDev& g = sc.g;
+    const KParams& kp = sc.c.kp;
+    const float k = sc.kk, rho0 = sc.c.rho0, rho0_sq_inv = sc.c.rho0_sq_inv;
+    const int bdy = sc.b.present;
     const uint32_t i = particle_of<IND>(g, t);
     const f3 xi = ld3(f.x, f.y, f.z, i);
     const float rhoi = rho[i];
@@ -439,20 +451,25 @@ __global__ __launch_bounds__(kBlock) void k_pcisph_prs_acc(FluidDev f, GridDev g
     st3(ax, ay, az, i, a);
 }
 
-__global__ __launch_bounds__(kBlock) void k_ctl_pcisph(SolveCtl* ctl, const double* __restrict__ part,
-    uint32_t nb, uint32_t n, float thr, uint32_t min_iter, uint32_t max_iter)
+// One workgroup per scene: reduces the scene's per-block partials in a fixed
+// order and advances its loop control.
+template <bool BT>
+__global__ __launch_bounds__(kBlock) void k_ctl_pcisph(SceneArg sa, SolveCtl* ctls, const double* __restrict__ part,
+    uint32_t min_iter, uint32_t max_iter)
 {
     __shared__ double lds[kBlock / kWave];
+    const SceneDev& sc = scene_of_ctl<BT>(sa);
+    SolveCtl* ctl = ctls + sc.idx;
     if (ctl->done)
         return;
     double s = 0.0;
-    for (uint32_t k = threadIdx.x; k < nb; k += blockDim.x)
-        s += part[k];
+    for (uint32_t k = threadIdx.x; k < sc.nblk; k += blockDim.x)
+        s += part[sc.blk_first + k];
     s = block_reduce<0>(s, lds);
     if (threadIdx.x == 0) {
         const uint32_t l = ctl->iter + 1;
-        const double avg = s / double(n);
-        const bool keep = (l < min_iter) || (avg > double(thr));
+        const double avg = s / double(sc.n);
+        const bool keep = (l < min_iter) || (avg > double(sc.thr));
         ctl->iter = l;
         ctl->avg_density = avg;
         ctl->done = (!keep || (max_iter && l >= max_iter)) ? 1u : 0u;
@@ -470,16 +487,20 @@ __device__ __forceinline__ float jacobi_update(float omega, float aii, float Ap,
 }
 
 // PK: u of X holds the updated velocities.
-template <int K, bool IND, bool PK>
-__global__ __launch_bounds__(kBlock) REYN_OCC(REYN_OCC_LOOP) void k_iisph_aii_s(FluidDev f, GridDev g, int bdy, KParams kp,
-    float rho0, float dt, float omega, int warm, const float* __restrict__ rho,
+template <int K, bool IND, bool PK, bool BT>
+__global__ __launch_bounds__(kBlock) REYN_OCC(REYN_OCC_LOOP) void k_iisph_aii_s(SceneArg sa, FluidDev f,
+    float omega, int warm, const float* __restrict__ rho,
     const float* __restrict__ gbx, const float* __restrict__ gby, const float* __restrict__ gbz,
     float* __restrict__ aii, float* __restrict__ si, float* __restrict__ p, NbrDev L,
     float4* __restrict__ X)
 {
-    const uint32_t t = global_index();
-    if (t >= f.n)
+    REYN_SCENE(BT, sa);
+    if (!live)
         return;
+    const GridDev& g = sc.g;
+    const KParams& kp = sc.c.kp;
+    const float rho0 = sc.c.rho0, dt = sc.dt;
+    const int bdy = sc.b.present;
     const uint32_t i = particle_of<IND>(g, t);
     const f3 xi = ld3(f.x, f.y, f.z, i);
     const f3 vi = ld3(f.vx, f.vy, f.vz, i);
@@ -513,18 +534,20 @@ __global__ __launch_bounds__(kBlock) REYN_OCC(REYN_OCC_LOOP) void k_iisph_aii_s(
 // cache lines than the interleaved records) and, inside the solver loop
 // (ctl != NULL), writes the result to u of X for the Jacobi pass; the final
 // pass and the SoA mode write the work arrays.
-template <int K, bool IND, bool PK>
-__global__ __launch_bounds__(kBlock) REYN_OCC(REYN_OCC_PACC) void k_iisph_pacc(FluidDev f, GridDev g, int bdy, KParams kp,
-    float rho0_sq_inv, const float* __restrict__ p, const float* __restrict__ gbx,
+template <int K, bool IND, bool PK, bool BT>
+__global__ __launch_bounds__(kBlock) REYN_OCC(REYN_OCC_PACC) void k_iisph_pacc(SceneArg sa, FluidDev f,
+    const float* __restrict__ p, const float* __restrict__ gbx,
     const float* __restrict__ gby, const float* __restrict__ gbz, float* __restrict__ ax,
     float* __restrict__ ay, float* __restrict__ az, const SolveCtl* ctl, NbrDev L,
     const float4* __restrict__ P4, float4* __restrict__ X)
 {
-    if (loop_done(ctl))
+    REYN_SCENE(BT, sa);
+    if (loop_done(ctl, sc) || !live)
         return;
-    const uint32_t t = global_index();
-    if (t >= f.n)
-        return;
+    const GridDev& g = sc.g;
+    const KParams& kp = sc.c.kp;
+    const float rho0_sq_inv = sc.c.rho0_sq_inv;
+    const int bdy = sc.b.present;
     const uint32_t i = particle_of<IND>(g, t);
     const f3 xi = ld3(f.x, f.y, f.z, i);
     const float pi = p[i];
@@ -542,21 +565,25 @@ __global__ __launch_bounds__(kBlock) REYN_OCC(REYN_OCC_PACC) void k_iisph_pacc(F
 }
 
 // PK: the accelerations come from u of X; otherwise from the SoA work arrays.
-template <int K, bool IND, bool PK>
-__global__ __launch_bounds__(kBlock) REYN_OCC(REYN_OCC_LOOP) void k_iisph_jacobi(FluidDev f, GridDev g, int bdy, KParams kp,
-    float dt, float omega, float thr, const float* __restrict__ ax, const float* __restrict__ ay,
+template <int K, bool IND, bool PK, bool BT>
+__global__ __launch_bounds__(kBlock) REYN_OCC(REYN_OCC_LOOP) void k_iisph_jacobi(SceneArg sa, FluidDev f,
+    float omega, const float* __restrict__ ax, const float* __restrict__ ay,
     const float* __restrict__ az, const float* __restrict__ aii,
     const float* __restrict__ si, const float* __restrict__ gbx, const float* __restrict__ gby,
     const float* __restrict__ gbz, float* __restrict__ p, double* __restrict__ part_err,
     uint32_t* __restrict__ part_cnt, const SolveCtl* ctl, NbrDev L, float4* __restrict__ X)
 {
     __shared__ double lds[kBlock / kWave];
-    if (loop_done(ctl))
+    REYN_SCENE(BT, sa);
+    if (loop_done(ctl, sc))
         return;
-    const uint32_t t = global_index();
+    const GridDev& g = sc.g;
+    const KParams& kp = sc.c.kp;
+    const float dt = sc.dt, thr = sc.thr;
+    const int bdy = sc.b.present;
     double err_pos = 0.0;
     uint32_t viol = 0;
-    if (t < f.n) {
+    if (live) {
         const uint32_t i = particle_of<IND>(g, t);
         const f3 xi = ld3(f.x, f.y, f.z, i);
         f3 ai;
@@ -592,23 +619,26 @@ __global__ __launch_bounds__(kBlock) REYN_OCC(REYN_OCC_LOOP) void k_iisph_jacobi
         uint32_t c = 0;
         for (int w = 0; w < kBlock / kWave; ++w)
             c += vc[w];
-        const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
-        part_err[b] = es;
-        part_cnt[b] = c;
+        part_err[lb_] = es;
+        part_cnt[lb_] = c;
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_ctl_iisph(SolveCtl* ctl, const double* __restrict__ part_err,
-    const uint32_t* __restrict__ part_cnt, uint32_t nb, uint32_t n, float rho0, float eta,
-    uint32_t min_iter, uint32_t max_iter, int rule)
+template <bool BT>
+__global__ __launch_bounds__(kBlock) void k_ctl_iisph(SceneArg sa, SolveCtl* ctls, const double* __restrict__ part_err,
+    const uint32_t* __restrict__ part_cnt, float eta, uint32_t min_iter, uint32_t max_iter, int rule)
 {
     __shared__ double lds[kBlock / kWave];
+    const SceneDev& sc = scene_of_ctl<BT>(sa);
+    SolveCtl* ctl = ctls + sc.idx;
     if (ctl->done)
         return;
+    const uint32_t n = sc.n;
+    const float rho0 = sc.c.rho0;
     double s = 0.0, c = 0.0;
-    for (uint32_t k = threadIdx.x; k < nb; k += blockDim.x) {
-        s += part_err[k];
-        c += double(part_cnt[k]);
+    for (uint32_t k = threadIdx.x; k < sc.nblk; k += blockDim.x) {
+        s += part_err[sc.blk_first + k];
+        c += double(part_cnt[sc.blk_first + k]);
     }
     s = block_reduce<0>(s, lds);
     c = block_reduce<0>(c, lds);
@@ -624,27 +654,43 @@ __global__ __launch_bounds__(kBlock) void k_ctl_iisph(SolveCtl* ctl, const doubl
     }
 }
 
-__global__ void k_ctl_reset(SolveCtl* ctl)
+__global__ void k_ctl_reset(SolveCtl* ctl, uint32_t count)
 {
-    if (threadIdx.x == 0) {
-        ctl->iter = 0;
-        ctl->done = 0;
-        ctl->n_violating = 0;
-        ctl->pad = 0;
-        ctl->mean_rel_error = 0.0;
-        ctl->avg_density = 0.0;
+    for (uint32_t k = threadIdx.x; k < count; k += blockDim.x) {
+        ctl[k].iter = 0;
+        ctl[k].done = 0;
+        ctl[k].n_violating = 0;
+        ctl[k].pad = 0;
+        ctl[k].mean_rel_error = 0.0;
+        ctl[k].avg_density = 0.0;
     }
 }
 
 // ---- host-side helpers ---------------------------------------------------------------
 
+// One launch target: the fluid arrays, the scene argument (one scene by value,
+// or a batch table), whether the grid indexes an unsorted state, and the
+// number of workgroups.
 struct Common {
     FluidDev f;
-    GridDev g;
-    BdyDev b;
+    SceneArg a;
     bool ind;
     uint32_t nb;
 };
+
+void one_scene(Common* c, const FluidDev& f, uint32_t nb)
+{
+    c->f = f;
+    c->nb = nb;
+    c->ind = false;
+    c->a = SceneArg {};
+    SceneDev& s = c->a.one;
+    s.first = 0;
+    s.n = f.n;
+    s.blk_first = 0;
+    s.nblk = nb;
+    s.idx = 0;
+}
 
 int prepare(reyn_ctx* ctx, const reyn_sph_params* prm, const reyn_particles* fluid,
     const reyn_grid* grid, const reyn_boundary* bdy, Common* c)
@@ -658,11 +704,11 @@ int prepare(reyn_ctx* ctx, const reyn_sph_params* prm, const reyn_particles* flu
     if (fluid->n >= (1u << 28))
         return set_error(ctx, REYN_ERR_CAPACITY, "at most 2^28 - 1 fluid particles per call (got %u)", fluid->n);
     REYN_HIP(ctx, hipSetDevice(ctx->device));
-    c->f = to_dev(*fluid);
-    c->g = to_dev(*grid);
-    c->b = to_dev(bdy);
+    one_scene(c, to_dev(*fluid), blocks_for(fluid->n));
     c->ind = grid->sorted != nullptr;
-    c->nb = blocks_for(fluid->n);
+    c->a.one.g = to_dev(*grid);
+    c->a.one.b = to_dev(bdy);
+    c->a.one.c = make_sphk(*prm);
     return REYN_OK;
 }
 
@@ -672,70 +718,86 @@ template <class F> int with_pk(const float4* X, F&& f)
     return X ? f(std::true_type {}) : f(std::false_type {});
 }
 
-template <bool GB>
-int launch_density(reyn_ctx* ctx, int kernel, const Common& c, const KParams& kp, float* rho,
-    const Scratch* s)
+// A batch always reorders its state and packs records: only those variants of
+// the neighbour kernels are built for it.
+template <bool BT, bool IND, bool PK> constexpr bool kVariant = !BT || (!IND && PK);
+
+int no_variant(reyn_ctx* ctx) { return set_error(ctx, REYN_ERR_INVALID_ARGUMENT, "unsupported batch layout"); }
+
+template <bool BT, bool GB>
+int launch_density(reyn_ctx* ctx, int kernel, const Common& c, float* rho, const Scratch* s)
 {
     if (s && s->P4) {
         {
             ProfScope _prof(ctx, "k_pack");
-            k_pack<<<c.nb, kBlock, 0, ctx->stream>>>(c.f, s->P4, s->X);
+            k_pack<BT><<<c.nb, kBlock, 0, ctx->stream>>>(c.a, c.f, s->P4, s->X);
         }
         REYN_LAUNCHED(ctx, "k_pack");
     }
     return with_kernel(ctx, kernel, c.ind, [&](auto KC, auto IC) -> int {
         constexpr int K = decltype(KC)::value;
         constexpr bool I = decltype(IC)::value;
-        {
-            ProfScope _prof(ctx, "k_density");
-            k_density<K, I, GB><<<c.nb, kBlock, 0, ctx->stream>>>(c.f, c.g, c.b, kp, rho,
-                GB ? s->gbx : nullptr, GB ? s->gby : nullptr, GB ? s->gbz : nullptr, GB ? s->rb : nullptr,
-                s ? s->L : NbrDev {}, s ? s->P4 : nullptr, s ? s->X : nullptr);
+        if constexpr (!kVariant<BT, I, true>) {
+            return no_variant(ctx);
+        } else {
+            {
+                ProfScope _prof(ctx, "k_density");
+                k_density<K, I, GB, BT><<<c.nb, kBlock, 0, ctx->stream>>>(c.a, c.f, rho,
+                    GB ? s->gbx : nullptr, GB ? s->gby : nullptr, GB ? s->gbz : nullptr, GB ? s->rb : nullptr,
+                    s ? s->L : NbrDev {}, s ? s->P4 : nullptr, s ? s->X : nullptr);
+            }
+            REYN_LAUNCHED(ctx, "k_density");
+            return REYN_OK;
         }
-        REYN_LAUNCHED(ctx, "k_density");
-        return REYN_OK;
     });
 }
 
-int launch_nonpressure(reyn_ctx* ctx, int kernel, const Common& c, const SphK& k, const float* rho,
-    float* ax, float* ay, float* az, const NbrDev& L, const float4* X)
+template <bool BT>
+int launch_nonpressure(reyn_ctx* ctx, int kernel, const Common& c, const float* rho, float* ax, float* ay,
+    float* az, const NbrDev& L, const float4* X)
 {
     return with_kernel(ctx, kernel, c.ind, [&](auto KC, auto IC) -> int {
         return with_pk(X, [&](auto PC) -> int {
             constexpr int K = decltype(KC)::value;
             constexpr bool I = decltype(IC)::value;
             constexpr bool PK = decltype(PC)::value;
-            {
-                ProfScope _prof(ctx, "k_nonpressure");
-                k_nonpressure<K, I, PK><<<c.nb, kBlock, 0, ctx->stream>>>(c.f, c.g, k, rho, ax, ay, az, L, X);
+            if constexpr (!kVariant<BT, I, PK>) {
+                return no_variant(ctx);
+            } else {
+                {
+                    ProfScope _prof(ctx, "k_nonpressure");
+                    k_nonpressure<K, I, PK, BT><<<c.nb, kBlock, 0, ctx->stream>>>(c.a, c.f, rho, ax, ay, az, L, X);
+                }
+                REYN_LAUNCHED(ctx, "k_nonpressure");
+                return REYN_OK;
             }
-            REYN_LAUNCHED(ctx, "k_nonpressure");
-            return REYN_OK;
         });
     });
 }
 
-int launch_integrate(reyn_ctx* ctx, const Common& c, const float* ax, const float* ay,
-    const float* az, float dt, const SolveCtl* ctl, int positions, float4* X = nullptr)
+template <bool BT>
+int launch_integrate(reyn_ctx* ctx, const Common& c, const float* ax, const float* ay, const float* az,
+    const SolveCtl* ctl, int positions, float4* X = nullptr)
 {
     {
         ProfScope _prof(ctx, positions ? "k_integrate" : "k_integrate_v");
-        k_integrate<<<c.nb, kBlock, 0, ctx->stream>>>(c.f, ax, ay, az, dt, ctl, positions, X);
+        k_integrate<BT><<<c.nb, kBlock, 0, ctx->stream>>>(c.a, c.f, ax, ay, az, ctl, positions, X);
     }
     REYN_LAUNCHED(ctx, "k_integrate");
     return REYN_OK;
 }
 
-// Enqueue loop iterations in chunks until the control block reports done.
+// Enqueue loop iterations in chunks until every scene's control block
+// reports done; `nscenes` control blocks at `ctl`, copied into `out`.
 template <class Iter>
-int run_loop(reyn_ctx* ctx, SolveCtl* ctl, uint32_t min_iter, uint32_t max_iter, uint32_t check_every,
-    Iter&& iter, SolveCtl* out)
+int run_loop(reyn_ctx* ctx, SolveCtl* ctl, uint32_t nscenes, uint32_t min_iter, uint32_t max_iter,
+    uint32_t check_every, Iter&& iter, SolveCtl* out)
 {
     SolveCtl* host = nullptr;
-    REYN_TRY(ctx_pinned(ctx, sizeof(SolveCtl), reinterpret_cast<void**>(&host)));
+    REYN_TRY(ctx_pinned(ctx, sizeof(SolveCtl) * nscenes, reinterpret_cast<void**>(&host)));
     {
         ProfScope _prof(ctx, "k_ctl_reset");
-        k_ctl_reset<<<1, kWave, 0, ctx->stream>>>(ctl);
+        k_ctl_reset<<<1, kBlock, 0, ctx->stream>>>(ctl, nscenes);
     }
     REYN_LAUNCHED(ctx, "k_ctl_reset");
     const uint32_t chunk = check_every ? check_every : 4;
@@ -750,15 +812,180 @@ int run_loop(reyn_ctx* ctx, SolveCtl* ctl, uint32_t min_iter, uint32_t max_iter,
         for (uint32_t k = 0; k < m; ++k)
             REYN_TRY(iter());
         launched += m;
-        REYN_HIP(ctx, hipMemcpyAsync(host, ctl, sizeof(SolveCtl), hipMemcpyDeviceToHost, ctx->stream));
+        REYN_HIP(ctx, hipMemcpyAsync(host, ctl, sizeof(SolveCtl) * nscenes, hipMemcpyDeviceToHost, ctx->stream));
         REYN_HIP(ctx, hipStreamSynchronize(ctx->stream));
-        if (host->done)
+        bool all = true;
+        for (uint32_t k = 0; k < nscenes; ++k)
+            all = all && host[k].done;
+        if (all)
             break;
         if (launched >= hard_cap)
             return set_error(ctx, REYN_ERR_NOT_CONVERGED, "solver did not converge in %u iterations", launched);
     }
-    *out = *host;
+    for (uint32_t k = 0; k < nscenes; ++k)
+        out[k] = host[k];
     return REYN_OK;
+}
+
+
+// The solver steps on a launch target: one scene (BT = false) or a batch
+// (BT = true, every scene advancing with its own time step and loop control).
+
+template <bool BT>
+int sesph_run(reyn_ctx* ctx, int kernel, const Common& c, float k, const reyn_sesph_work* work, const Scratch& s)
+{
+    REYN_TRY((launch_density<BT, true>(ctx, kernel, c, work->rho, &s)));
+    REYN_TRY(with_kernel(ctx, kernel, c.ind, [&](auto KC, auto IC) -> int {
+        return with_pk(s.X, [&](auto PC) -> int {
+            constexpr int K = decltype(KC)::value;
+            constexpr bool I = decltype(IC)::value;
+            constexpr bool PK = decltype(PC)::value;
+            if constexpr (!kVariant<BT, I, PK>) {
+                return no_variant(ctx);
+            } else {
+                {
+                    ProfScope _prof(ctx, "k_sesph_acc");
+                    k_sesph_acc<K, I, PK, BT><<<c.nb, kBlock, 0, ctx->stream>>>(c.a, c.f, k, work->rho, s.gbx,
+                        s.gby, s.gbz, work->ax, work->ay, work->az, s.L, s.X);
+                }
+                REYN_LAUNCHED(ctx, "k_sesph_acc");
+                return REYN_OK;
+            }
+        });
+    }));
+    return launch_integrate<BT>(ctx, c, work->ax, work->ay, work->az, nullptr, 1);
+}
+
+template <bool BT>
+int pcisph_run(reyn_ctx* ctx, int kernel, const Common& c, const reyn_pcisph_work* work, const Scratch& s,
+    uint32_t nscenes, uint32_t min_iter, uint32_t max_iter, SolveCtl* res)
+{
+    REYN_TRY((launch_density<BT, true>(ctx, kernel, c, work->rho, &s)));
+    REYN_TRY(launch_nonpressure<BT>(ctx, kernel, c, work->rho, work->ax, work->ay, work->az, s.L, s.X));
+    REYN_TRY(run_loop(
+        ctx, s.ctl, nscenes, min_iter, max_iter, 4,
+        [&]() -> int {
+            REYN_TRY(launch_integrate<BT>(ctx, c, work->ax, work->ay, work->az, s.ctl, 0, s.X));
+            REYN_TRY(with_kernel(ctx, kernel, c.ind, [&](auto KC, auto IC) -> int {
+                return with_pk(s.X, [&](auto PC) -> int {
+                    constexpr int K = decltype(KC)::value;
+                    constexpr bool I = decltype(IC)::value;
+                    constexpr bool PK = decltype(PC)::value;
+                    if constexpr (!kVariant<BT, I, PK>) {
+                        return no_variant(ctx);
+                    } else {
+                        {
+                            ProfScope _prof(ctx, "k_predict_rho");
+                            k_predict_rho<K, I, PK, BT><<<c.nb, kBlock, 0, ctx->stream>>>(c.a, c.f, s.gbx, s.gby,
+                                s.gbz, s.rb, work->rho, s.part_d, s.ctl, s.L, s.X);
+                        }
+                        REYN_LAUNCHED(ctx, "k_predict_rho");
+                        {
+                            ProfScope _prof(ctx, "k_pcisph_prs_acc");
+                            k_pcisph_prs_acc<K, I, PK, BT><<<c.nb, kBlock, 0, ctx->stream>>>(c.a, c.f, work->rho,
+                                s.gbx, s.gby, s.gbz, work->ax, work->ay, work->az, s.ctl, s.L, s.X);
+                        }
+                        REYN_LAUNCHED(ctx, "k_pcisph_prs_acc");
+                        return REYN_OK;
+                    }
+                });
+            }));
+            {
+                ProfScope _prof(ctx, "k_ctl_pcisph");
+                k_ctl_pcisph<BT><<<nscenes, kBlock, 0, ctx->stream>>>(c.a, s.ctl, s.part_d, min_iter, max_iter);
+            }
+            REYN_LAUNCHED(ctx, "k_ctl_pcisph");
+            return REYN_OK;
+        },
+        res));
+    return launch_integrate<BT>(ctx, c, work->ax, work->ay, work->az, nullptr, 1);
+}
+
+struct IisphLoop {
+    float omega, eta;
+    uint32_t min_iter, max_iter, check_every;
+    int rule, warm;
+};
+
+template <bool BT>
+int iisph_run(reyn_ctx* ctx, int kind, const Common& c, const IisphLoop& lp, const reyn_iisph_work* work,
+    const Scratch& s, uint32_t nscenes, SolveCtl* res)
+{
+    REYN_TRY((launch_density<BT, true>(ctx, kind, c, work->rho, &s)));
+    REYN_TRY(launch_nonpressure<BT>(ctx, kind, c, work->rho, work->ax, work->ay, work->az, s.L, s.X));
+    REYN_TRY(launch_integrate<BT>(ctx, c, work->ax, work->ay, work->az, nullptr, 0, s.X));
+    REYN_TRY(with_kernel(ctx, kind, c.ind, [&](auto KC, auto IC) -> int {
+        return with_pk(s.X, [&](auto PC) -> int {
+            constexpr int K = decltype(KC)::value;
+            constexpr bool I = decltype(IC)::value;
+            constexpr bool PK = decltype(PC)::value;
+            if constexpr (!kVariant<BT, I, PK>) {
+                return no_variant(ctx);
+            } else {
+                {
+                    ProfScope _prof(ctx, "k_iisph_aii_s");
+                    k_iisph_aii_s<K, I, PK, BT><<<c.nb, kBlock, 0, ctx->stream>>>(c.a, c.f, lp.omega, lp.warm,
+                        work->rho, s.gbx, s.gby, s.gbz, work->a_ii, work->s_i, work->p, s.L, s.X);
+                }
+                REYN_LAUNCHED(ctx, "k_iisph_aii_s");
+                return REYN_OK;
+            }
+        });
+    }));
+    auto pacc = [&](const SolveCtl* ctl) -> int {
+        return with_kernel(ctx, kind, c.ind, [&](auto KC, auto IC) -> int {
+            return with_pk(s.X, [&](auto PC) -> int {
+                constexpr int K = decltype(KC)::value;
+                constexpr bool I = decltype(IC)::value;
+                constexpr bool PK = decltype(PC)::value;
+                if constexpr (!kVariant<BT, I, PK>) {
+                    return no_variant(ctx);
+                } else {
+                    {
+                        ProfScope _prof(ctx, "k_iisph_pacc");
+                        k_iisph_pacc<K, I, PK, BT><<<c.nb, kBlock, 0, ctx->stream>>>(c.a, c.f, work->p, s.gbx,
+                            s.gby, s.gbz, work->ax, work->ay, work->az, ctl, s.L, s.P4, s.X);
+                    }
+                    REYN_LAUNCHED(ctx, "k_iisph_pacc");
+                    return REYN_OK;
+                }
+            });
+        });
+    };
+    REYN_TRY(run_loop(
+        ctx, s.ctl, nscenes, lp.min_iter, lp.max_iter, lp.check_every,
+        [&]() -> int {
+            REYN_TRY(pacc(s.ctl));
+            REYN_TRY(with_kernel(ctx, kind, c.ind, [&](auto KC, auto IC) -> int {
+                return with_pk(s.X, [&](auto PC) -> int {
+                    constexpr int K = decltype(KC)::value;
+                    constexpr bool I = decltype(IC)::value;
+                    constexpr bool PK = decltype(PC)::value;
+                    if constexpr (!kVariant<BT, I, PK>) {
+                        return no_variant(ctx);
+                    } else {
+                        {
+                            ProfScope _prof(ctx, "k_iisph_jacobi");
+                            k_iisph_jacobi<K, I, PK, BT><<<c.nb, kBlock, 0, ctx->stream>>>(c.a, c.f, lp.omega,
+                                work->ax, work->ay, work->az, work->a_ii, work->s_i, s.gbx, s.gby, s.gbz, work->p,
+                                s.part_d, s.part_u, s.ctl, s.L, s.X);
+                        }
+                        REYN_LAUNCHED(ctx, "k_iisph_jacobi");
+                        return REYN_OK;
+                    }
+                });
+            }));
+            {
+                ProfScope _prof(ctx, "k_ctl_iisph");
+                k_ctl_iisph<BT><<<nscenes, kBlock, 0, ctx->stream>>>(c.a, s.ctl, s.part_d, s.part_u, lp.eta,
+                    lp.min_iter, lp.max_iter, lp.rule);
+            }
+            REYN_LAUNCHED(ctx, "k_ctl_iisph");
+            return REYN_OK;
+        },
+        res));
+    REYN_TRY(pacc(nullptr));
+    return launch_integrate<BT>(ctx, c, work->ax, work->ay, work->az, nullptr, 1);
 }
 
 } // namespace
@@ -777,7 +1004,7 @@ int reyn_density(reyn_ctx* ctx, const reyn_sph_params* prm, const reyn_particles
         return set_error(ctx, REYN_ERR_INVALID_ARGUMENT, "density: rho is NULL");
     if (fluid->n == 0)
         return REYN_OK;
-    return launch_density<false>(ctx, prm->kernel, c, make_kparams(prm->kernel, prm->h_bar), rho, nullptr);
+    return launch_density<false, false>(ctx, prm->kernel, c, rho, nullptr);
 }
 
 int reyn_nonpressure_acc(reyn_ctx* ctx, const reyn_sph_params* prm, const reyn_particles* fluid,
@@ -789,7 +1016,7 @@ int reyn_nonpressure_acc(reyn_ctx* ctx, const reyn_sph_params* prm, const reyn_p
         return REYN_OK;
     if (!rho || !ax || !ay || !az)
         return set_error(ctx, REYN_ERR_INVALID_ARGUMENT, "nonpressure_acc: NULL buffer");
-    return launch_nonpressure(ctx, prm->kernel, c, make_sphk(*prm), rho, ax, ay, az, NbrDev {}, nullptr);
+    return launch_nonpressure<false>(ctx, prm->kernel, c, rho, ax, ay, az, NbrDev {}, nullptr);
 }
 
 int reyn_predict_rho(reyn_ctx* ctx, const reyn_sph_params* prm, const reyn_particles* fluid,
@@ -801,20 +1028,20 @@ int reyn_predict_rho(reyn_ctx* ctx, const reyn_sph_params* prm, const reyn_parti
         return REYN_OK;
     if (!rho)
         return set_error(ctx, REYN_ERR_INVALID_ARGUMENT, "predict_rho: rho is NULL");
+    c.a.one.dt = dt;
     Scratch s;
     REYN_TRY(get_scratch(ctx, fluid->n, &s));
-    const KParams kp = make_kparams(prm->kernel, prm->h_bar);
     // the boundary sums come from a density pass into scratch; its density
     // output lands in `rho` and is overwritten by the prediction
-    if (c.b.present)
-        REYN_TRY(launch_density<true>(ctx, prm->kernel, c, kp, rho, &s));
+    if (c.a.one.b.present)
+        REYN_TRY((launch_density<false, true>(ctx, prm->kernel, c, rho, &s)));
     return with_kernel(ctx, prm->kernel, c.ind, [&](auto KC, auto IC) -> int {
         constexpr int K = decltype(KC)::value;
         constexpr bool I = decltype(IC)::value;
         {
             ProfScope _prof(ctx, "k_predict_rho");
-            k_predict_rho<K, I, false><<<c.nb, kBlock, 0, ctx->stream>>>(c.f, c.g, c.b.present, kp, dt,
-                s.gbx, s.gby, s.gbz, s.rb, rho, nullptr, nullptr, NbrDev {}, nullptr);
+            k_predict_rho<K, I, false, false><<<c.nb, kBlock, 0, ctx->stream>>>(c.a, c.f, s.gbx, s.gby, s.gbz,
+                s.rb, rho, nullptr, nullptr, NbrDev {}, nullptr);
         }
         REYN_LAUNCHED(ctx, "k_predict_rho");
         return REYN_OK;
@@ -829,10 +1056,10 @@ int reyn_integrate(reyn_ctx* ctx, const reyn_particles* fluid, const float* ax, 
     REYN_TRY(check_fluid(ctx, fluid));
     if (fluid->n == 0)
         return REYN_OK;
-    Common c {};
-    c.f = to_dev(*fluid);
-    c.nb = blocks_for(fluid->n);
-    return launch_integrate(ctx, c, ax, ay, az, dt, nullptr, 1);
+    Common c;
+    one_scene(&c, to_dev(*fluid), blocks_for(fluid->n));
+    c.a.one.dt = dt;
+    return launch_integrate<false>(ctx, c, ax, ay, az, nullptr, 1);
 }
 
 int reyn_integrate_a_v_only(reyn_ctx* ctx, const reyn_particles* fluid, const float* ax,
@@ -843,10 +1070,10 @@ int reyn_integrate_a_v_only(reyn_ctx* ctx, const reyn_particles* fluid, const fl
     REYN_TRY(check_fluid(ctx, fluid));
     if (fluid->n == 0)
         return REYN_OK;
-    Common c {};
-    c.f = to_dev(*fluid);
-    c.nb = blocks_for(fluid->n);
-    return launch_integrate(ctx, c, ax, ay, az, dt, nullptr, 0);
+    Common c;
+    one_scene(&c, to_dev(*fluid), blocks_for(fluid->n));
+    c.a.one.dt = dt;
+    return launch_integrate<false>(ctx, c, ax, ay, az, nullptr, 0);
 }
 
 int reyn_sesph_step(reyn_ctx* ctx, const reyn_sesph_params* prm, const reyn_particles* fluid,
@@ -860,25 +1087,10 @@ int reyn_sesph_step(reyn_ctx* ctx, const reyn_sesph_params* prm, const reyn_part
         return REYN_OK;
     if (!work->rho || !work->ax || !work->ay || !work->az)
         return set_error(ctx, REYN_ERR_INVALID_ARGUMENT, "sesph: NULL work buffer");
+    c.a.one.dt = dt;
     Scratch s;
     REYN_TRY(get_scratch(ctx, fluid->n, &s, true));
-    const SphK k = make_sphk(prm->sph);
-    REYN_TRY(launch_density<true>(ctx, prm->sph.kernel, c, k.kp, work->rho, &s));
-    REYN_TRY(with_kernel(ctx, prm->sph.kernel, c.ind, [&](auto KC, auto IC) -> int {
-        return with_pk(s.X, [&](auto PC) -> int {
-            constexpr int K = decltype(KC)::value;
-            constexpr bool I = decltype(IC)::value;
-            constexpr bool PK = decltype(PC)::value;
-            {
-                ProfScope _prof(ctx, "k_sesph_acc");
-                k_sesph_acc<K, I, PK><<<c.nb, kBlock, 0, ctx->stream>>>(c.f, c.g, c.b.present, k, prm->k,
-                    work->rho, s.gbx, s.gby, s.gbz, work->ax, work->ay, work->az, s.L, s.X);
-            }
-            REYN_LAUNCHED(ctx, "k_sesph_acc");
-            return REYN_OK;
-        });
-    }));
-    return launch_integrate(ctx, c, work->ax, work->ay, work->az, dt, nullptr, 1);
+    return sesph_run<false>(ctx, prm->sph.kernel, c, prm->k, work, s);
 }
 
 int reyn_pcisph_delta(int32_t kernel, float h_bar, float h, double* delta)
@@ -955,50 +1167,13 @@ int reyn_pcisph_step(reyn_ctx* ctx, const reyn_pcisph_params* prm, const reyn_pa
         return set_error(ctx, REYN_ERR_INVALID_ARGUMENT, "pcisph: NULL work buffer");
     if (!(dt > 0.f))
         return set_error(ctx, REYN_ERR_INVALID_ARGUMENT, "pcisph: dt must be positive");
+    c.a.one.dt = dt;
+    c.a.one.kk = static_cast<float>((prm->delta / dt) / dt);
+    c.a.one.thr = (prm->eta_rho_max + 1.f) * prm->sph.rho_0;
     Scratch s;
     REYN_TRY(get_scratch(ctx, fluid->n, &s, true));
-    const SphK k = make_sphk(prm->sph);
-    const float kk = static_cast<float>((prm->delta / dt) / dt);
-    const float rho0 = prm->sph.rho_0;
-    const float thr = (prm->eta_rho_max + 1.f) * rho0;
-    REYN_TRY(launch_density<true>(ctx, prm->sph.kernel, c, k.kp, work->rho, &s));
-    REYN_TRY(launch_nonpressure(ctx, prm->sph.kernel, c, k, work->rho, work->ax, work->ay, work->az, s.L, s.X));
     SolveCtl res;
-    REYN_TRY(run_loop(
-        ctx, s.ctl, prm->min_iter, prm->max_iter, 4,
-        [&]() -> int {
-            REYN_TRY(launch_integrate(ctx, c, work->ax, work->ay, work->az, dt, s.ctl, 0, s.X));
-            REYN_TRY(with_kernel(ctx, prm->sph.kernel, c.ind, [&](auto KC, auto IC) -> int {
-                return with_pk(s.X, [&](auto PC) -> int {
-                    constexpr int K = decltype(KC)::value;
-                    constexpr bool I = decltype(IC)::value;
-                    constexpr bool PK = decltype(PC)::value;
-                    {
-                        ProfScope _prof(ctx, "k_predict_rho");
-                        k_predict_rho<K, I, PK><<<c.nb, kBlock, 0, ctx->stream>>>(c.f, c.g, c.b.present, k.kp,
-                            dt, s.gbx, s.gby, s.gbz, s.rb, work->rho, s.part_d, s.ctl, s.L, s.X);
-                    }
-                    REYN_LAUNCHED(ctx, "k_predict_rho");
-                    {
-                        ProfScope _prof(ctx, "k_pcisph_prs_acc");
-                        k_pcisph_prs_acc<K, I, PK><<<c.nb, kBlock, 0, ctx->stream>>>(c.f, c.g, c.b.present,
-                            k.kp, kk, rho0, 1.f / (rho0 * rho0), work->rho, s.gbx, s.gby, s.gbz, work->ax,
-                            work->ay, work->az, s.ctl, s.L, s.X);
-                    }
-                    REYN_LAUNCHED(ctx, "k_pcisph_prs_acc");
-                    return REYN_OK;
-                });
-            }));
-            {
-                ProfScope _prof(ctx, "k_ctl_pcisph");
-                k_ctl_pcisph<<<1, kBlock, 0, ctx->stream>>>(s.ctl, s.part_d, c.nb, fluid->n, thr,
-                    prm->min_iter, prm->max_iter);
-            }
-            REYN_LAUNCHED(ctx, "k_ctl_pcisph");
-            return REYN_OK;
-        },
-        &res));
-    REYN_TRY(launch_integrate(ctx, c, work->ax, work->ay, work->az, dt, nullptr, 1));
+    REYN_TRY(pcisph_run<false>(ctx, prm->sph.kernel, c, work, s, 1, prm->min_iter, prm->max_iter, &res));
     if (info) {
         info->iterations = res.iter;
         info->avg_density = res.avg_density;
@@ -1024,79 +1199,14 @@ int reyn_iisph_step(reyn_ctx* ctx, const reyn_iisph_params* prm, const reyn_part
         return set_error(ctx, REYN_ERR_INVALID_ARGUMENT, "iisph: dt must be positive");
     if (prm->stop_rule != REYN_IISPH_STOP_ANY && prm->stop_rule != REYN_IISPH_STOP_MEAN)
         return set_error(ctx, REYN_ERR_INVALID_ARGUMENT, "iisph: unknown stop rule %d", prm->stop_rule);
+    c.a.one.dt = dt;
+    c.a.one.thr = prm->eta_rho_max * prm->sph.rho_0;
     Scratch s;
     REYN_TRY(get_scratch(ctx, fluid->n, &s, true));
-    const SphK k = make_sphk(prm->sph);
-    const float rho0 = prm->sph.rho_0;
-    const float rho0_sq_inv = 1.f / (rho0 * rho0);
-    const float thr = prm->eta_rho_max * rho0;
-    const int kind = prm->sph.kernel;
-    const int bd = c.b.present;
-    REYN_TRY(launch_density<true>(ctx, kind, c, k.kp, work->rho, &s));
-    REYN_TRY(launch_nonpressure(ctx, kind, c, k, work->rho, work->ax, work->ay, work->az, s.L, s.X));
-    REYN_TRY(launch_integrate(ctx, c, work->ax, work->ay, work->az, dt, nullptr, 0, s.X));
-    REYN_TRY(with_kernel(ctx, kind, c.ind, [&](auto KC, auto IC) -> int {
-        return with_pk(s.X, [&](auto PC) -> int {
-            constexpr int K = decltype(KC)::value;
-            constexpr bool I = decltype(IC)::value;
-            constexpr bool PK = decltype(PC)::value;
-            {
-                ProfScope _prof(ctx, "k_iisph_aii_s");
-                k_iisph_aii_s<K, I, PK><<<c.nb, kBlock, 0, ctx->stream>>>(c.f, c.g, bd, k.kp, rho0, dt,
-                    prm->omega, prm->warm_start, work->rho, s.gbx, s.gby, s.gbz, work->a_ii, work->s_i, work->p,
-                    s.L, s.X);
-            }
-            REYN_LAUNCHED(ctx, "k_iisph_aii_s");
-            return REYN_OK;
-        });
-    }));
-    auto pacc = [&](const SolveCtl* ctl) -> int {
-        return with_kernel(ctx, kind, c.ind, [&](auto KC, auto IC) -> int {
-            return with_pk(s.X, [&](auto PC) -> int {
-                constexpr int K = decltype(KC)::value;
-                constexpr bool I = decltype(IC)::value;
-                constexpr bool PK = decltype(PC)::value;
-                {
-                    ProfScope _prof(ctx, "k_iisph_pacc");
-                    k_iisph_pacc<K, I, PK><<<c.nb, kBlock, 0, ctx->stream>>>(c.f, c.g, bd, k.kp, rho0_sq_inv,
-                        work->p, s.gbx, s.gby, s.gbz, work->ax, work->ay, work->az, ctl, s.L, s.P4, s.X);
-                }
-                REYN_LAUNCHED(ctx, "k_iisph_pacc");
-                return REYN_OK;
-            });
-        });
-    };
+    const IisphLoop lp { prm->omega, prm->eta_rho_max, prm->min_iter, prm->max_iter, prm->check_every,
+        prm->stop_rule, prm->warm_start };
     SolveCtl res;
-    REYN_TRY(run_loop(
-        ctx, s.ctl, prm->min_iter, prm->max_iter, prm->check_every,
-        [&]() -> int {
-            REYN_TRY(pacc(s.ctl));
-            REYN_TRY(with_kernel(ctx, kind, c.ind, [&](auto KC, auto IC) -> int {
-                return with_pk(s.X, [&](auto PC) -> int {
-                    constexpr int K = decltype(KC)::value;
-                    constexpr bool I = decltype(IC)::value;
-                    constexpr bool PK = decltype(PC)::value;
-                    {
-                        ProfScope _prof(ctx, "k_iisph_jacobi");
-                        k_iisph_jacobi<K, I, PK><<<c.nb, kBlock, 0, ctx->stream>>>(c.f, c.g, bd, k.kp, dt,
-                            prm->omega, thr, work->ax, work->ay, work->az, work->a_ii, work->s_i, s.gbx, s.gby,
-                            s.gbz, work->p, s.part_d, s.part_u, s.ctl, s.L, s.X);
-                    }
-                    REYN_LAUNCHED(ctx, "k_iisph_jacobi");
-                    return REYN_OK;
-                });
-            }));
-            {
-                ProfScope _prof(ctx, "k_ctl_iisph");
-                k_ctl_iisph<<<1, kBlock, 0, ctx->stream>>>(s.ctl, s.part_d, s.part_u, c.nb, fluid->n, rho0,
-                    prm->eta_rho_max, prm->min_iter, prm->max_iter, prm->stop_rule);
-            }
-            REYN_LAUNCHED(ctx, "k_ctl_iisph");
-            return REYN_OK;
-        },
-        &res));
-    REYN_TRY(pacc(nullptr));
-    REYN_TRY(launch_integrate(ctx, c, work->ax, work->ay, work->az, dt, nullptr, 1));
+    REYN_TRY(iisph_run<false>(ctx, prm->sph.kernel, c, lp, work, s, 1, &res));
     if (info) {
         info->iterations = res.iter;
         info->n_violating = res.n_violating;

@@ -420,6 +420,66 @@ inline BdyDev to_dev(const reyn_boundary* b)
     return d;
 }
 
+// ---- scenes ---------------------------------------------------------------------
+// The constants of one scene: its fluid grid, boundary, kernel, material
+// constants, time step and particle range. A call on one scene passes it by
+// value; a batch passes a device table and the scene of every workgroup.
+// Workgroups are aligned to scene starts, so a workgroup sees one scene, its
+// particles map to threads as in a call on that scene alone, and its partial
+// sums are the ones that call would form.
+
+struct SphK {
+    KParams kp;
+    float h, nu, rho0;
+    f3 g;
+    float visc_c; // 10 nu
+    float visc_eps; // 0.01 h^2, formed in double
+    float rho0_sq_inv; // 1 / rho0^2
+};
+
+struct SceneDev {
+    GridDev g;
+    BdyDev b;
+    SphK c;
+    float dt; // time step of the step in progress
+    float kk; // PCISPH stiffness (delta / dt) / dt
+    float thr; // the solver loop's error threshold for this scene
+    uint32_t first, n; // particle range [first, first + n)
+    uint32_t blk_first, nblk; // its workgroups (after the XCD remap)
+    uint32_t idx; // scene index: its loop-control block
+};
+
+struct SceneArg {
+    SceneDev one;
+    const SceneDev* table; // batch: per scene; unused for one scene
+    const uint32_t* blk_scene; // batch: the scene of every workgroup
+};
+
+template <bool BT> __device__ __forceinline__ const SceneDev& scene_of(const SceneArg& a, uint32_t lb)
+{
+    if constexpr (BT)
+        return a.table[a.blk_scene[lb]];
+    else
+        return a.one;
+}
+
+// The scene of a loop-control workgroup: workgroup s of a batch reduces
+// scene s.
+template <bool BT> __device__ __forceinline__ const SceneDev& scene_of_ctl(const SceneArg& a)
+{
+    if constexpr (BT)
+        return a.table[blockIdx.x];
+    else
+        return a.one;
+}
+
+// The workgroup's scene, the thread's particle slot and whether it holds one.
+#define REYN_SCENE(BT, sa)                                                                                 \
+    const uint32_t lb_ = xcd_block(blockIdx.x, gridDim.x);                                                 \
+    const SceneDev& sc = scene_of<BT>((sa), lb_);                                                          \
+    const uint32_t t = sc.first + (lb_ - sc.blk_first) * kBlock + threadIdx.x;                            \
+    const bool live = t < sc.first + sc.n
+
 // Thread -> particle map: in Resort::no mode threads follow the cell order
 // through `sorted`, so a wavefront's neighbourhoods overlap in cache.
 template <bool IND> __device__ __forceinline__ uint32_t particle_of(const GridDev& g, uint32_t t)
