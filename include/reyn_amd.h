@@ -368,6 +368,68 @@ int reyn_hard_enforce_bounds(reyn_ctx* ctx, reyn_float3 bound_min,
 int reyn_reduce_f32(reyn_ctx* ctx, const float* x, uint32_t n, int32_t op,
     double* out);
 
+/* ---- scene batches ---------------------------------------------------------
+ * Many independent scenes advanced together: every phase of a step is one
+ * launch over the particles of all scenes (workgroups aligned to scene
+ * starts), and each scene keeps its own grid, boundary, spacing, time step and
+ * pressure-loop control. A scene's result is the same alone (a batch of one)
+ * and in any batch. No counterpart in the reference, which runs one scene per
+ * process (src/main.cu:46-78); the step it batches is the reference's
+ * PCISPH or IISPH step with the main loop's grid, time-step controller and
+ * bounds (main.cu:59-78, cfl.cu:39-52, scene.cu:466-491). */
+
+enum { REYN_SOLVER_PCISPH = 1, REYN_SOLVER_IISPH = 2 };
+
+typedef struct reyn_batch reyn_batch;
+
+typedef struct reyn_batch_scene {
+    uint32_t n;                  /* fluid particles */
+    const float *x, *y, *z;      /* host arrays of n: initial positions */
+    const float *vx, *vy, *vz;   /* host arrays of n, or NULL for rest */
+    float mass;                  /* per fluid particle */
+    uint32_t n_bdy;              /* boundary samples */
+    const float *bx, *by, *bz;   /* host arrays of n_bdy */
+    reyn_float3 bound_min;       /* the fluid domain: grid extent and the */
+    reyn_float3 bound_max;       /* hard_enforce_bounds box */
+    float h;                     /* particle spacing; kernel support 2h */
+} reyn_batch_scene;
+
+typedef struct reyn_batch_params {
+    int32_t solver;          /* REYN_SOLVER_* */
+    int32_t kernel;          /* REYN_KERNEL_*, shared by all scenes */
+    float nu, rho_0;
+    reyn_float3 g;
+    float cell_ratio;        /* fluid and boundary grids */
+    /* time step: the reference main loop's controller (main.cu:59-65) */
+    float dt0, cfl_lambda, change_factor;
+    uint32_t iter_target;
+    /* pressure loop: PCISPH uses eta, min_iter, max_iter; IISPH all */
+    float eta_rho_max, omega;
+    uint32_t min_iter, max_iter;
+    int32_t stop_rule, warm_start;
+} reyn_batch_params;
+
+typedef struct reyn_batch_scene_info {
+    uint32_t steps;          /* steps taken */
+    uint32_t iterations;     /* pressure iterations of the last step */
+    float dt;                /* time step of the last step */
+    double time;             /* simulated time */
+    double loop_stat;        /* PCISPH: mean predicted density; IISPH: mean error */
+} reyn_batch_scene_info;
+
+/* Uploads the scenes, builds their boundary grids and Akinci masses. */
+int reyn_batch_create(reyn_ctx* ctx, const reyn_batch_params* prm,
+    const reyn_batch_scene* scenes, uint32_t count, reyn_batch** out);
+int reyn_batch_destroy(reyn_batch* b);
+/* Advances every scene by `steps` steps (time step, grid + reorder, solver
+ * step, bounds). */
+int reyn_batch_step(reyn_batch* b, uint32_t steps);
+int reyn_batch_info(const reyn_batch* b, uint32_t scene, reyn_batch_scene_info* out);
+/* Copies a scene's state (in its current cell order) to host arrays of its n;
+ * any pointer may be NULL. */
+int reyn_batch_download(reyn_batch* b, uint32_t scene, float* x, float* y, float* z,
+    float* vx, float* vy, float* vz, float* rho, float* p);
+
 #if defined(__GNUC__)
 #pragma GCC visibility pop
 #endif

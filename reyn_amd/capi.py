@@ -137,6 +137,30 @@ class IisphWork(C.Structure):
     ]
 
 
+SOLVER_PCISPH, SOLVER_IISPH = 1, 2
+
+
+class BatchScene(C.Structure):
+    _fields_ = [
+        ("n", u32), ("x", vp), ("y", vp), ("z", vp), ("vx", vp), ("vy", vp), ("vz", vp), ("mass", f32),
+        ("n_bdy", u32), ("bx", vp), ("by", vp), ("bz", vp), ("bound_min", Float3), ("bound_max", Float3),
+        ("h", f32),
+    ]
+
+
+class BatchParams(C.Structure):
+    _fields_ = [
+        ("solver", i32), ("kernel", i32), ("nu", f32), ("rho_0", f32), ("g", Float3), ("cell_ratio", f32),
+        ("dt0", f32), ("cfl_lambda", f32), ("change_factor", f32), ("iter_target", u32),
+        ("eta_rho_max", f32), ("omega", f32), ("min_iter", u32), ("max_iter", u32), ("stop_rule", i32),
+        ("warm_start", i32),
+    ]
+
+
+class BatchSceneInfo(C.Structure):
+    _fields_ = [("steps", u32), ("iterations", u32), ("dt", f32), ("time", f64), ("loop_stat", f64)]
+
+
 P = C.POINTER
 _SIGS = {
     "reyn_abi_version": (i32, []),
@@ -179,6 +203,11 @@ _SIGS = {
     "reyn_simple_dt_controller": (i32, [vp, u32, f32, f32, f32, P(Particles), Float3, u32, f32, P(f32)]),
     "reyn_hard_enforce_bounds": (i32, [vp, Float3, Float3, P(Particles)]),
     "reyn_reduce_f32": (i32, [vp, vp, u32, i32, P(f64)]),
+    "reyn_batch_create": (i32, [vp, P(BatchParams), P(BatchScene), u32, P(vp)]),
+    "reyn_batch_destroy": (i32, [vp]),
+    "reyn_batch_step": (i32, [vp, u32]),
+    "reyn_batch_info": (i32, [vp, u32, P(BatchSceneInfo)]),
+    "reyn_batch_download": (i32, [vp, u32, vp, vp, vp, vp, vp, vp, vp, vp]),
 }
 
 EXPORTED = tuple(_SIGS)

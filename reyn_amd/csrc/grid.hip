@@ -13,24 +13,7 @@
 #include <cmath>
 #include <new>
 
-#include "sph.hpp"
-
-struct reyn_grid_builder {
-    reyn_ctx* ctx = nullptr;
-    reyn_grid geom {}; // prefix/sorted/n filled by construct
-    float search_radius = 0.f;
-    size_t ncells = 0;
-    size_t nblocks_scan = 0;
-    uint32_t* counts = nullptr; // ncells
-    uint32_t* prefix = nullptr; // ncells + 1
-    uint32_t* block_sums = nullptr; // nblocks_scan
-    uint32_t* status = nullptr; // [0] out-of-bounds points, [1] max cell count, [2] oversized cells
-    uint32_t cap = 0; // per-point capacity of the arrays below
-    uint32_t* keys = nullptr;
-    uint32_t* slots = nullptr;
-    uint32_t* tmp = nullptr;
-    uint32_t* sorted = nullptr;
-};
+#include "grid.hpp"
 
 namespace reyn {
 namespace {
@@ -80,20 +63,25 @@ __global__ void k_zero_status(uint32_t* status)
         status[threadIdx.x] = 0u;
 }
 
-__global__ __launch_bounds__(kBlock) void k_count(uint32_t n, const float* __restrict__ xs,
-    const float* __restrict__ ys, const float* __restrict__ zs, GridDev g,
+// Keys and in-cell slots; for a batch the key of a point of scene s is offset
+// by cell_off[s], the first cell of that scene in the concatenated tables.
+template <bool BT>
+__global__ __launch_bounds__(kBlock) void k_count(SceneArg sa, const float* __restrict__ xs,
+    const float* __restrict__ ys, const float* __restrict__ zs, const uint32_t* __restrict__ cell_off,
     uint32_t* __restrict__ counts, uint32_t* __restrict__ keys, uint32_t* __restrict__ slots,
     uint32_t* __restrict__ status)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    REYN_SCENE(BT, sa);
+    const GridDev& g = sc.g;
+    const uint32_t i = t;
     const int lane = threadIdx.x & (kWave - 1);
-    const bool valid = i < n;
+    const bool valid = live;
     uint32_t key = 0xFFFFFFFFu;
     bool outside = false;
     if (valid) {
         const f3 p = ld3(xs, ys, zs, i);
         const int3c c = cell_of(g, p);
-        key = static_cast<uint32_t>(c.z * g.nxny + c.y * g.nx + c.x);
+        key = static_cast<uint32_t>(c.z * g.nxny + c.y * g.nx + c.x) + (BT ? cell_off[sc.idx] : 0u);
         const int R = g.range;
         outside = c.x < R || c.y < R || c.z < R || c.x > g.nx - 1 - R || c.y > g.ny - 1 - R
             || c.z > g.nz - 1 - R;
@@ -336,21 +324,14 @@ int ensure_capacity(reyn_grid_builder* b, uint32_t n)
     return REYN_OK;
 }
 
-// count -> scan -> scatter -> stabilize; fills b->geom.prefix/sorted/n.
-int build(reyn_grid_builder* b, float search_radius, const float* xx, const float* xy,
-    const float* xz, uint32_t n, uint32_t* status_host)
+// count -> scan -> scatter -> stabilize over the builder's cells for n
+// points; `count` launches the key/slot kernel.
+template <class Count>
+int sort_points(reyn_grid_builder* b, uint32_t n, uint32_t* status_host, Count&& count)
 {
     reyn_ctx* ctx = b->ctx;
-    if (!(search_radius > 0.f) || search_radius > b->search_radius * 1.0000001f)
-        return set_error(ctx, REYN_ERR_INVALID_ARGUMENT,
-            "construct: search radius %g must be positive and at most the builder's %g",
-            static_cast<double>(search_radius), static_cast<double>(b->search_radius));
-    if (n > 0 && (!xx || !xy || !xz))
-        return set_error(ctx, REYN_ERR_INVALID_ARGUMENT, "construct: NULL position pointer");
     REYN_HIP(ctx, hipSetDevice(ctx->device));
     REYN_TRY(ensure_capacity(b, n > 0 ? n : 1));
-    b->geom.r_c_2 = search_radius * search_radius;
-    GridDev g = to_dev(b->geom);
     hipStream_t st = ctx->stream;
     {
         ProfScope _prof(ctx, "k_zero_status");
@@ -361,7 +342,7 @@ int build(reyn_grid_builder* b, float search_radius, const float* xx, const floa
     if (n > 0) {
         {
             ProfScope _prof(ctx, "k_count");
-            k_count<<<nb, kBlock, 0, st>>>(n, xx, xy, xz, g, b->counts, b->keys, b->slots, b->status);
+            count(st);
         }
         REYN_LAUNCHED(ctx, "k_count");
     }
@@ -394,11 +375,45 @@ int build(reyn_grid_builder* b, float search_radius, const float* xx, const floa
         }
         REYN_LAUNCHED(ctx, "k_stabilize");
     }
+    REYN_HIP(ctx, hipMemcpyAsync(status_host, b->status, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    return REYN_OK;
+}
+
+// One scene: fills b->geom.prefix/sorted/n.
+int build(reyn_grid_builder* b, float search_radius, const float* xx, const float* xy,
+    const float* xz, uint32_t n, uint32_t* status_host)
+{
+    reyn_ctx* ctx = b->ctx;
+    if (!(search_radius > 0.f) || search_radius > b->search_radius * 1.0000001f)
+        return set_error(ctx, REYN_ERR_INVALID_ARGUMENT,
+            "construct: search radius %g must be positive and at most the builder's %g",
+            static_cast<double>(search_radius), static_cast<double>(b->search_radius));
+    if (n > 0 && (!xx || !xy || !xz))
+        return set_error(ctx, REYN_ERR_INVALID_ARGUMENT, "construct: NULL position pointer");
+    b->geom.r_c_2 = search_radius * search_radius;
+    SceneArg sa {};
+    sa.one.g = to_dev(b->geom);
+    sa.one.n = n;
+    sa.one.nblk = blocks_for(n);
+    REYN_TRY(sort_points(b, n, status_host, [&](hipStream_t st) {
+        k_count<false><<<blocks_for(n), kBlock, 0, st>>>(sa, xx, xy, xz, nullptr, b->counts, b->keys, b->slots,
+            b->status);
+    }));
     b->geom.prefix = b->prefix;
     b->geom.sorted = b->sorted;
     b->geom.n = n;
-    REYN_HIP(ctx, hipMemcpyAsync(status_host, b->status, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     return REYN_OK;
+}
+
+} // namespace
+
+int sort_batch(reyn_grid_builder* b, const SceneArg& sa, uint32_t nblocks, const uint32_t* cell_off,
+    const float* xx, const float* xy, const float* xz, uint32_t n, uint32_t* status_host)
+{
+    return sort_points(b, n, status_host, [&](hipStream_t st) {
+        k_count<true><<<nblocks, kBlock, 0, st>>>(sa, xx, xy, xz, cell_off, b->counts, b->keys, b->slots,
+            b->status);
+    });
 }
 
 int finish_build(reyn_grid_builder* b, const uint32_t* status_host)
@@ -439,19 +454,13 @@ int launch_gather(reyn_ctx* ctx, const uint32_t* sorted, uint32_t n, const float
     return REYN_OK;
 }
 
-} // namespace
 } // namespace reyn
 
-using namespace reyn;
+namespace reyn {
 
-extern "C" {
-
-int reyn_grid_builder_create(reyn_ctx* ctx, reyn_float3 bmin, reyn_float3 bmax,
-    float search_radius, float cell_ratio, reyn_grid_builder** out)
+int grid_geometry(reyn_ctx* ctx, reyn_float3 bmin, reyn_float3 bmax, float search_radius, float cell_ratio,
+    reyn_grid* out, size_t* ncells)
 {
-    if (!ctx || !out)
-        return REYN_ERR_INVALID_ARGUMENT;
-    *out = nullptr;
     if (!(search_radius > 0.f) || !std::isfinite(search_radius))
         return set_error(ctx, REYN_ERR_INVALID_ARGUMENT, "grid: search radius must be positive");
     if (!(cell_ratio >= 0.25f && cell_ratio <= 4.f))
@@ -472,21 +481,30 @@ int reyn_grid_builder_create(reyn_ctx* ctx, reyn_float3 bmin, reyn_float3 bmax,
     const double cells = double(n[0]) * double(n[1]) * double(n[2]);
     if (cells >= 1073741824.0)
         return set_error(ctx, REYN_ERR_CAPACITY, "grid: %.0f cells exceed the 2^30 cell limit", cells);
+    *out = reyn_grid {};
+    out->bound_min = { lo[0], lo[1], lo[2] };
+    out->cell_size = cell;
+    out->r_c_2 = search_radius * search_radius;
+    out->nx = n[0];
+    out->ny = n[1];
+    out->nz = n[2];
     // cells searched on each side: ceil(radius / cell)
-    const int range = static_cast<int>(std::ceil(1.0 / double(cell_ratio) - 1e-6));
+    out->range = static_cast<int>(std::ceil(1.0 / double(cell_ratio) - 1e-6));
+    *ncells = size_t(n[0]) * size_t(n[1]) * size_t(n[2]);
+    return REYN_OK;
+}
+
+int builder_alloc(reyn_ctx* ctx, size_t ncells, float search_radius, reyn_grid_builder** out)
+{
+    *out = nullptr;
+    if (ncells >= (size_t(1) << 32) - 1)
+        return set_error(ctx, REYN_ERR_CAPACITY, "grid: %zu cells exceed the 2^32 - 1 cell limit", ncells);
     reyn_grid_builder* b = new (std::nothrow) reyn_grid_builder();
     if (!b)
         return REYN_ERR_OUT_OF_MEMORY;
     b->ctx = ctx;
     b->search_radius = search_radius;
-    b->geom.bound_min = { lo[0], lo[1], lo[2] };
-    b->geom.cell_size = cell;
-    b->geom.r_c_2 = search_radius * search_radius;
-    b->geom.nx = n[0];
-    b->geom.ny = n[1];
-    b->geom.nz = n[2];
-    b->geom.range = range;
-    b->ncells = size_t(n[0]) * size_t(n[1]) * size_t(n[2]);
+    b->ncells = ncells;
     b->nblocks_scan = (b->ncells + kScanTile - 1) / kScanTile;
     int rc = check_hip(ctx, hipSetDevice(ctx->device), "hipSetDevice");
     if (rc == REYN_OK)
@@ -505,6 +523,28 @@ int reyn_grid_builder_create(reyn_ctx* ctx, reyn_float3 bmin, reyn_float3 bmax,
         reyn_grid_builder_destroy(b);
         return rc;
     }
+    *out = b;
+    return REYN_OK;
+}
+
+} // namespace reyn
+
+using namespace reyn;
+
+extern "C" {
+
+int reyn_grid_builder_create(reyn_ctx* ctx, reyn_float3 bmin, reyn_float3 bmax,
+    float search_radius, float cell_ratio, reyn_grid_builder** out)
+{
+    if (!ctx || !out)
+        return REYN_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    reyn_grid geom {};
+    size_t ncells = 0;
+    REYN_TRY(grid_geometry(ctx, bmin, bmax, search_radius, cell_ratio, &geom, &ncells));
+    reyn_grid_builder* b = nullptr;
+    REYN_TRY(builder_alloc(ctx, ncells, search_radius, &b));
+    b->geom = geom;
     *out = b;
     return REYN_OK;
 }

@@ -14,24 +14,12 @@
 #include <algorithm>
 #include <cmath>
 
-#include "sph.hpp"
+#include <vector>
+
+#include "batch.hpp"
 
 namespace reyn {
 namespace {
-
-SphK make_sphk(const reyn_sph_params& p)
-{
-    SphK c;
-    c.kp = make_kparams(p.kernel, p.h_bar);
-    c.h = p.h;
-    c.nu = p.nu;
-    c.rho0 = p.rho_0;
-    c.g = { p.g.x, p.g.y, p.g.z };
-    c.visc_c = 10.f * p.nu;
-    c.visc_eps = static_cast<float>(0.01 * double(p.h) * double(p.h));
-    c.rho0_sq_inv = 1.f / (p.rho_0 * p.rho_0);
-    return c;
-}
 
 struct SolveCtl {
     uint32_t iter;
@@ -81,13 +69,15 @@ constexpr uint32_t kRing = REYN_LIST_RING, kFlush = 4;
 
 // lists: the solver-step scratch (packed records, and the neighbour list
 // while its entries fit 32-bit byte offsets); otherwise only the boundary
-// sums and the loop control.
-int get_scratch(reyn_ctx* ctx, uint32_t n, Scratch* s, bool lists = false)
+// sums and the loop control. `nblocks` workgroups (0: one scene of n) and
+// `nscenes` loop-control blocks.
+int get_scratch(reyn_ctx* ctx, uint32_t n, Scratch* s, bool lists = false, uint32_t nblocks = 0,
+    uint32_t nscenes = 1)
 {
-    const size_t nb = blocks_for(n);
+    const size_t nb = nblocks ? nblocks : blocks_for(n);
     auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
     const size_t f = al(size_t(n) * 4);
-    const size_t head = 4 * f + al(nb * 8) + al(nb * 4) + al(sizeof(SolveCtl));
+    const size_t head = 4 * f + al(nb * 8) + al(nb * 4) + al(sizeof(SolveCtl) * nscenes);
     // records and list entries are addressed with 32-bit byte offsets
     const bool records = lists && size_t(n) * 32 < (size_t(1) << 32);
     const bool keep_list = records && size_t(n) * kListCap * 4 < (size_t(1) << 32);
@@ -989,6 +979,36 @@ int iisph_run(reyn_ctx* ctx, int kind, const Common& c, const IisphLoop& lp, con
 }
 
 } // namespace
+
+int batch_solve(reyn_ctx* ctx, const BatchTarget& bt, const BatchSolver& bs, uint32_t* iters, double* errs)
+{
+    Common c;
+    c.f = bt.f;
+    c.a = bt.a;
+    c.ind = false;
+    c.nb = bt.nblocks;
+    Scratch s;
+    REYN_TRY(get_scratch(ctx, bt.f.n, &s, true, bt.nblocks, bt.nscenes));
+    if (!s.X)
+        return set_error(ctx, REYN_ERR_CAPACITY, "batch: %u particles exceed the packed-record limit", bt.f.n);
+    std::vector<SolveCtl> res(bt.nscenes);
+    if (bs.solver == REYN_SOLVER_PCISPH) {
+        const reyn_pcisph_work w { bt.rho, bt.ax, bt.ay, bt.az };
+        REYN_TRY(pcisph_run<true>(ctx, bs.kernel, c, &w, s, bt.nscenes, bs.min_iter, bs.max_iter, res.data()));
+    } else if (bs.solver == REYN_SOLVER_IISPH) {
+        const reyn_iisph_work w { bt.rho, bt.p, bt.a_ii, bt.s_i, bt.ax, bt.ay, bt.az };
+        const IisphLoop lp { bs.omega, bs.eta, bs.min_iter, bs.max_iter, bs.check_every, bs.stop_rule, bs.warm };
+        REYN_TRY(iisph_run<true>(ctx, bs.kernel, c, lp, &w, s, bt.nscenes, res.data()));
+    } else {
+        return set_error(ctx, REYN_ERR_INVALID_ARGUMENT, "batch: unknown solver %d", bs.solver);
+    }
+    for (uint32_t k = 0; k < bt.nscenes; ++k) {
+        iters[k] = res[k].iter;
+        errs[k] = bs.solver == REYN_SOLVER_PCISPH ? res[k].avg_density : res[k].mean_rel_error;
+    }
+    return REYN_OK;
+}
+
 } // namespace reyn
 
 using namespace reyn;

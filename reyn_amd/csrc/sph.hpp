@@ -437,6 +437,20 @@ struct SphK {
     float rho0_sq_inv; // 1 / rho0^2
 };
 
+inline SphK make_sphk(const reyn_sph_params& p)
+{
+    SphK c;
+    c.kp = make_kparams(p.kernel, p.h_bar);
+    c.h = p.h;
+    c.nu = p.nu;
+    c.rho0 = p.rho_0;
+    c.g = { p.g.x, p.g.y, p.g.z };
+    c.visc_c = 10.f * p.nu;
+    c.visc_eps = static_cast<float>(0.01 * double(p.h) * double(p.h));
+    c.rho0_sq_inv = 1.f / (p.rho_0 * p.rho_0);
+    return c;
+}
+
 struct SceneDev {
     GridDev g;
     BdyDev b;
@@ -447,6 +461,7 @@ struct SceneDev {
     uint32_t first, n; // particle range [first, first + n)
     uint32_t blk_first, nblk; // its workgroups (after the XCD remap)
     uint32_t idx; // scene index: its loop-control block
+    f3 box_lo, box_hi; // bounds of the scene's domain (batch bounds pass)
 };
 
 struct SceneArg {

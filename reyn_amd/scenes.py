@@ -144,3 +144,21 @@ def config3_iisph(side: int = 128, seed: int = 3) -> Scene:
     radius = float(rng.uniform(0.1, 0.25))
     return dam_break((side, side, side), s, (2.0, 1.25, 2.0), jitter=0.01, seed=seed,
                      sphere=((1.5, 0.25, 1.0), radius))
+
+
+def config4_batch(count: int = 64, seed: int = 4, n_lo: float = 1e4, n_hi: float = 3e5) -> list[Scene]:
+    """`count` dam-break scenes: N_s = round(exp U[ln n_lo, ln n_hi]) target
+    particles, a box (a, 1, 1) with a ~ U[1, 3], a fluid column of fill
+    f ~ U[0.2, 0.6] along x at full height, spacing s = (f a / N_s)^(1/3)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        n_s = int(round(float(np.exp(rng.uniform(np.log(n_lo), np.log(n_hi))))))
+        a = float(rng.uniform(1.0, 3.0))
+        f = float(rng.uniform(0.2, 0.6))
+        s = float((f * a / n_s) ** (1.0 / 3.0))
+        counts = (max(1, int(f * a / s)), max(1, int(1.0 / s)), max(1, int(1.0 / s)))
+        sc = dam_break(counts, s, (a, 1.0, 1.0), jitter=0.01, seed=seed * 1000 + k)
+        sc.meta.update({"target_n": n_s, "aspect": a, "fill": f})
+        out.append(sc)
+    return out

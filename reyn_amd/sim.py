@@ -365,3 +365,78 @@ class Simulation:
         info = getattr(self.solver, "info", None)
         return StepStats(dt, int(iters), float(info.mean_rel_error) if info else 0.0,
                          int(info.n_violating) if info else 0)
+
+
+class Batch:
+    """Many scenes advanced together (`reyn_batch_*`): one launch per phase over
+    all scenes, each scene with its own grid, boundary, spacing, time step
+    (the reference main loop's controller) and pressure-loop control."""
+
+    def __init__(self, ctx: Context, scenes, solver: str = "pcisph", kernel: str | int = "W6", nu: float = 0.001,
+                 cell_ratio: float = 0.5, dt0: float = 0.0005, cfl_lambda: float = 1.0,
+                 change_factor: float = 0.001, iter_target: int = 10, eta_rho_max: float | None = None,
+                 omega: float = 0.5, min_iter: int = 3, max_iter: int = 100,
+                 stop_rule: int = capi.IISPH_STOP_MEAN, warm_start: bool = True, g=GRAVITY):
+        self.ctx = ctx
+        self.h = None
+        self.scenes = list(scenes)
+        solver_id = {"pcisph": capi.SOLVER_PCISPH, "iisph": capi.SOLVER_IISPH}[solver]
+        if eta_rho_max is None:
+            eta_rho_max = 0.001
+        k = capi.KERNELS[kernel] if isinstance(kernel, str) else int(kernel)
+        self.prm = capi.BatchParams(solver_id, k, float(nu), float(self.scenes[0].rho_0), capi.f3(g),
+                                    float(cell_ratio), float(dt0), float(cfl_lambda), float(change_factor),
+                                    int(iter_target), float(eta_rho_max), float(omega), int(min_iter),
+                                    int(max_iter), int(stop_rule), int(bool(warm_start)))
+        keep = []
+
+        def ptr(a):
+            a = np.ascontiguousarray(a, dtype=np.float32)
+            keep.append(a)
+            return a.ctypes.data
+
+        arr = (capi.BatchScene * len(self.scenes))()
+        for i, sc in enumerate(self.scenes):
+            x = np.asarray(sc.x, np.float32)
+            xb = np.asarray(sc.xb, np.float32)
+            arr[i] = capi.BatchScene(sc.n, ptr(x[:, 0]), ptr(x[:, 1]), ptr(x[:, 2]), None, None, None,
+                                     float(sc.mass), sc.n_bdy, ptr(xb[:, 0]) if sc.n_bdy else None,
+                                     ptr(xb[:, 1]) if sc.n_bdy else None, ptr(xb[:, 2]) if sc.n_bdy else None,
+                                     capi.f3(sc.bound_min), capi.f3(sc.bound_max), float(sc.spacing))
+        self.h = C.c_void_p()
+        ctx.check(ctx._lib.reyn_batch_create(ctx.h, C.byref(self.prm), arr, len(self.scenes), C.byref(self.h)),
+                  "batch create")
+        ctx.adopt(self)
+
+    @property
+    def n(self) -> int:
+        return sum(sc.n for sc in self.scenes)
+
+    def step(self, steps: int = 1) -> None:
+        self.ctx.check(self.ctx._lib.reyn_batch_step(self.h, int(steps)), "batch step")
+
+    def info(self, scene: int) -> capi.BatchSceneInfo:
+        out = capi.BatchSceneInfo()
+        self.ctx.check(self.ctx._lib.reyn_batch_info(self.h, int(scene), C.byref(out)), "batch info")
+        return out
+
+    def download(self, scene: int) -> dict:
+        n = self.scenes[scene].n
+        out = {k: np.empty(n, np.float32) for k in ("xx", "xy", "xz", "vx", "vy", "vz", "rho", "p")}
+        ptrs = [out[k].ctypes.data for k in ("xx", "xy", "xz", "vx", "vy", "vz", "rho", "p")]
+        self.ctx.check(self.ctx._lib.reyn_batch_download(self.h, int(scene), *ptrs), "batch download")
+        return out
+
+    def release(self) -> None:
+        if self.h is not None and self.h.value and self.ctx.h:
+            self.ctx._lib.reyn_batch_destroy(self.h)
+        self.h = None
+
+    close = release
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+

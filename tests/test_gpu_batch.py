@@ -1,0 +1,67 @@
+"""Scene batches (reyn_batch_*): a scene advanced inside a batch gives the
+same bits as the same scene alone, and a batch of one gives the same bits as
+the single-scene entry points driven by the reference's main loop (grid +
+reorder, controller time step, solver step, bounds)."""
+
+import numpy as np
+import pytest
+
+from reyn_amd import capi, scenes
+from reyn_amd.sim import IISPH, PCISPH, Batch, BoundarySamples, Particles, Simulation
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("xx", "xy", "xz", "vx", "vy", "vz", "rho")
+
+
+def small_scenes(count=4, seed=7):
+    return scenes.config4_batch(count=count, seed=seed, n_lo=2000, n_hi=8000)
+
+
+def assert_same(a, b, what):
+    for k in FIELDS:
+        assert np.array_equal(a[k], b[k]), f"{what}: field {k} differs (max |d| {np.abs(a[k] - b[k]).max()})"
+
+
+@pytest.mark.parametrize("solver,kernel", [("pcisph", "W6"), ("iisph", "B3")])
+def test_scene_alone_equals_scene_in_batch(ctx, solver, kernel):
+    scs = small_scenes()
+    print(f"{solver}: scenes of {[sc.n for sc in scs]} particles", flush=True)
+    mix = Batch(ctx, scs, solver=solver, kernel=kernel)
+    mix.step(12)
+    for s, sc in enumerate(scs):
+        alone = Batch(ctx, [sc], solver=solver, kernel=kernel)
+        alone.step(12)
+        a, b = alone.download(0), mix.download(s)
+        assert_same(a, b, f"scene {s}")
+        ia, ib = alone.info(0), mix.info(s)
+        assert (ia.steps, ia.iterations, ia.dt, ia.time) == (ib.steps, ib.iterations, ib.dt, ib.time)
+        alone.close()
+    mix.close()
+
+
+def test_batch_of_one_equals_single_scene_path(ctx):
+    sc = small_scenes(count=1, seed=11)[0]
+    h = sc.spacing
+    state = Particles(ctx, sc.x, None, sc.mass, sc.rho_0)
+    bdy = BoundarySamples(ctx, sc.xb, h, sc.rho_0)
+    solver = PCISPH(ctx, "W6", sc.n, 0.001, h, sc.rho_0, eta_rho_max=0.001, min_iter=3, max_iter=100)
+    sim = Simulation(ctx, solver, state, bdy, h, sc.bound_min, sc.bound_max, cfl_lambda=1.0,
+                     dt_policy="controller", dt0=0.0005, iter_target=10, change_factor=0.001)
+    b = Batch(ctx, [sc], solver="pcisph", kernel="W6")
+    for k in range(8):
+        st = sim.step()
+        b.step(1)
+        info = b.info(0)
+        assert (info.iterations, info.dt) == (st.iterations, np.float32(st.dt)), k
+    got = b.download(0)
+    ref = state.host()
+    ref["rho"] = solver.rho.numpy()
+    assert_same(got, ref, "batch of one vs single-scene path")
+    b.close()
+
+
+def test_batch_rejects_bad_input(ctx):
+    sc = small_scenes(count=1)[0]
+    with pytest.raises(capi.ReynError):
+        Batch(ctx, [sc], solver="pcisph", kernel="W6", dt0=0.0)
