@@ -407,6 +407,13 @@ typedef struct reyn_batch_params {
     float eta_rho_max, omega;
     uint32_t min_iter, max_iter;
     int32_t stop_rule, warm_start;
+    /* 1: after the first step, every step runs as captured HIP graphs (time
+     * step, grid, reorder and loop start; chunks of loop iterations; the end),
+     * with one host check of the loop control per launch */
+    int32_t graph;
+    /* loop iterations launched between host checks after the first min_iter
+     * (0 = 4); later chunks launch only the scenes whose loop is not done */
+    uint32_t check_every;
 } reyn_batch_params;
 
 typedef struct reyn_batch_scene_info {

@@ -153,7 +153,7 @@ class BatchParams(C.Structure):
         ("solver", i32), ("kernel", i32), ("nu", f32), ("rho_0", f32), ("g", Float3), ("cell_ratio", f32),
         ("dt0", f32), ("cfl_lambda", f32), ("change_factor", f32), ("iter_target", u32),
         ("eta_rho_max", f32), ("omega", f32), ("min_iter", u32), ("max_iter", u32), ("stop_rule", i32),
-        ("warm_start", i32),
+        ("warm_start", i32), ("graph", i32), ("check_every", u32),
     ]
 
 

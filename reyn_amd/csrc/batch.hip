@@ -95,6 +95,14 @@ __global__ __launch_bounds__(kBlock) void k_batch_bounds(SceneArg sa, FluidDev f
     st3(f.x, f.y, f.z, t, cl);
 }
 
+// Packed boundary records {x, y, z, m} of one scene (the samples are static).
+__global__ __launch_bounds__(kBlock) void k_batch_pack_bdy(BdyDev b, float4* __restrict__ B4)
+{
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < b.n)
+        B4[k] = make_float4(b.x[k], b.y[k], b.z[k], b.m[k]);
+}
+
 constexpr int kFields = 8; // x y z vx vy vz m p
 
 } // namespace
@@ -113,11 +121,16 @@ struct reyn_batch {
     std::vector<reyn_batch_scene_info> info;
     std::vector<uint32_t> iters;
     std::vector<double> stat;
+    std::vector<uint8_t> scene_done;
+    std::vector<uint32_t> blk_first, nblk; // workgroups of every scene
+    uint32_t* blk_list = nullptr; // device: the workgroups of a compacted launch
+    uint32_t* blk_list_host = nullptr; // pinned staging of that list
     std::vector<void*> allocs;
     float* st[2][kFields] = {};
     int cur = 0;
     float *rho = nullptr, *ax = nullptr, *ay = nullptr, *az = nullptr, *a_ii = nullptr, *s_i = nullptr;
     float *bm = nullptr;
+    float4* b4 = nullptr; // packed boundary records of all scenes
     SceneDev* table = nullptr;
     uint32_t* blk_scene = nullptr;
     uint32_t* cell_off = nullptr;
@@ -126,7 +139,12 @@ struct reyn_batch {
     double* vpart = nullptr;
     float* dt_dev = nullptr;
     uint32_t* bad = nullptr;
-    void* pinned = nullptr; // status, iterations, time steps, flags
+    void* scratch = nullptr; // the solver scratch of the batch
+    void* pinned = nullptr; // status, flags, time steps, loop control
+    // captured steps, per state buffer the step starts from: the time step,
+    // grid, reorder and loop start; one chunk of loop iterations; the end
+    bool graphs = false;
+    hipGraphExec_t g_begin[2] = {}, g_chunk[2] = {}, g_end[2] = {};
 };
 
 namespace {
@@ -206,6 +224,7 @@ int create(reyn_batch* b, const reyn_batch_params* prm, const reyn_batch_scene* 
         REYN_TRY(balloc_n(b, &bz[k], NB));
     }
     REYN_TRY(balloc_n(b, &b->bm, NB));
+    REYN_TRY(balloc_n(b, &b->b4, NB));
     REYN_TRY(balloc_n(b, &b->table, count));
     REYN_TRY(balloc_n(b, &b->blk_scene, b->nblocks));
     REYN_TRY(balloc_n(b, &b->cell_off, count));
@@ -214,7 +233,11 @@ int create(reyn_batch* b, const reyn_batch_params* prm, const reyn_batch_scene* 
     REYN_TRY(balloc_n(b, &b->vpart, b->nblocks));
     REYN_TRY(balloc_n(b, &b->dt_dev, count));
     REYN_TRY(balloc_n(b, &b->bad, count));
-    REYN_HIP(ctx, hipHostMalloc(&b->pinned, 64 + size_t(count) * 16, hipHostMallocDefault));
+    REYN_HIP(ctx, hipHostMalloc(&b->pinned, 256 + size_t(count) * 64, hipHostMallocDefault));
+    REYN_TRY(balloc(b, &b->scratch, batch_scratch_bytes(b->N, b->nblocks, count)));
+    REYN_TRY(balloc_n(b, &b->blk_list, b->nblocks));
+    REYN_HIP(ctx, hipHostMalloc(reinterpret_cast<void**>(&b->blk_list_host), sizeof(uint32_t) * b->nblocks,
+        hipHostMallocDefault));
     // fluid state
     std::vector<float> zeros;
     uint64_t cells = 0, boff = 0;
@@ -256,6 +279,12 @@ int create(reyn_batch* b, const reyn_batch_params* prm, const reyn_batch_scene* 
             reyn_boundary view { fields[0], fields[1], fields[2], b->bm + boff, sc.n_bdy, g };
             REYN_TRY(reyn_boundary_masses(ctx, &view, b->bm + boff, h, prm->rho_0, 5));
             bd = to_dev(&view);
+            bd.p4 = b->b4 + boff;
+            {
+                ProfScope _prof(ctx, "k_pack_bdy");
+                k_batch_pack_bdy<<<blocks_for(sc.n_bdy), kBlock, 0, ctx->stream>>>(bd, b->b4 + boff);
+            }
+            REYN_LAUNCHED(ctx, "k_pack_bdy");
             boff += nb;
         }
         // fluid grid geometry in the concatenated cell space
@@ -282,6 +311,8 @@ int create(reyn_batch* b, const reyn_batch_params* prm, const reyn_batch_scene* 
         d.box_lo = { sc.bound_min.x, sc.bound_min.y, sc.bound_min.z };
         d.box_hi = { sc.bound_max.x, sc.bound_max.y, sc.bound_max.z };
         blk_scene.insert(blk_scene.end(), d.nblk, s);
+        b->blk_first.push_back(d.blk_first);
+        b->nblk.push_back(d.nblk);
         if (prm->solver == REYN_SOLVER_PCISPH)
             REYN_TRY(reyn_pcisph_delta(prm->kernel, 2.f * h, h, &delta[s]));
     }
@@ -297,20 +328,61 @@ int create(reyn_batch* b, const reyn_batch_params* prm, const reyn_batch_scene* 
     b->info.assign(count, reyn_batch_scene_info {});
     b->iters.assign(count, 0u);
     b->stat.assign(count, 0.0);
+    b->scene_done.assign(count, 0);
     return REYN_OK;
 }
 
-int step(reyn_batch* b)
+// Host staging in the pinned buffer.
+struct Staging {
+    uint32_t* status; // 4: build status
+    uint32_t* bad; // S: non-finite velocities per scene
+    float* dt; // S: time step per scene
+    void* ctl; // S loop-control blocks
+};
+Staging staging(const reyn_batch* b)
+{
+    char* p = static_cast<char*>(b->pinned);
+    Staging st;
+    st.status = reinterpret_cast<uint32_t*>(p);
+    st.bad = reinterpret_cast<uint32_t*>(p + 64);
+    st.dt = reinterpret_cast<float*>(p + 64 + 4 * size_t(b->S));
+    st.ctl = p + 128 + 8 * size_t(b->S) + (64 - (8 * size_t(b->S)) % 64) % 64;
+    return st;
+}
+
+BatchTarget target(const reyn_batch* b)
+{
+    BatchTarget bt {};
+    bt.f = batch_fluid(b);
+    bt.a = batch_arg(b);
+    bt.nblocks = b->nblocks;
+    bt.nscenes = b->S;
+    bt.rho = b->rho;
+    bt.p = b->st[b->cur][kFields - 1];
+    bt.a_ii = b->a_ii;
+    bt.s_i = b->s_i;
+    bt.ax = b->ax;
+    bt.ay = b->ay;
+    bt.az = b->az;
+    bt.scratch = b->scratch;
+    return bt;
+}
+
+BatchSolver solver_of(const reyn_batch* b)
+{
+    const reyn_batch_params& prm = b->prm;
+    return BatchSolver { prm.solver, prm.kernel, prm.omega, prm.eta_rho_max, prm.min_iter, prm.max_iter, 4,
+        prm.stop_rule, prm.warm_start };
+}
+
+// Time step, grid over all scenes, reorder into the other state buffer and
+// the solver's begin phase with the first min_iter loop iterations. Flips
+// `cur`. No host synchronisation.
+int launch_begin(reyn_batch* b, uint32_t* status_host)
 {
     reyn_ctx* ctx = b->ctx;
     const reyn_batch_params& prm = b->prm;
     const SceneArg sa = batch_arg(b);
-    const uint32_t S = b->S;
-    uint32_t* status = static_cast<uint32_t*>(b->pinned);
-    uint32_t* iters_host = status + 16;
-    float* dt_host = reinterpret_cast<float*>(iters_host + S);
-    uint32_t* bad_host = reinterpret_cast<uint32_t*>(dt_host + S);
-    // time step per scene, on the device
     {
         ProfScope _prof(ctx, "k_batch_vmax");
         k_batch_vmax<<<b->nblocks, kBlock, 0, ctx->stream>>>(sa, batch_fluid(b), b->vpart);
@@ -320,58 +392,164 @@ int step(reyn_batch* b)
         prm.solver == REYN_SOLVER_PCISPH ? 1 : 0 };
     {
         ProfScope _prof(ctx, "k_batch_dt");
-        k_batch_dt<<<S, kBlock, 0, ctx->stream>>>(b->table, b->vpart, b->iters_dev, b->delta, dp, b->dt_dev, b->bad);
+        k_batch_dt<<<b->S, kBlock, 0, ctx->stream>>>(b->table, b->vpart, b->iters_dev, b->delta, dp, b->dt_dev,
+            b->bad);
     }
     REYN_LAUNCHED(ctx, "k_batch_dt");
-    // grid over all scenes and the reorder of the state and the pressure
-    {
-        const FluidDev f = batch_fluid(b);
-        REYN_TRY(sort_batch(b->fgrid, sa, b->nblocks, b->cell_off, f.x, f.y, f.z, b->N, status));
-        const int nf = prm.solver == REYN_SOLVER_IISPH ? kFields : kFields - 1;
-        REYN_TRY(launch_gather(ctx, b->fgrid->sorted, b->N, b->st[b->cur], b->st[1 - b->cur], nf));
-        b->cur = 1 - b->cur;
-        REYN_TRY(finish_build(b->fgrid, status));
-    }
-    REYN_HIP(ctx, hipMemcpyAsync(bad_host, b->bad, sizeof(uint32_t) * S, hipMemcpyDeviceToHost, ctx->stream));
-    REYN_HIP(ctx, hipMemcpyAsync(dt_host, b->dt_dev, sizeof(float) * S, hipMemcpyDeviceToHost, ctx->stream));
-    REYN_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    for (uint32_t s = 0; s < S; ++s)
-        if (bad_host[s])
-            return set_error(ctx, REYN_ERR_NOT_FINITE, "batch: scene %u has non-finite velocities", s);
-    // the solver step of every scene
-    BatchTarget bt {};
-    bt.f = batch_fluid(b);
-    bt.a = sa;
-    bt.nblocks = b->nblocks;
-    bt.nscenes = S;
-    bt.rho = b->rho;
-    bt.p = b->st[b->cur][kFields - 1];
-    bt.a_ii = b->a_ii;
-    bt.s_i = b->s_i;
-    bt.ax = b->ax;
-    bt.ay = b->ay;
-    bt.az = b->az;
-    const BatchSolver bs { prm.solver, prm.kernel, prm.omega, prm.eta_rho_max, prm.min_iter, prm.max_iter, 4,
-        prm.stop_rule, prm.warm_start };
-    REYN_TRY(batch_solve(ctx, bt, bs, b->iters.data(), b->stat.data()));
+    const FluidDev f = batch_fluid(b);
+    REYN_TRY(sort_batch(b->fgrid, sa, b->nblocks, b->cell_off, f.x, f.y, f.z, b->N, status_host));
+    const int nf = prm.solver == REYN_SOLVER_IISPH ? kFields : kFields - 1;
+    REYN_TRY(launch_gather(ctx, b->fgrid->sorted, b->N, b->st[b->cur], b->st[1 - b->cur], nf));
+    b->cur = 1 - b->cur;
+    return batch_phase(ctx, target(b), solver_of(b), BATCH_BEGIN, std::max(prm.min_iter, 1u));
+}
+
+int launch_chunk(reyn_batch* b, uint32_t count)
+{
+    return batch_phase(b->ctx, target(b), solver_of(b), BATCH_ITERATE, count);
+}
+
+// A chunk of loop iterations over the workgroups of the scenes whose loop is
+// not done yet (the done scenes' workgroups would only return at once).
+int launch_chunk_active(reyn_batch* b, uint32_t count)
+{
+    reyn_ctx* ctx = b->ctx;
+    uint32_t n = 0;
+    for (uint32_t s = 0; s < b->S; ++s)
+        if (!b->scene_done[s])
+            for (uint32_t k = 0; k < b->nblk[s]; ++k)
+                b->blk_list_host[n++] = b->blk_first[s] + k;
+    if (n == 0)
+        return REYN_OK;
+    REYN_HIP(ctx, hipMemcpyAsync(b->blk_list, b->blk_list_host, sizeof(uint32_t) * n, hipMemcpyHostToDevice,
+        ctx->stream));
+    BatchTarget bt = target(b);
+    bt.a.blk_list = b->blk_list;
+    bt.nlaunch = n;
+    return batch_phase(ctx, bt, solver_of(b), BATCH_ITERATE, count);
+}
+
+int launch_end(reyn_batch* b)
+{
+    reyn_ctx* ctx = b->ctx;
+    REYN_TRY(batch_phase(ctx, target(b), solver_of(b), BATCH_END, 0));
     {
         ProfScope _prof(ctx, "k_batch_bounds");
-        k_batch_bounds<<<b->nblocks, kBlock, 0, ctx->stream>>>(sa, batch_fluid(b));
+        k_batch_bounds<<<b->nblocks, kBlock, 0, ctx->stream>>>(batch_arg(b), batch_fluid(b));
     }
     REYN_LAUNCHED(ctx, "k_batch_bounds");
-    for (uint32_t s = 0; s < S; ++s)
-        iters_host[s] = b->iters[s];
-    REYN_TRY(upload(ctx, b->iters_dev, iters_host, sizeof(uint32_t) * S));
-    for (uint32_t s = 0; s < S; ++s) {
+    return batch_save_iterations(ctx, target(b), b->iters_dev);
+}
+
+// Copies the loop control (and with `first` the build status, flags and time
+// steps) to the host and waits; returns whether every scene's loop is done.
+int fetch(reyn_batch* b, bool first, bool* done)
+{
+    reyn_ctx* ctx = b->ctx;
+    const Staging st = staging(b);
+    const uint32_t S = b->S;
+    if (first) {
+        REYN_HIP(ctx, hipMemcpyAsync(st.status, b->fgrid->status, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost,
+            ctx->stream));
+        REYN_HIP(ctx, hipMemcpyAsync(st.bad, b->bad, sizeof(uint32_t) * S, hipMemcpyDeviceToHost, ctx->stream));
+        REYN_HIP(ctx, hipMemcpyAsync(st.dt, b->dt_dev, sizeof(float) * S, hipMemcpyDeviceToHost, ctx->stream));
+    }
+    REYN_HIP(ctx, hipMemcpyAsync(st.ctl, batch_loop_control(target(b)), batch_loop_control_bytes(S),
+        hipMemcpyDeviceToHost, ctx->stream));
+    REYN_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (first) {
+        for (uint32_t s = 0; s < S; ++s)
+            if (st.bad[s])
+                return set_error(ctx, REYN_ERR_NOT_FINITE, "batch: scene %u has non-finite velocities", s);
+        REYN_TRY(finish_build(b->fgrid, st.status));
+    }
+    int all = 0;
+    batch_read_control(st.ctl, S, b->prm.solver, b->iters.data(), b->stat.data(), b->scene_done.data(), &all);
+    *done = all != 0;
+    return REYN_OK;
+}
+
+template <class F> int capture(reyn_batch* b, hipGraphExec_t* out, F&& launches)
+{
+    reyn_ctx* ctx = b->ctx;
+    const bool prof = ctx->prof;
+    ctx->prof = false; // no event records inside a captured graph
+    REYN_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
+    const int rc = launches();
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+    ctx->prof = prof;
+    if (rc != REYN_OK) {
+        if (graph)
+            (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    REYN_HIP(ctx, e);
+    const hipError_t ei = hipGraphInstantiate(out, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    REYN_HIP(ctx, ei);
+    return REYN_OK;
+}
+
+uint32_t chunk_of(const reyn_batch* b) { return b->prm.check_every ? b->prm.check_every : 4u; }
+
+// Captures the three graphs for both state buffers (after a first step run
+// directly, which sized every buffer).
+int build_graphs(reyn_batch* b)
+{
+    const int cur0 = b->cur;
+    for (int k = 0; k < 2; ++k) {
+        b->cur = k;
+        REYN_TRY(capture(b, &b->g_begin[k], [&] { return launch_begin(b, nullptr); }));
+        // launch_begin flipped cur: the chunk and end graphs of the buffer it
+        // leaves the state in
+        REYN_TRY(capture(b, &b->g_chunk[b->cur], [&] { return launch_chunk(b, chunk_of(b)); }));
+        REYN_TRY(capture(b, &b->g_end[b->cur], [&] { return launch_end(b); }));
+    }
+    b->cur = cur0;
+    b->graphs = true;
+    return REYN_OK;
+}
+
+int step(reyn_batch* b, bool use_graph)
+{
+    reyn_ctx* ctx = b->ctx;
+    const uint32_t chunk = chunk_of(b);
+    const uint32_t min_it = std::max(b->prm.min_iter, 1u);
+    const uint32_t cap = b->prm.max_iter ? b->prm.max_iter : 100000u;
+    if (use_graph) {
+        REYN_HIP(ctx, hipGraphLaunch(b->g_begin[b->cur], ctx->stream));
+        b->cur = 1 - b->cur;
+    } else {
+        REYN_TRY(launch_begin(b, nullptr));
+    }
+    bool done = false;
+    REYN_TRY(fetch(b, true, &done));
+    uint32_t launched = min_it;
+    while (!done) {
+        if (launched >= cap)
+            return set_error(ctx, REYN_ERR_NOT_CONVERGED, "batch: a scene did not converge in %u iterations", launched);
+        const bool all_active = std::find(b->scene_done.begin(), b->scene_done.end(), 1) == b->scene_done.end();
+        if (use_graph && all_active)
+            REYN_HIP(ctx, hipGraphLaunch(b->g_chunk[b->cur], ctx->stream));
+        else
+            REYN_TRY(launch_chunk_active(b, chunk));
+        launched += chunk;
+        REYN_TRY(fetch(b, false, &done));
+    }
+    if (use_graph)
+        REYN_HIP(ctx, hipGraphLaunch(b->g_end[b->cur], ctx->stream));
+    else
+        REYN_TRY(launch_end(b));
+    const Staging st = staging(b);
+    for (uint32_t s = 0; s < b->S; ++s) {
         reyn_batch_scene_info& in = b->info[s];
         in.steps += 1;
         in.iterations = b->iters[s];
-        in.dt = dt_host[s];
-        in.time += double(dt_host[s]);
+        in.dt = st.dt[s];
+        in.time += double(st.dt[s]);
         in.loop_stat = b->stat[s];
     }
-    // the pinned iteration counts are read by the upload above
-    REYN_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return REYN_OK;
 }
 
@@ -417,6 +595,10 @@ int reyn_batch_destroy(reyn_batch* b)
         (void)hipSetDevice(b->ctx->device);
         (void)hipStreamSynchronize(b->ctx->stream);
     }
+    for (int k = 0; k < 2; ++k)
+        for (hipGraphExec_t g : { b->g_begin[k], b->g_chunk[k], b->g_end[k] })
+            if (g)
+                (void)hipGraphExecDestroy(g);
     for (reyn_grid_builder* g : b->bgrid)
         reyn_grid_builder_destroy(g);
     reyn_grid_builder_destroy(b->fgrid);
@@ -424,6 +606,8 @@ int reyn_batch_destroy(reyn_batch* b)
         (void)hipFree(p);
     if (b->pinned)
         (void)hipHostFree(b->pinned);
+    if (b->blk_list_host)
+        (void)hipHostFree(b->blk_list_host);
     delete b;
     return REYN_OK;
 }
@@ -433,8 +617,15 @@ int reyn_batch_step(reyn_batch* b, uint32_t steps)
     if (!b)
         return REYN_ERR_INVALID_ARGUMENT;
     REYN_HIP(b->ctx, hipSetDevice(b->ctx->device));
-    for (uint32_t k = 0; k < steps; ++k)
-        REYN_TRY(step(b));
+    for (uint32_t k = 0; k < steps; ++k) {
+        // the first step runs directly (it sizes every buffer), then the
+        // captured graphs take over when asked for
+        const bool use = b->prm.graph && b->info[0].steps > 0;
+        if (use && !b->graphs)
+            REYN_TRY(build_graphs(b));
+        REYN_TRY(step(b, use));
+    }
+    REYN_HIP(b->ctx, hipStreamSynchronize(b->ctx->stream));
     return REYN_OK;
 }
 

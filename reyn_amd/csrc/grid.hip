@@ -375,7 +375,8 @@ int sort_points(reyn_grid_builder* b, uint32_t n, uint32_t* status_host, Count&&
         }
         REYN_LAUNCHED(ctx, "k_stabilize");
     }
-    REYN_HIP(ctx, hipMemcpyAsync(status_host, b->status, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    if (status_host)
+        REYN_HIP(ctx, hipMemcpyAsync(status_host, b->status, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     return REYN_OK;
 }
 

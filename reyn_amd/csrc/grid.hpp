@@ -34,6 +34,8 @@ int builder_alloc(reyn_ctx* ctx, size_t ncells, float search_radius, reyn_grid_b
 
 // Counting sort of the n points of a scene batch over the builder's cells:
 // point t of scene s gets the key cell_off[s] + its cell in scene s's grid.
+// The build status is copied to `status_host` unless it is NULL (then it
+// stays in b->status).
 int sort_batch(reyn_grid_builder* b, const SceneArg& sa, uint32_t nblocks, const uint32_t* cell_off,
     const float* xx, const float* xy, const float* xz, uint32_t n, uint32_t* status_host);
 

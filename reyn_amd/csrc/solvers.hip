@@ -41,6 +41,7 @@ struct Scratch {
     // packed records (NULL when the step has none):
     float4* P4; // {x, y, z, m}: walked and summed by the density pass
     float4* X; // interleaved records of the later loops, see `aux4`
+    float4* B4; // packed boundary records (NULL: the boundary is walked in SoA)
 };
 
 // Neighbours kept per particle; a fluid particle at rest density has about
@@ -67,12 +68,8 @@ constexpr uint32_t kListCap = 64;
 #endif
 constexpr uint32_t kRing = REYN_LIST_RING, kFlush = 4;
 
-// lists: the solver-step scratch (packed records, and the neighbour list
-// while its entries fit 32-bit byte offsets); otherwise only the boundary
-// sums and the loop control. `nblocks` workgroups (0: one scene of n) and
-// `nscenes` loop-control blocks.
-int get_scratch(reyn_ctx* ctx, uint32_t n, Scratch* s, bool lists = false, uint32_t nblocks = 0,
-    uint32_t nscenes = 1)
+size_t carve_scratch(char* p, uint32_t n, Scratch* s, bool lists, uint32_t nblocks, uint32_t nscenes,
+    uint32_t nbdy)
 {
     const size_t nb = nblocks ? nblocks : blocks_for(n);
     auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
@@ -83,8 +80,10 @@ int get_scratch(reyn_ctx* ctx, uint32_t n, Scratch* s, bool lists = false, uint3
     const bool keep_list = records && size_t(n) * kListCap * 4 < (size_t(1) << 32);
     const size_t rec_bytes = records ? al(size_t(n) * 16) + al(size_t(n) * 32) : 0;
     const size_t list_bytes = keep_list ? f + al(size_t(n) * kListCap * 4) : 0;
-    char* p = nullptr;
-    REYN_TRY(ctx_scratch(ctx, head + rec_bytes + list_bytes, reinterpret_cast<void**>(&p)));
+    const size_t bdy_bytes = (records && nbdy) ? al(size_t(nbdy) * 16) : 0;
+    const size_t total = head + rec_bytes + list_bytes + bdy_bytes;
+    if (!p)
+        return total;
     s->gbx = reinterpret_cast<float*>(p);
     s->gby = reinterpret_cast<float*>(p + f);
     s->gbz = reinterpret_cast<float*>(p + 2 * f);
@@ -102,6 +101,21 @@ int get_scratch(reyn_ctx* ctx, uint32_t n, Scratch* s, bool lists = false, uint3
         s->L.cnt = reinterpret_cast<uint32_t*>(p + head + rec_bytes);
         s->L.idx = reinterpret_cast<uint32_t*>(p + head + rec_bytes + f);
     }
+    s->B4 = bdy_bytes ? reinterpret_cast<float4*>(p + head + rec_bytes + list_bytes) : nullptr;
+    return total;
+}
+
+// lists: the solver-step scratch (packed records, and the neighbour list
+// while its entries fit 32-bit byte offsets); otherwise only the boundary
+// sums and the loop control. `nblocks` workgroups (0: one scene of n),
+// `nscenes` loop-control blocks, `nbdy` boundary samples to pack. Carved out
+// of the context scratch.
+int get_scratch(reyn_ctx* ctx, uint32_t n, Scratch* s, bool lists = false, uint32_t nblocks = 0,
+    uint32_t nscenes = 1, uint32_t nbdy = 0)
+{
+    char* p = nullptr;
+    REYN_TRY(ctx_scratch(ctx, carve_scratch(nullptr, n, s, lists, nblocks, nscenes, nbdy), reinterpret_cast<void**>(&p)));
+    carve_scratch(p, n, s, lists, nblocks, nscenes, nbdy);
     return REYN_OK;
 }
 
@@ -137,6 +151,14 @@ __device__ __forceinline__ void put_auxw(float4* __restrict__ X, uint32_t i, flo
 }
 
 // ---- density ------------------------------------------------------------------
+
+// Packed boundary records {x, y, z, m}.
+__global__ __launch_bounds__(kBlock) void k_pack_bdy(BdyDev b, float4* __restrict__ B4)
+{
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < b.n)
+        B4[k] = make_float4(b.x[k], b.y[k], b.z[k], b.m[k]);
+}
 
 // P4[i] = {x_i, y_i, z_i, m_i}, the records the density pass walks, and the
 // interleaved records X of the later loops with u = v_i (see `aux4`).
@@ -242,12 +264,15 @@ __global__ __launch_bounds__(kBlock) REYN_OCC(REYN_OCC_DENSITY) void k_density(S
     ksum accb;
     ksum3 grad;
     if (b.present) {
-        walk<false>(b.grid, xi, b.x, b.y, b.z, [&](uint32_t k, f3 xik, float d2) {
-            const float mk = b.m[k];
+        auto visit = [&](f3 xik, float d2, float mk) {
             accb.add(mk * kW<K>(kp, d2));
             if (GB)
                 grad.add(mk * kdW<K>(kp, xik, d2));
-        });
+        };
+        if (b.p4)
+            walk4<false>(b.grid, xi, b.p4, [&](uint32_t, f3 xik, float d2, float mk) { visit(xik, d2, mk); });
+        else
+            walk<false>(b.grid, xi, b.x, b.y, b.z, [&](uint32_t k, f3 xik, float d2) { visit(xik, d2, b.m[k]); });
     }
     const float rbi = accb.value();
     const float r = acc.value() + rbi;
@@ -715,8 +740,19 @@ template <bool BT, bool IND, bool PK> constexpr bool kVariant = !BT || (!IND && 
 int no_variant(reyn_ctx* ctx) { return set_error(ctx, REYN_ERR_INVALID_ARGUMENT, "unsupported batch layout"); }
 
 template <bool BT, bool GB>
-int launch_density(reyn_ctx* ctx, int kernel, const Common& c, float* rho, const Scratch* s)
+int launch_density(reyn_ctx* ctx, int kernel, const Common& c0, float* rho, const Scratch* s)
 {
+    // one scene: its boundary is packed into scratch for this pass (a batch
+    // packs its boundaries once)
+    Common c = c0;
+    if (!BT && s && s->B4 && c.a.one.b.present && !c.a.one.b.p4) {
+        {
+            ProfScope _prof(ctx, "k_pack_bdy");
+            k_pack_bdy<<<blocks_for(c.a.one.b.n), kBlock, 0, ctx->stream>>>(c.a.one.b, s->B4);
+        }
+        REYN_LAUNCHED(ctx, "k_pack_bdy");
+        c.a.one.b.p4 = s->B4;
+    }
     if (s && s->P4) {
         {
             ProfScope _prof(ctx, "k_pack");
@@ -847,48 +883,67 @@ int sesph_run(reyn_ctx* ctx, int kernel, const Common& c, float k, const reyn_se
 }
 
 template <bool BT>
+int pcisph_begin(reyn_ctx* ctx, int kernel, const Common& c, const reyn_pcisph_work* work, const Scratch& s)
+{
+    REYN_TRY((launch_density<BT, true>(ctx, kernel, c, work->rho, &s)));
+    return launch_nonpressure<BT>(ctx, kernel, c, work->rho, work->ax, work->ay, work->az, s.L, s.X);
+}
+
+// One loop iteration: velocities with the current accelerations, predicted
+// densities, pressure accelerations, then the loop control of every scene.
+template <bool BT>
+int pcisph_iteration(reyn_ctx* ctx, int kernel, const Common& c, const reyn_pcisph_work* work, const Scratch& s,
+    uint32_t nscenes, uint32_t min_iter, uint32_t max_iter)
+{
+    REYN_TRY(launch_integrate<BT>(ctx, c, work->ax, work->ay, work->az, s.ctl, 0, s.X));
+    REYN_TRY(with_kernel(ctx, kernel, c.ind, [&](auto KC, auto IC) -> int {
+        return with_pk(s.X, [&](auto PC) -> int {
+            constexpr int K = decltype(KC)::value;
+            constexpr bool I = decltype(IC)::value;
+            constexpr bool PK = decltype(PC)::value;
+            if constexpr (!kVariant<BT, I, PK>) {
+                return no_variant(ctx);
+            } else {
+                {
+                    ProfScope _prof(ctx, "k_predict_rho");
+                    k_predict_rho<K, I, PK, BT><<<c.nb, kBlock, 0, ctx->stream>>>(c.a, c.f, s.gbx, s.gby, s.gbz,
+                        s.rb, work->rho, s.part_d, s.ctl, s.L, s.X);
+                }
+                REYN_LAUNCHED(ctx, "k_predict_rho");
+                {
+                    ProfScope _prof(ctx, "k_pcisph_prs_acc");
+                    k_pcisph_prs_acc<K, I, PK, BT><<<c.nb, kBlock, 0, ctx->stream>>>(c.a, c.f, work->rho, s.gbx,
+                        s.gby, s.gbz, work->ax, work->ay, work->az, s.ctl, s.L, s.X);
+                }
+                REYN_LAUNCHED(ctx, "k_pcisph_prs_acc");
+                return REYN_OK;
+            }
+        });
+    }));
+    {
+        ProfScope _prof(ctx, "k_ctl_pcisph");
+        k_ctl_pcisph<BT><<<nscenes, kBlock, 0, ctx->stream>>>(c.a, s.ctl, s.part_d, min_iter, max_iter);
+    }
+    REYN_LAUNCHED(ctx, "k_ctl_pcisph");
+    return REYN_OK;
+}
+
+template <bool BT>
+int pcisph_end(reyn_ctx* ctx, const Common& c, const reyn_pcisph_work* work)
+{
+    return launch_integrate<BT>(ctx, c, work->ax, work->ay, work->az, nullptr, 1);
+}
+
+template <bool BT>
 int pcisph_run(reyn_ctx* ctx, int kernel, const Common& c, const reyn_pcisph_work* work, const Scratch& s,
     uint32_t nscenes, uint32_t min_iter, uint32_t max_iter, SolveCtl* res)
 {
-    REYN_TRY((launch_density<BT, true>(ctx, kernel, c, work->rho, &s)));
-    REYN_TRY(launch_nonpressure<BT>(ctx, kernel, c, work->rho, work->ax, work->ay, work->az, s.L, s.X));
+    REYN_TRY(pcisph_begin<BT>(ctx, kernel, c, work, s));
     REYN_TRY(run_loop(
         ctx, s.ctl, nscenes, min_iter, max_iter, 4,
-        [&]() -> int {
-            REYN_TRY(launch_integrate<BT>(ctx, c, work->ax, work->ay, work->az, s.ctl, 0, s.X));
-            REYN_TRY(with_kernel(ctx, kernel, c.ind, [&](auto KC, auto IC) -> int {
-                return with_pk(s.X, [&](auto PC) -> int {
-                    constexpr int K = decltype(KC)::value;
-                    constexpr bool I = decltype(IC)::value;
-                    constexpr bool PK = decltype(PC)::value;
-                    if constexpr (!kVariant<BT, I, PK>) {
-                        return no_variant(ctx);
-                    } else {
-                        {
-                            ProfScope _prof(ctx, "k_predict_rho");
-                            k_predict_rho<K, I, PK, BT><<<c.nb, kBlock, 0, ctx->stream>>>(c.a, c.f, s.gbx, s.gby,
-                                s.gbz, s.rb, work->rho, s.part_d, s.ctl, s.L, s.X);
-                        }
-                        REYN_LAUNCHED(ctx, "k_predict_rho");
-                        {
-                            ProfScope _prof(ctx, "k_pcisph_prs_acc");
-                            k_pcisph_prs_acc<K, I, PK, BT><<<c.nb, kBlock, 0, ctx->stream>>>(c.a, c.f, work->rho,
-                                s.gbx, s.gby, s.gbz, work->ax, work->ay, work->az, s.ctl, s.L, s.X);
-                        }
-                        REYN_LAUNCHED(ctx, "k_pcisph_prs_acc");
-                        return REYN_OK;
-                    }
-                });
-            }));
-            {
-                ProfScope _prof(ctx, "k_ctl_pcisph");
-                k_ctl_pcisph<BT><<<nscenes, kBlock, 0, ctx->stream>>>(c.a, s.ctl, s.part_d, min_iter, max_iter);
-            }
-            REYN_LAUNCHED(ctx, "k_ctl_pcisph");
-            return REYN_OK;
-        },
+        [&]() -> int { return pcisph_iteration<BT>(ctx, kernel, c, work, s, nscenes, min_iter, max_iter); },
         res));
-    return launch_integrate<BT>(ctx, c, work->ax, work->ay, work->az, nullptr, 1);
+    return pcisph_end<BT>(ctx, c, work);
 }
 
 struct IisphLoop {
@@ -898,13 +953,37 @@ struct IisphLoop {
 };
 
 template <bool BT>
-int iisph_run(reyn_ctx* ctx, int kind, const Common& c, const IisphLoop& lp, const reyn_iisph_work* work,
-    const Scratch& s, uint32_t nscenes, SolveCtl* res)
+int iisph_pacc(reyn_ctx* ctx, int kind, const Common& c, const reyn_iisph_work* work, const Scratch& s,
+    const SolveCtl* ctl)
+{
+    return with_kernel(ctx, kind, c.ind, [&](auto KC, auto IC) -> int {
+        return with_pk(s.X, [&](auto PC) -> int {
+            constexpr int K = decltype(KC)::value;
+            constexpr bool I = decltype(IC)::value;
+            constexpr bool PK = decltype(PC)::value;
+            if constexpr (!kVariant<BT, I, PK>) {
+                return no_variant(ctx);
+            } else {
+                {
+                    ProfScope _prof(ctx, "k_iisph_pacc");
+                    k_iisph_pacc<K, I, PK, BT><<<c.nb, kBlock, 0, ctx->stream>>>(c.a, c.f, work->p, s.gbx, s.gby,
+                        s.gbz, work->ax, work->ay, work->az, ctl, s.L, s.P4, s.X);
+                }
+                REYN_LAUNCHED(ctx, "k_iisph_pacc");
+                return REYN_OK;
+            }
+        });
+    });
+}
+
+template <bool BT>
+int iisph_begin(reyn_ctx* ctx, int kind, const Common& c, const IisphLoop& lp, const reyn_iisph_work* work,
+    const Scratch& s)
 {
     REYN_TRY((launch_density<BT, true>(ctx, kind, c, work->rho, &s)));
     REYN_TRY(launch_nonpressure<BT>(ctx, kind, c, work->rho, work->ax, work->ay, work->az, s.L, s.X));
     REYN_TRY(launch_integrate<BT>(ctx, c, work->ax, work->ay, work->az, nullptr, 0, s.X));
-    REYN_TRY(with_kernel(ctx, kind, c.ind, [&](auto KC, auto IC) -> int {
+    return with_kernel(ctx, kind, c.ind, [&](auto KC, auto IC) -> int {
         return with_pk(s.X, [&](auto PC) -> int {
             constexpr int K = decltype(KC)::value;
             constexpr bool I = decltype(IC)::value;
@@ -921,92 +1000,156 @@ int iisph_run(reyn_ctx* ctx, int kind, const Common& c, const IisphLoop& lp, con
                 return REYN_OK;
             }
         });
-    }));
-    auto pacc = [&](const SolveCtl* ctl) -> int {
-        return with_kernel(ctx, kind, c.ind, [&](auto KC, auto IC) -> int {
-            return with_pk(s.X, [&](auto PC) -> int {
-                constexpr int K = decltype(KC)::value;
-                constexpr bool I = decltype(IC)::value;
-                constexpr bool PK = decltype(PC)::value;
-                if constexpr (!kVariant<BT, I, PK>) {
-                    return no_variant(ctx);
-                } else {
-                    {
-                        ProfScope _prof(ctx, "k_iisph_pacc");
-                        k_iisph_pacc<K, I, PK, BT><<<c.nb, kBlock, 0, ctx->stream>>>(c.a, c.f, work->p, s.gbx,
-                            s.gby, s.gbz, work->ax, work->ay, work->az, ctl, s.L, s.P4, s.X);
-                    }
-                    REYN_LAUNCHED(ctx, "k_iisph_pacc");
-                    return REYN_OK;
+    });
+}
+
+// One loop iteration: pressure accelerations, the relaxed Jacobi update, then
+// the loop control of every scene.
+template <bool BT>
+int iisph_iteration(reyn_ctx* ctx, int kind, const Common& c, const IisphLoop& lp, const reyn_iisph_work* work,
+    const Scratch& s, uint32_t nscenes)
+{
+    REYN_TRY(iisph_pacc<BT>(ctx, kind, c, work, s, s.ctl));
+    REYN_TRY(with_kernel(ctx, kind, c.ind, [&](auto KC, auto IC) -> int {
+        return with_pk(s.X, [&](auto PC) -> int {
+            constexpr int K = decltype(KC)::value;
+            constexpr bool I = decltype(IC)::value;
+            constexpr bool PK = decltype(PC)::value;
+            if constexpr (!kVariant<BT, I, PK>) {
+                return no_variant(ctx);
+            } else {
+                {
+                    ProfScope _prof(ctx, "k_iisph_jacobi");
+                    k_iisph_jacobi<K, I, PK, BT><<<c.nb, kBlock, 0, ctx->stream>>>(c.a, c.f, lp.omega, work->ax,
+                        work->ay, work->az, work->a_ii, work->s_i, s.gbx, s.gby, s.gbz, work->p, s.part_d,
+                        s.part_u, s.ctl, s.L, s.X);
                 }
-            });
+                REYN_LAUNCHED(ctx, "k_iisph_jacobi");
+                return REYN_OK;
+            }
         });
-    };
+    }));
+    {
+        ProfScope _prof(ctx, "k_ctl_iisph");
+        k_ctl_iisph<BT><<<nscenes, kBlock, 0, ctx->stream>>>(c.a, s.ctl, s.part_d, s.part_u, lp.eta, lp.min_iter,
+            lp.max_iter, lp.rule);
+    }
+    REYN_LAUNCHED(ctx, "k_ctl_iisph");
+    return REYN_OK;
+}
+
+template <bool BT>
+int iisph_end(reyn_ctx* ctx, int kind, const Common& c, const reyn_iisph_work* work, const Scratch& s)
+{
+    REYN_TRY(iisph_pacc<BT>(ctx, kind, c, work, s, nullptr));
+    return launch_integrate<BT>(ctx, c, work->ax, work->ay, work->az, nullptr, 1);
+}
+
+template <bool BT>
+int iisph_run(reyn_ctx* ctx, int kind, const Common& c, const IisphLoop& lp, const reyn_iisph_work* work,
+    const Scratch& s, uint32_t nscenes, SolveCtl* res)
+{
+    REYN_TRY(iisph_begin<BT>(ctx, kind, c, lp, work, s));
     REYN_TRY(run_loop(
         ctx, s.ctl, nscenes, lp.min_iter, lp.max_iter, lp.check_every,
-        [&]() -> int {
-            REYN_TRY(pacc(s.ctl));
-            REYN_TRY(with_kernel(ctx, kind, c.ind, [&](auto KC, auto IC) -> int {
-                return with_pk(s.X, [&](auto PC) -> int {
-                    constexpr int K = decltype(KC)::value;
-                    constexpr bool I = decltype(IC)::value;
-                    constexpr bool PK = decltype(PC)::value;
-                    if constexpr (!kVariant<BT, I, PK>) {
-                        return no_variant(ctx);
-                    } else {
-                        {
-                            ProfScope _prof(ctx, "k_iisph_jacobi");
-                            k_iisph_jacobi<K, I, PK, BT><<<c.nb, kBlock, 0, ctx->stream>>>(c.a, c.f, lp.omega,
-                                work->ax, work->ay, work->az, work->a_ii, work->s_i, s.gbx, s.gby, s.gbz, work->p,
-                                s.part_d, s.part_u, s.ctl, s.L, s.X);
-                        }
-                        REYN_LAUNCHED(ctx, "k_iisph_jacobi");
-                        return REYN_OK;
-                    }
-                });
-            }));
-            {
-                ProfScope _prof(ctx, "k_ctl_iisph");
-                k_ctl_iisph<BT><<<nscenes, kBlock, 0, ctx->stream>>>(c.a, s.ctl, s.part_d, s.part_u, lp.eta,
-                    lp.min_iter, lp.max_iter, lp.rule);
-            }
-            REYN_LAUNCHED(ctx, "k_ctl_iisph");
-            return REYN_OK;
-        },
-        res));
-    REYN_TRY(pacc(nullptr));
-    return launch_integrate<BT>(ctx, c, work->ax, work->ay, work->az, nullptr, 1);
+        [&]() -> int { return iisph_iteration<BT>(ctx, kind, c, lp, work, s, nscenes); }, res));
+    return iisph_end<BT>(ctx, kind, c, work, s);
+}
+
+// The device state of a batch as a launch target with its own scratch.
+void batch_common(const BatchTarget& bt, Common* c, Scratch* s)
+{
+    c->f = bt.f;
+    c->a = bt.a;
+    c->ind = false;
+    c->nb = bt.a.blk_list ? bt.nlaunch : bt.nblocks;
+    carve_scratch(static_cast<char*>(bt.scratch), bt.f.n, s, true, bt.nblocks, bt.nscenes, 0);
 }
 
 } // namespace
 
-int batch_solve(reyn_ctx* ctx, const BatchTarget& bt, const BatchSolver& bs, uint32_t* iters, double* errs)
+size_t batch_scratch_bytes(uint32_t n, uint32_t nblocks, uint32_t nscenes)
+{
+    Scratch s;
+    return carve_scratch(nullptr, n, &s, true, nblocks, nscenes, 0);
+}
+
+const void* batch_loop_control(const BatchTarget& bt)
 {
     Common c;
-    c.f = bt.f;
-    c.a = bt.a;
-    c.ind = false;
-    c.nb = bt.nblocks;
     Scratch s;
-    REYN_TRY(get_scratch(ctx, bt.f.n, &s, true, bt.nblocks, bt.nscenes));
+    batch_common(bt, &c, &s);
+    return s.ctl;
+}
+
+size_t batch_loop_control_bytes(uint32_t nscenes) { return sizeof(SolveCtl) * nscenes; }
+
+namespace {
+__global__ void k_save_iterations(const SolveCtl* __restrict__ ctl, uint32_t* __restrict__ out, uint32_t n)
+{
+    for (uint32_t k = threadIdx.x; k < n; k += blockDim.x)
+        out[k] = ctl[k].iter;
+}
+} // namespace
+
+int batch_save_iterations(reyn_ctx* ctx, const BatchTarget& bt, uint32_t* iters_dev)
+{
+    Common c;
+    Scratch s;
+    batch_common(bt, &c, &s);
+    {
+        ProfScope _prof(ctx, "k_save_iterations");
+        k_save_iterations<<<1, kBlock, 0, ctx->stream>>>(s.ctl, iters_dev, bt.nscenes);
+    }
+    REYN_LAUNCHED(ctx, "k_save_iterations");
+    return REYN_OK;
+}
+
+void batch_read_control(const void* host, uint32_t nscenes, int solver, uint32_t* iters, double* stats,
+    uint8_t* scene_done, int* done)
+{
+    const SolveCtl* c = static_cast<const SolveCtl*>(host);
+    *done = 1;
+    for (uint32_t k = 0; k < nscenes; ++k) {
+        iters[k] = c[k].iter;
+        stats[k] = solver == REYN_SOLVER_PCISPH ? c[k].avg_density : c[k].mean_rel_error;
+        scene_done[k] = c[k].done ? 1 : 0;
+        *done = *done && c[k].done;
+    }
+}
+
+int batch_phase(reyn_ctx* ctx, const BatchTarget& bt, const BatchSolver& bs, int phase, uint32_t count)
+{
+    Common c;
+    Scratch s;
+    batch_common(bt, &c, &s);
     if (!s.X)
         return set_error(ctx, REYN_ERR_CAPACITY, "batch: %u particles exceed the packed-record limit", bt.f.n);
-    std::vector<SolveCtl> res(bt.nscenes);
-    if (bs.solver == REYN_SOLVER_PCISPH) {
-        const reyn_pcisph_work w { bt.rho, bt.ax, bt.ay, bt.az };
-        REYN_TRY(pcisph_run<true>(ctx, bs.kernel, c, &w, s, bt.nscenes, bs.min_iter, bs.max_iter, res.data()));
-    } else if (bs.solver == REYN_SOLVER_IISPH) {
-        const reyn_iisph_work w { bt.rho, bt.p, bt.a_ii, bt.s_i, bt.ax, bt.ay, bt.az };
-        const IisphLoop lp { bs.omega, bs.eta, bs.min_iter, bs.max_iter, bs.check_every, bs.stop_rule, bs.warm };
-        REYN_TRY(iisph_run<true>(ctx, bs.kernel, c, lp, &w, s, bt.nscenes, res.data()));
-    } else {
+    const IisphLoop lp { bs.omega, bs.eta, bs.min_iter, bs.max_iter, bs.check_every, bs.stop_rule, bs.warm };
+    const reyn_pcisph_work pw { bt.rho, bt.ax, bt.ay, bt.az };
+    const reyn_iisph_work iw { bt.rho, bt.p, bt.a_ii, bt.s_i, bt.ax, bt.ay, bt.az };
+    const bool pc = bs.solver == REYN_SOLVER_PCISPH;
+    if (!pc && bs.solver != REYN_SOLVER_IISPH)
         return set_error(ctx, REYN_ERR_INVALID_ARGUMENT, "batch: unknown solver %d", bs.solver);
+    switch (phase) {
+    case BATCH_BEGIN:
+        REYN_TRY(pc ? pcisph_begin<true>(ctx, bs.kernel, c, &pw, s) : iisph_begin<true>(ctx, bs.kernel, c, lp, &iw, s));
+        {
+            ProfScope _prof(ctx, "k_ctl_reset");
+            k_ctl_reset<<<1, kBlock, 0, ctx->stream>>>(s.ctl, bt.nscenes);
+        }
+        REYN_LAUNCHED(ctx, "k_ctl_reset");
+        [[fallthrough]];
+    case BATCH_ITERATE:
+        for (uint32_t k = 0; k < count; ++k)
+            REYN_TRY(pc ? pcisph_iteration<true>(ctx, bs.kernel, c, &pw, s, bt.nscenes, bs.min_iter, bs.max_iter)
+                        : iisph_iteration<true>(ctx, bs.kernel, c, lp, &iw, s, bt.nscenes));
+        return REYN_OK;
+    case BATCH_END:
+        return pc ? pcisph_end<true>(ctx, c, &pw) : iisph_end<true>(ctx, bs.kernel, c, &iw, s);
+    default:
+        return set_error(ctx, REYN_ERR_INVALID_ARGUMENT, "batch: unknown phase %d", phase);
     }
-    for (uint32_t k = 0; k < bt.nscenes; ++k) {
-        iters[k] = res[k].iter;
-        errs[k] = bs.solver == REYN_SOLVER_PCISPH ? res[k].avg_density : res[k].mean_rel_error;
-    }
-    return REYN_OK;
 }
 
 } // namespace reyn
@@ -1109,7 +1252,7 @@ int reyn_sesph_step(reyn_ctx* ctx, const reyn_sesph_params* prm, const reyn_part
         return set_error(ctx, REYN_ERR_INVALID_ARGUMENT, "sesph: NULL work buffer");
     c.a.one.dt = dt;
     Scratch s;
-    REYN_TRY(get_scratch(ctx, fluid->n, &s, true));
+    REYN_TRY(get_scratch(ctx, fluid->n, &s, true, 0, 1, c.a.one.b.present ? c.a.one.b.n : 0));
     return sesph_run<false>(ctx, prm->sph.kernel, c, prm->k, work, s);
 }
 
@@ -1191,7 +1334,7 @@ int reyn_pcisph_step(reyn_ctx* ctx, const reyn_pcisph_params* prm, const reyn_pa
     c.a.one.kk = static_cast<float>((prm->delta / dt) / dt);
     c.a.one.thr = (prm->eta_rho_max + 1.f) * prm->sph.rho_0;
     Scratch s;
-    REYN_TRY(get_scratch(ctx, fluid->n, &s, true));
+    REYN_TRY(get_scratch(ctx, fluid->n, &s, true, 0, 1, c.a.one.b.present ? c.a.one.b.n : 0));
     SolveCtl res;
     REYN_TRY(pcisph_run<false>(ctx, prm->sph.kernel, c, work, s, 1, prm->min_iter, prm->max_iter, &res));
     if (info) {
@@ -1222,7 +1365,7 @@ int reyn_iisph_step(reyn_ctx* ctx, const reyn_iisph_params* prm, const reyn_part
     c.a.one.dt = dt;
     c.a.one.thr = prm->eta_rho_max * prm->sph.rho_0;
     Scratch s;
-    REYN_TRY(get_scratch(ctx, fluid->n, &s, true));
+    REYN_TRY(get_scratch(ctx, fluid->n, &s, true, 0, 1, c.a.one.b.present ? c.a.one.b.n : 0));
     const IisphLoop lp { prm->omega, prm->eta_rho_max, prm->min_iter, prm->max_iter, prm->check_every,
         prm->stop_rule, prm->warm_start };
     SolveCtl res;

@@ -401,6 +401,7 @@ inline FluidDev to_dev(const reyn_particles& p)
 
 struct BdyDev {
     const float *x, *y, *z, *m;
+    const float4* p4; // packed {x, y, z, m} of the same samples, or NULL
     uint32_t n;
     GridDev grid;
     int present;
@@ -468,6 +469,7 @@ struct SceneArg {
     SceneDev one;
     const SceneDev* table; // batch: per scene; unused for one scene
     const uint32_t* blk_scene; // batch: the scene of every workgroup
+    const uint32_t* blk_list; // batch: the workgroups launched (NULL: all)
 };
 
 template <bool BT> __device__ __forceinline__ const SceneDev& scene_of(const SceneArg& a, uint32_t lb)
@@ -490,7 +492,8 @@ template <bool BT> __device__ __forceinline__ const SceneDev& scene_of_ctl(const
 
 // The workgroup's scene, the thread's particle slot and whether it holds one.
 #define REYN_SCENE(BT, sa)                                                                                 \
-    const uint32_t lb_ = xcd_block(blockIdx.x, gridDim.x);                                                 \
+    const uint32_t lb_ = (BT && (sa).blk_list) ? (sa).blk_list[xcd_block(blockIdx.x, gridDim.x)]             \
+                                               : xcd_block(blockIdx.x, gridDim.x);                           \
     const SceneDev& sc = scene_of<BT>((sa), lb_);                                                          \
     const uint32_t t = sc.first + (lb_ - sc.blk_first) * kBlock + threadIdx.x;                            \
     const bool live = t < sc.first + sc.n

@@ -376,7 +376,8 @@ class Batch:
                  cell_ratio: float = 0.5, dt0: float = 0.0005, cfl_lambda: float = 1.0,
                  change_factor: float = 0.001, iter_target: int = 10, eta_rho_max: float | None = None,
                  omega: float = 0.5, min_iter: int = 3, max_iter: int = 100,
-                 stop_rule: int = capi.IISPH_STOP_MEAN, warm_start: bool = True, g=GRAVITY):
+                 stop_rule: int = capi.IISPH_STOP_MEAN, warm_start: bool = True, g=GRAVITY, graph: bool = False,
+                 check_every: int = 4):
         self.ctx = ctx
         self.h = None
         self.scenes = list(scenes)
@@ -387,7 +388,8 @@ class Batch:
         self.prm = capi.BatchParams(solver_id, k, float(nu), float(self.scenes[0].rho_0), capi.f3(g),
                                     float(cell_ratio), float(dt0), float(cfl_lambda), float(change_factor),
                                     int(iter_target), float(eta_rho_max), float(omega), int(min_iter),
-                                    int(max_iter), int(stop_rule), int(bool(warm_start)))
+                                    int(max_iter), int(stop_rule), int(bool(warm_start)), int(bool(graph)),
+                                    int(check_every))
         keep = []
 
         def ptr(a):

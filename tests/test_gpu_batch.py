@@ -61,6 +61,25 @@ def test_batch_of_one_equals_single_scene_path(ctx):
     b.close()
 
 
+@pytest.mark.parametrize("solver,kernel", [("pcisph", "W6"), ("iisph", "B3")])
+def test_graph_replays_match_direct_launches(ctx, solver, kernel):
+    """The captured step (graphs for both state buffers, replayed every step
+    with host checks in between) against the same batch launched directly,
+    compared after every step."""
+    scs = small_scenes(count=3, seed=5)
+    direct = Batch(ctx, scs, solver=solver, kernel=kernel, graph=False)
+    graphed = Batch(ctx, scs, solver=solver, kernel=kernel, graph=True)
+    for k in range(9):
+        direct.step(1)
+        graphed.step(1)
+        for s in range(len(scs)):
+            assert_same(graphed.download(s), direct.download(s), f"step {k} scene {s}")
+            a, b = graphed.info(s), direct.info(s)
+            assert (a.iterations, a.dt) == (b.iterations, b.dt), (k, s)
+    direct.close()
+    graphed.close()
+
+
 def test_batch_rejects_bad_input(ctx):
     sc = small_scenes(count=1)[0]
     with pytest.raises(capi.ReynError):
