@@ -5,15 +5,26 @@ Workload: a 128^3 fluid block [0,1]^3 at spacing s = 1/128 in the box
 [0,2]x[0,1.25]x[0,2] with 2 wall layers (~600k boundary samples) and a seeded
 sphere obstacle; cubic B-spline kernel with support 2s, rho_0 = 1000,
 nu = 0.001, relaxed Jacobi omega = 0.5, stop at mean compression error
-<= 0.1% or 100 iterations, CFL time step lambda = 0.4. One step is the
-reference's per-step loop (src/main.cu:60-78): CFL time step, grid build and
-reorder of every particle field (and the warm-start pressure), the IISPH
-step, bounds enforcement.
+<= 0.1% or 100 iterations, the reference main loop's time-step controller
+(lambda 1, 10 target iterations, 0.1% change per step, from dt0 = 5e-4). One
+step is the reference's per-step loop (src/main.cu:60-78): time step, grid
+build and reorder of every particle field (and the warm-start pressure), the
+IISPH step, bounds enforcement.
 
 Prints one JSON line with particle-steps/s, the roofline of the dominant
 kernel (algorithmic bytes per launch / its average HIP-event duration, timed
 live on the launch stream over the timed steps) and the CPU baseline (the
 oracle's FP32 OpenMP port of the same step, on the same state, on this host).
+Two more objects in the same line:
+
+- "batch": 64 PCISPH scenes of 10k-300k particles (seeded) advanced in one
+  batch with the autotuned cell ratio and check interval, against the same
+  64 scenes solved one at a time through the single-scene entry points.
+- "serving": 32 seeded IISPH jobs (5k-500k particles, 200-2000 steps) with
+  Poisson arrivals at 80% of the measured batch capacity, served by
+  continuous batching under the HBM budget, against one job at a time in
+  arrival order (FIFO); p50/p99 time-to-result, particle-steps/s, and whether
+  every served result equals its alone result bit for bit.
 """
 
 from __future__ import annotations
@@ -137,6 +148,108 @@ def cpu_baseline(sc, host_state, p, dt_steps, args, log) -> dict | None:
                       f"{wall:.2f} s wall"}
 
 
+def batch_leg(ctx, args, log) -> dict:
+    """64 PCISPH scenes in one batch (autotuned) against 64 sequential plain
+    solves (the reference's main loop per scene through the single-scene
+    entry points); both timed over `batch_steps` steps after 2 untimed ones,
+    set-up excluded."""
+    from reyn_amd import scenes
+    from reyn_amd.sim import PCISPH, Batch, BoundarySamples, Particles, Simulation
+    from reyn_amd.tune import autotune_batch
+
+    scs = scenes.config4_batch()
+    n = sum(sc.n for sc in scs)
+    opts = dict(solver="pcisph", kernel="W6", eta_rho_max=0.001, min_iter=3, max_iter=100)
+    best, trials = autotune_batch(ctx, scs, log=log, **opts)
+    b = Batch(ctx, scs, **opts, **best)
+    b.step(2)
+    ctx.synchronize()
+    t = time.perf_counter()
+    b.step(args.batch_steps)
+    ctx.synchronize()
+    t_batch = time.perf_counter() - t
+    it_b = [b.info(k).iterations for k in range(len(scs))]
+    b.close()
+    t_seq = 0.0
+    for sc in scs:
+        h = sc.spacing
+        state = Particles(ctx, sc.x, None, sc.mass, sc.rho_0)
+        bdy = BoundarySamples(ctx, sc.xb, h, sc.rho_0)
+        solver = PCISPH(ctx, "W6", sc.n, 0.001, h, sc.rho_0, eta_rho_max=0.001, min_iter=3, max_iter=100)
+        sim = Simulation(ctx, solver, state, bdy, h, sc.bound_min, sc.bound_max, cfl_lambda=1.0,
+                         dt_policy="controller", dt0=0.0005, iter_target=10, change_factor=0.001)
+        sim.step()
+        sim.step()
+        ctx.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.batch_steps):
+            sim.step()
+        ctx.synchronize()
+        t_seq += time.perf_counter() - t
+        for o in (solver, bdy, state, sim.builder):
+            getattr(o, "close", lambda: None)()
+    v_b = n * args.batch_steps / t_batch
+    v_s = n * args.batch_steps / t_seq
+    log(f"batch: {v_b / 1e6:.1f} M particle-steps/s batched, {v_s / 1e6:.1f} sequential plain, "
+        f"x{v_b / v_s:.2f}; iterations {min(it_b)}..{max(it_b)}")
+    return {
+        "workload": f"64 PCISPH dam-break scenes of 10k-300k particles (log-uniform, seed 4; {n} in total), "
+                    "W6 kernel, eta 0.1%, min 3 iterations, time-step controller, grid + reorder every step",
+        "particles": n, "steps": args.batch_steps,
+        "batched_particle_steps_per_s": round(v_b, 1),
+        "sequential_plain_particle_steps_per_s": round(v_s, 1),
+        "speedup": round(v_b / v_s, 3),
+        "batched_s": round(t_batch, 4), "sequential_s": round(t_seq, 4),
+        "last_iterations_range": [min(it_b), max(it_b)],
+        "autotune": {"chosen": best, "trials": [tr.as_dict() for tr in trials]},
+    }
+
+
+def serve_leg(ctx, args, log) -> dict:
+    """Continuous batching of seeded IISPH jobs against FIFO one at a time."""
+    from reyn_amd import scenes
+    from reyn_amd.serve import Server, make_jobs, poisson_arrivals, run_fifo, same_results
+    from reyn_amd.sim import Batch
+
+    spec = scenes.config5_jobs(args.serve_jobs)
+    opts = dict(solver="iisph", kernel="B3", rho_0=1000.0, eta_rho_max=0.001, omega=0.5, min_iter=3,
+                max_iter=100, cell_ratio=1.0, check_every=2)
+    # capacity: every job resident at once, 10 steps after 2
+    b = Batch(ctx, [sc for sc, _ in spec], **opts)
+    b.step(2)
+    ctx.synchronize()
+    t = time.perf_counter()
+    b.step(10)
+    ctx.synchronize()
+    cap = sum(sc.n for sc, _ in spec) * 10 / (time.perf_counter() - t)
+    b.close()
+    mean_work = float(np.mean([sc.n * st for sc, st in spec]))
+    rate = args.serve_load * cap / mean_work
+    arrivals = poisson_arrivals(len(spec), rate, seed=5)
+    log(f"serving: capacity {cap / 1e6:.1f} M particle-steps/s, {rate:.2f} arrivals/s")
+    served = make_jobs(spec, arrivals)
+    srv = Server(ctx, **opts)
+    rep = srv.run(served)
+    srv.close()
+    alone = make_jobs(spec, arrivals)
+    fifo = run_fifo(ctx, alone, **opts)
+    same, diff = same_results(served, alone)
+    log(f"serving: p50 {rep['p50_s']:.2f} s / p99 {rep['p99_s']:.2f} s served, p50 {fifo['p50_s']:.2f} s / "
+        f"p99 {fifo['p99_s']:.2f} s FIFO; {rep['particle_steps_per_s'] / 1e6:.1f} vs "
+        f"{fifo['particle_steps_per_s'] / 1e6:.1f} M particle-steps/s; {same}/{len(spec)} results identical")
+    return {
+        "workload": f"{len(spec)} IISPH dam-break jobs (seed 5): N log-uniform in [5k, 500k], box sides U[1,3], "
+                    "sphere obstacle with p 0.5, steps U{200..2000}, B3, stop at mean error <= 0.1% or 100 "
+                    f"iterations; Poisson arrivals at {args.serve_load:.0%} of the measured batch capacity",
+        "capacity_particle_steps_per_s": round(cap, 1), "arrival_rate_per_s": round(rate, 4),
+        "served": rep, "fifo_one_at_a_time": fifo,
+        "p50_speedup": round(fifo["p50_s"] / rep["p50_s"], 3),
+        "p99_speedup": round(fifo["p99_s"] / rep["p99_s"], 3),
+        "throughput_speedup": round(rep["particle_steps_per_s"] / fifo["particle_steps_per_s"], 3),
+        "results_identical": f"{same}/{len(spec)}", "jobs_differing": diff,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -146,6 +259,11 @@ def main() -> int:
     ap.add_argument("--cpu-steps", type=int, default=10,
                     help="IISPH steps of the CPU baseline sample (about 1 s each on 16 cores)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batch leg")
+    ap.add_argument("--batch-steps", type=int, default=8)
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--serve-jobs", type=int, default=32)
+    ap.add_argument("--serve-load", type=float, default=0.8)
     ap.add_argument("--no-profile", action="store_true", help="time without per-launch events")
     ap.add_argument("--dt-policy", choices=("controller", "cfl"), default="controller",
                     help="controller: the reference main loop's simple_dt_controller from dt0 = 5e-4 "
@@ -265,6 +383,10 @@ def main() -> int:
         sc.mb = bdy.m.numpy()
         sc.bdy_min, sc.bdy_max = bdy.bound_min, bdy.bound_max
         cpu = cpu_baseline(sc, host, solver.p.numpy(), sim.next_dt(), args, log)
+    for o in (solver, bdy, state, sim.builder):
+        getattr(o, "close", lambda: None)()
+    batch = None if args.no_batch else batch_leg(ctx, args, log)
+    serving = None if args.no_serve else serve_leg(ctx, args, log)
 
     out = {
         "metric": METRIC,
@@ -302,6 +424,8 @@ def main() -> int:
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "batch": batch,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
     ctx.close()

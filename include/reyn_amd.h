@@ -120,6 +120,8 @@ int reyn_ctx_synchronize(reyn_ctx* ctx);
 /* The context's hipStream_t, as an opaque pointer. */
 void* reyn_ctx_stream(reyn_ctx* ctx);
 int reyn_device_count(int* count);
+/* Free and total device memory of the context's GPU, in bytes. */
+int reyn_device_memory(reyn_ctx* ctx, uint64_t* free_bytes, uint64_t* total_bytes);
 int reyn_abi_version(void);
 
 /* Device memory helpers (replace `DeviceBuffer<T>` storage,
@@ -424,16 +426,31 @@ typedef struct reyn_batch_scene_info {
     double loop_stat;        /* PCISPH: mean predicted density; IISPH: mean error */
 } reyn_batch_scene_info;
 
-/* Uploads the scenes, builds their boundary grids and Akinci masses. */
+/* Uploads the scenes, builds their boundary grids and Akinci masses. The
+ * scenes get the ids 0 .. count-1; `count` may be 0 (an empty batch that
+ * scenes join later). */
 int reyn_batch_create(reyn_ctx* ctx, const reyn_batch_params* prm,
     const reyn_batch_scene* scenes, uint32_t count, reyn_batch** out);
 int reyn_batch_destroy(reyn_batch* b);
-/* Advances every scene by `steps` steps (time step, grid + reorder, solver
- * step, bounds). */
+/* Scenes join and leave between steps. Joining scenes start at rest state
+ * with the batch's dt0; resident scenes keep their state, time step and loop
+ * history, so a scene's result does not depend on which scenes shared its
+ * steps. `ids` (may be NULL) receives the new scenes' ids. */
+int reyn_batch_add(reyn_batch* b, const reyn_batch_scene* scenes, uint32_t count,
+    uint32_t* ids);
+int reyn_batch_remove(reyn_batch* b, const uint32_t* ids, uint32_t count);
+/* The ids of the resident scenes (up to `cap`) and their number. */
+int reyn_batch_scenes(const reyn_batch* b, uint32_t* ids, uint32_t cap, uint32_t* count);
+/* Device bytes a scene occupies while resident in a batch with `prm`: state,
+ * work arrays, solver scratch, grid tables and boundary. */
+int reyn_batch_scene_bytes(const reyn_batch_params* prm, const reyn_batch_scene* scene,
+    uint64_t* bytes);
+/* Advances every resident scene by `steps` steps (time step, grid + reorder,
+ * solver step, bounds). */
 int reyn_batch_step(reyn_batch* b, uint32_t steps);
 int reyn_batch_info(const reyn_batch* b, uint32_t scene, reyn_batch_scene_info* out);
 /* Copies a scene's state (in its current cell order) to host arrays of its n;
- * any pointer may be NULL. */
+ * any pointer may be NULL. `scene` is the scene's id. */
 int reyn_batch_download(reyn_batch* b, uint32_t scene, float* x, float* y, float* z,
     float* vx, float* vy, float* vz, float* rho, float* p);
 

@@ -27,6 +27,7 @@ KERNELS = {"C2": KERNEL_C2, "B3": KERNEL_B3, "W6": KERNEL_W6, "COS": KERNEL_COS}
 IISPH_STOP_ANY, IISPH_STOP_MEAN = 0, 1
 
 u32 = C.c_uint32
+u64 = C.c_uint64
 i32 = C.c_int32
 f32 = C.c_float
 f64 = C.c_double
@@ -205,7 +206,12 @@ _SIGS = {
     "reyn_reduce_f32": (i32, [vp, vp, u32, i32, P(f64)]),
     "reyn_batch_create": (i32, [vp, P(BatchParams), P(BatchScene), u32, P(vp)]),
     "reyn_batch_destroy": (i32, [vp]),
+    "reyn_batch_add": (i32, [vp, P(BatchScene), u32, P(u32)]),
+    "reyn_batch_remove": (i32, [vp, P(u32), u32]),
+    "reyn_batch_scenes": (i32, [vp, P(u32), u32, P(u32)]),
+    "reyn_batch_scene_bytes": (i32, [P(BatchParams), P(BatchScene), P(u64)]),
     "reyn_batch_step": (i32, [vp, u32]),
+    "reyn_device_memory": (i32, [vp, P(u64), P(u64)]),
     "reyn_batch_info": (i32, [vp, u32, P(BatchSceneInfo)]),
     "reyn_batch_download": (i32, [vp, u32, vp, vp, vp, vp, vp, vp, vp, vp]),
 }

@@ -157,6 +157,18 @@ int reyn_device_count(int* count)
     return REYN_OK;
 }
 
+int reyn_device_memory(reyn_ctx* ctx, uint64_t* free_bytes, uint64_t* total_bytes)
+{
+    if (!ctx || !free_bytes || !total_bytes)
+        return set_error(ctx, REYN_ERR_INVALID_ARGUMENT, "device memory: NULL argument");
+    REYN_HIP(ctx, hipSetDevice(ctx->device));
+    size_t f = 0, t = 0;
+    REYN_HIP(ctx, hipMemGetInfo(&f, &t));
+    *free_bytes = f;
+    *total_bytes = t;
+    return REYN_OK;
+}
+
 int reyn_ctx_create(int device, reyn_ctx** out)
 {
     if (!out)

@@ -162,3 +162,35 @@ def config4_batch(count: int = 64, seed: int = 4, n_lo: float = 1e4, n_hi: float
         sc.meta.update({"target_n": n_s, "aspect": a, "fill": f})
         out.append(sc)
     return out
+
+
+def config5_jobs(count: int, seed: int = 5, n_lo: float = 5e3, n_hi: float = 5e5, steps_lo: int = 200,
+                 steps_hi: int = 2000, sphere_p: float = 0.5) -> list[tuple[Scene, int]]:
+    """`count` serving jobs (scene, steps): N = round(exp U[ln n_lo, ln n_hi])
+    target particles in a box with sides U[1, 3]^3, a dam-break block of fill
+    f ~ U[0.2, 0.6] along x at 0.6 of the height and full depth, spacing
+    s = (fluid volume / N)^(1/3), with probability `sphere_p` a sphere obstacle
+    of radius U[0.1, 0.25] * min(height, depth) (at most half the free length)
+    resting on the floor in front of the block, and steps ~ U{steps_lo ..
+    steps_hi}."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        n_s = int(round(float(np.exp(rng.uniform(np.log(n_lo), np.log(n_hi))))))
+        a, b, c = (float(v) for v in rng.uniform(1.0, 3.0, 3))
+        f = float(rng.uniform(0.2, 0.6))
+        fluid = np.array([f * a, 0.6 * b, c])
+        s = float((fluid.prod() / n_s) ** (1.0 / 3.0))
+        counts = tuple(max(1, int(e / s)) for e in fluid)
+        sphere = None
+        has_sphere = rng.uniform() < sphere_p
+        r = min(float(rng.uniform(0.1, 0.25)) * min(b, c), 0.5 * (1.0 - f) * a - 2 * s)
+        u = float(rng.uniform())
+        if has_sphere and r >= 4 * s:  # the sphere fits between the block and the far wall
+            lo, hi = f * a + r + 2 * s, a - r - 2 * s
+            sphere = ((lo + u * max(0.0, hi - lo), r + 2 * s, 0.5 * c), r)
+        steps = int(rng.integers(steps_lo, steps_hi + 1))
+        sc = dam_break(counts, s, (a, b, c), jitter=0.01, seed=seed * 1000 + k, sphere=sphere)
+        sc.meta.update({"target_n": n_s, "fill": f, "job": k, "steps": steps})
+        out.append((sc, steps))
+    return out

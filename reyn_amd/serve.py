@@ -1,0 +1,245 @@
+"""Serving: a request loop that keeps simulation jobs resident in one
+continuously batched ``reyn_batch``.
+
+Jobs arrive over time; each is a scene and a number of steps. Waiting jobs
+are admitted in arrival order whenever the device bytes of the resident jobs
+plus the next job fit the HBM budget (``reyn_batch_scene_bytes`` per job).
+Every batch step advances all resident jobs by one step of their own (own
+grid, boundary, time step and pressure loop, in one launch per phase), a job
+that has taken its steps is downloaded and leaves, and jobs that arrived
+meanwhile join before the next step. A job's result does not depend on which
+jobs shared its steps: it is the same, bit for bit, as the job run alone.
+
+The reference has no serving layer (one scene per process, src/main.cu:46-78);
+each job's step is the reference's IISPH or PCISPH step with the main loop's
+grid, time-step controller and bounds (main.cu:59-78).
+
+`run_fifo` is the one-job-at-a-time baseline on the same arrivals: each job
+alone through the same kernels, in arrival order, with its service time
+measured and its time-to-result taken on the arrival clock.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import time
+from collections import deque
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .device import Context
+from .sim import Batch, batch_params, scene_bytes
+
+#: Device memory held back from the admission budget (runtime, the grid and
+#: layout rebuilds that run while jobs join and leave).
+RESERVE_BYTES = 16 << 30
+
+
+@dataclass
+class Job:
+    """One request: a scene advanced `steps` steps from rest. Times are
+    seconds on the run's clock (0 = the start of the run)."""
+
+    jid: int
+    scene: object
+    steps: int
+    arrival: float = 0.0
+    bytes: int = 0
+    admitted: float | None = None
+    finished: float | None = None
+    taken: int = 0
+    result: dict | None = None
+    info: dict = field(default_factory=dict)
+
+    @property
+    def particle_steps(self) -> int:
+        return int(self.scene.n) * int(self.steps)
+
+    @property
+    def time_to_result(self) -> float:
+        return self.finished - self.arrival
+
+
+def device_memory(ctx: Context) -> tuple[int, int]:
+    free, total = C.c_uint64(), C.c_uint64()
+    ctx.check(ctx._lib.reyn_device_memory(ctx.h, C.byref(free), C.byref(total)), "device memory")
+    return int(free.value), int(total.value)
+
+
+def poisson_arrivals(count: int, rate: float, seed: int) -> np.ndarray:
+    """Arrival times of a Poisson process of `rate` per second (seeded)."""
+    rng = np.random.default_rng(seed)
+    return np.cumsum(rng.exponential(1.0 / rate, size=count))
+
+
+def summary(jobs: list[Job], wall: float) -> dict:
+    """p50/p99 time-to-result, mean queueing delay and particle-steps/s of a run
+    (all particle-steps over the span from the first arrival to the last
+    result)."""
+    ttr = np.array([j.time_to_result for j in jobs])
+    span = max(j.finished for j in jobs) - min(j.arrival for j in jobs)
+    work = sum(j.particle_steps for j in jobs)
+    wait = [j.admitted - j.arrival for j in jobs if j.admitted is not None]
+    return {
+        "jobs": len(jobs),
+        "p50_s": float(np.percentile(ttr, 50)),
+        "p99_s": float(np.percentile(ttr, 99)),
+        "mean_s": float(ttr.mean()),
+        "max_s": float(ttr.max()),
+        "mean_wait_s": float(np.mean(wait)) if wait else None,
+        "particle_steps": int(work),
+        "span_s": float(span),
+        "wall_s": float(wall),
+        "particle_steps_per_s": float(work / span),
+    }
+
+
+class Server:
+    """The request loop over one batch. `budget_bytes` defaults to the
+    device's total memory less `reserve_bytes`, capped by what is free when the
+    server starts; `max_resident` bounds the resident jobs."""
+
+    def __init__(self, ctx: Context, budget_bytes: int | None = None, reserve_bytes: int = RESERVE_BYTES,
+                 max_resident: int = 512, keep_results: bool = True, **batch_options):
+        self.ctx = ctx
+        self.options = dict(batch_options)
+        self.batch = Batch(ctx, [], **self.options)
+        self.prm = self.batch.prm
+        free, total = device_memory(ctx)
+        limit = min(total - reserve_bytes, free - reserve_bytes)
+        self.budget = int(limit if budget_bytes is None else min(budget_bytes, limit))
+        if self.budget <= 0:
+            raise RuntimeError(f"serve: no device memory left for jobs (free {free}, reserve {reserve_bytes})")
+        self.max_resident = int(max_resident)
+        self.keep_results = keep_results
+        self.used = 0
+        self.peak_used = 0
+        self.peak_resident = 0
+        self.batch_steps = 0
+        self.layout_changes = 0
+
+    def job_bytes(self, scene) -> int:
+        return scene_bytes(self.ctx, self.prm, scene)
+
+    def close(self) -> None:
+        self.batch.close()
+
+    def run(self, jobs: list[Job], clock=time.perf_counter, log=None) -> dict:
+        """Serves `jobs` (arrival times relative to the start) in real time and
+        returns the run's summary; each job gets its admission and finish
+        times, its step record and (with keep_results) its final state."""
+        jobs = sorted(jobs, key=lambda j: (j.arrival, j.jid))
+        for j in jobs:
+            j.bytes = self.job_bytes(j.scene)
+            if j.bytes > self.budget:
+                raise RuntimeError(f"serve: job {j.jid} needs {j.bytes} bytes, more than the budget {self.budget}")
+        waiting: deque[Job] = deque()
+        resident: dict[int, Job] = {}
+        nxt = 0
+        t0 = clock()
+        while nxt < len(jobs) or waiting or resident:
+            now = clock() - t0
+            while nxt < len(jobs) and jobs[nxt].arrival <= now:
+                waiting.append(jobs[nxt])
+                nxt += 1
+            admit = []
+            while (waiting and len(resident) + len(admit) < self.max_resident
+                   and self.used + waiting[0].bytes <= self.budget):
+                j = waiting.popleft()
+                self.used += j.bytes
+                admit.append(j)
+            if admit:
+                ids = self.batch.add([j.scene for j in admit])
+                self.layout_changes += 1
+                t_adm = clock() - t0
+                for j, bid in zip(admit, ids):
+                    j.admitted = t_adm
+                    resident[bid] = j
+                self.peak_used = max(self.peak_used, self.used)
+                self.peak_resident = max(self.peak_resident, len(resident))
+                if log:
+                    log(f"serve t={t_adm:7.3f}s admitted {[j.jid for j in admit]}, resident {len(resident)}, "
+                        f"{self.used / 2**30:.2f} GiB")
+            if not resident:
+                if nxt < len(jobs):
+                    time.sleep(max(0.0, jobs[nxt].arrival - (clock() - t0)))
+                continue
+            self.batch.step(1)
+            self.batch_steps += 1
+            done = []
+            for bid, j in resident.items():
+                j.taken += 1
+                if j.taken >= j.steps:
+                    done.append(bid)
+            if done:
+                fin = []
+                for bid in done:
+                    j = resident.pop(bid)
+                    if self.keep_results:
+                        j.result = self.batch.download(bid)
+                    inf = self.batch.info(bid)
+                    j.info = {"steps": inf.steps, "iterations": inf.iterations, "dt": inf.dt, "time": inf.time}
+                    self.used -= j.bytes
+                    fin.append(j)
+                self.batch.remove(done)
+                self.layout_changes += 1
+                t_fin = clock() - t0
+                for j in fin:
+                    j.finished = t_fin
+                if log:
+                    log(f"serve t={t_fin:7.3f}s finished {len(done)}, resident {len(resident)}")
+        wall = clock() - t0
+        out = summary(jobs, wall)
+        out.update({"batch_steps": self.batch_steps, "layout_changes": self.layout_changes,
+                    "peak_resident": self.peak_resident, "peak_bytes": int(self.peak_used),
+                    "budget_bytes": int(self.budget)})
+        return out
+
+
+def run_fifo(ctx: Context, jobs: list[Job], keep_results: bool = True, clock=time.perf_counter,
+             **batch_options) -> dict:
+    """One job at a time in arrival order, each alone (a batch of one: the
+    same bits as the single-scene entry points) for all its steps. Service
+    times are measured; a job starts when it has arrived and the previous job
+    has finished."""
+    jobs = sorted(jobs, key=lambda j: (j.arrival, j.jid))
+    t_free = 0.0
+    busy = 0.0
+    for j in jobs:
+        t = clock()
+        b = Batch(ctx, [j.scene], **batch_options)
+        b.step(j.steps)
+        if keep_results:
+            j.result = b.download(0)
+        inf = b.info(0)
+        j.info = {"steps": inf.steps, "iterations": inf.iterations, "dt": inf.dt, "time": inf.time}
+        b.close()
+        service = clock() - t
+        busy += service
+        j.admitted = max(t_free, j.arrival)
+        j.finished = j.admitted + service
+        j.taken = j.steps
+        t_free = j.finished
+    out = summary(jobs, busy)
+    out["busy_s"] = float(busy)
+    return out
+
+
+def make_jobs(scene_steps, arrivals) -> list[Job]:
+    return [Job(k, sc, int(st), float(a)) for k, ((sc, st), a) in enumerate(zip(scene_steps, arrivals))]
+
+
+def same_results(a: list[Job], b: list[Job]) -> tuple[int, list[int]]:
+    """Jobs (by id) whose final states differ in any bit between two runs."""
+    bb = {j.jid: j for j in b}
+    diff = []
+    for j in a:
+        o = bb[j.jid]
+        if any(not np.array_equal(j.result[k], o.result[k]) for k in j.result) or j.info != o.info:
+            diff.append(j.jid)
+    return len(a) - len(diff), diff
+
+
+__all__ = ["Job", "Server", "run_fifo", "make_jobs", "poisson_arrivals", "summary", "same_results",
+           "device_memory", "batch_params", "RESERVE_BYTES"]

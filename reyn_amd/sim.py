@@ -367,52 +367,106 @@ class Simulation:
                          int(info.n_violating) if info else 0)
 
 
-class Batch:
-    """Many scenes advanced together (`reyn_batch_*`): one launch per phase over
-    all scenes, each scene with its own grid, boundary, spacing, time step
-    (the reference main loop's controller) and pressure-loop control."""
-
-    def __init__(self, ctx: Context, scenes, solver: str = "pcisph", kernel: str | int = "W6", nu: float = 0.001,
+def batch_params(scenes=(), solver: str = "pcisph", kernel: str | int = "W6", nu: float = 0.001,
                  cell_ratio: float = 0.5, dt0: float = 0.0005, cfl_lambda: float = 1.0,
                  change_factor: float = 0.001, iter_target: int = 10, eta_rho_max: float | None = None,
                  omega: float = 0.5, min_iter: int = 3, max_iter: int = 100,
                  stop_rule: int = capi.IISPH_STOP_MEAN, warm_start: bool = True, g=GRAVITY, graph: bool = False,
-                 check_every: int = 4):
+                 check_every: int = 4, rho_0: float | None = None) -> capi.BatchParams:
+    """`reyn_batch_params` from keyword options (rho_0 defaults to the first
+    scene's)."""
+    solver_id = {"pcisph": capi.SOLVER_PCISPH, "iisph": capi.SOLVER_IISPH}[solver]
+    if eta_rho_max is None:
+        eta_rho_max = 0.001
+    if rho_0 is None:
+        rho_0 = scenes[0].rho_0 if len(scenes) else 1000.0
+    k = capi.KERNELS[kernel] if isinstance(kernel, str) else int(kernel)
+    return capi.BatchParams(solver_id, k, float(nu), float(rho_0), capi.f3(g),
+                            float(cell_ratio), float(dt0), float(cfl_lambda), float(change_factor),
+                            int(iter_target), float(eta_rho_max), float(omega), int(min_iter),
+                            int(max_iter), int(stop_rule), int(bool(warm_start)), int(bool(graph)),
+                            int(check_every))
+
+
+def batch_scenes(scenes, keep: list):
+    """A `reyn_batch_scene` array over the scenes' host arrays (held in `keep`)."""
+    def ptr(a):
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        keep.append(a)
+        return a.ctypes.data
+
+    arr = (capi.BatchScene * max(1, len(scenes)))()
+    for i, sc in enumerate(scenes):
+        x = np.asarray(sc.x, np.float32)
+        xb = np.asarray(sc.xb, np.float32)
+        arr[i] = capi.BatchScene(sc.n, ptr(x[:, 0]), ptr(x[:, 1]), ptr(x[:, 2]), None, None, None,
+                                 float(sc.mass), sc.n_bdy, ptr(xb[:, 0]) if sc.n_bdy else None,
+                                 ptr(xb[:, 1]) if sc.n_bdy else None, ptr(xb[:, 2]) if sc.n_bdy else None,
+                                 capi.f3(sc.bound_min), capi.f3(sc.bound_max), float(sc.spacing))
+    return arr
+
+
+def scene_bytes(ctx: Context, prm: capi.BatchParams, sc) -> int:
+    """Device bytes the scene occupies while resident in a batch (`reyn_batch_scene_bytes`)."""
+    keep = []
+    arr = batch_scenes([sc], keep)
+    out = C.c_uint64()
+    ctx.check(ctx._lib.reyn_batch_scene_bytes(C.byref(prm), arr, C.byref(out)), "batch scene bytes")
+    return int(out.value)
+
+
+class Batch:
+    """Many scenes advanced together (`reyn_batch_*`): one launch per phase over
+    all scenes, each scene with its own grid, boundary, spacing, time step
+    (the reference main loop's controller) and pressure-loop control. Scenes
+    are addressed by id (0 .. len(scenes)-1 for the initial ones); `add` and
+    `remove` change the resident set between steps."""
+
+    def __init__(self, ctx: Context, scenes, **options):
         self.ctx = ctx
         self.h = None
-        self.scenes = list(scenes)
-        solver_id = {"pcisph": capi.SOLVER_PCISPH, "iisph": capi.SOLVER_IISPH}[solver]
-        if eta_rho_max is None:
-            eta_rho_max = 0.001
-        k = capi.KERNELS[kernel] if isinstance(kernel, str) else int(kernel)
-        self.prm = capi.BatchParams(solver_id, k, float(nu), float(self.scenes[0].rho_0), capi.f3(g),
-                                    float(cell_ratio), float(dt0), float(cfl_lambda), float(change_factor),
-                                    int(iter_target), float(eta_rho_max), float(omega), int(min_iter),
-                                    int(max_iter), int(stop_rule), int(bool(warm_start)), int(bool(graph)),
-                                    int(check_every))
+        scenes = list(scenes)
+        self.prm = batch_params(scenes, **options)
+        self.scenes = {}
         keep = []
-
-        def ptr(a):
-            a = np.ascontiguousarray(a, dtype=np.float32)
-            keep.append(a)
-            return a.ctypes.data
-
-        arr = (capi.BatchScene * len(self.scenes))()
-        for i, sc in enumerate(self.scenes):
-            x = np.asarray(sc.x, np.float32)
-            xb = np.asarray(sc.xb, np.float32)
-            arr[i] = capi.BatchScene(sc.n, ptr(x[:, 0]), ptr(x[:, 1]), ptr(x[:, 2]), None, None, None,
-                                     float(sc.mass), sc.n_bdy, ptr(xb[:, 0]) if sc.n_bdy else None,
-                                     ptr(xb[:, 1]) if sc.n_bdy else None, ptr(xb[:, 2]) if sc.n_bdy else None,
-                                     capi.f3(sc.bound_min), capi.f3(sc.bound_max), float(sc.spacing))
+        arr = batch_scenes(scenes, keep)
         self.h = C.c_void_p()
-        ctx.check(ctx._lib.reyn_batch_create(ctx.h, C.byref(self.prm), arr, len(self.scenes), C.byref(self.h)),
+        ctx.check(ctx._lib.reyn_batch_create(ctx.h, C.byref(self.prm), arr, len(scenes), C.byref(self.h)),
                   "batch create")
+        self.scenes = dict(enumerate(scenes))
         ctx.adopt(self)
 
     @property
     def n(self) -> int:
-        return sum(sc.n for sc in self.scenes)
+        return sum(sc.n for sc in self.scenes.values())
+
+    def add(self, scenes) -> list[int]:
+        scenes = list(scenes)
+        if not scenes:
+            return []
+        keep = []
+        arr = batch_scenes(scenes, keep)
+        ids = (C.c_uint32 * len(scenes))()
+        self.ctx.check(self.ctx._lib.reyn_batch_add(self.h, arr, len(scenes), ids), "batch add")
+        out = list(ids)
+        self.scenes.update(zip(out, scenes))
+        return out
+
+    def remove(self, ids) -> None:
+        ids = [int(i) for i in ids]
+        if not ids:
+            return
+        arr = (C.c_uint32 * len(ids))(*ids)
+        self.ctx.check(self.ctx._lib.reyn_batch_remove(self.h, arr, len(ids)), "batch remove")
+        for i in ids:
+            self.scenes.pop(i, None)
+
+    def ids(self) -> list[int]:
+        cnt = C.c_uint32()
+        self.ctx.check(self.ctx._lib.reyn_batch_scenes(self.h, None, 0, C.byref(cnt)), "batch scenes")
+        arr = (C.c_uint32 * max(1, cnt.value))()
+        self.ctx.check(self.ctx._lib.reyn_batch_scenes(self.h, arr, cnt.value, C.byref(cnt)), "batch scenes")
+        return list(arr)[:cnt.value]
 
     def step(self, steps: int = 1) -> None:
         self.ctx.check(self.ctx._lib.reyn_batch_step(self.h, int(steps)), "batch step")

@@ -84,3 +84,57 @@ def test_batch_rejects_bad_input(ctx):
     sc = small_scenes(count=1)[0]
     with pytest.raises(capi.ReynError):
         Batch(ctx, [sc], solver="pcisph", kernel="W6", dt0=0.0)
+
+
+@pytest.mark.parametrize("solver,kernel,graph", [("pcisph", "W6", False), ("iisph", "B3", False),
+                                                 ("iisph", "B3", True)])
+def test_scenes_joining_and_leaving_match_scenes_alone(ctx, solver, kernel, graph):
+    """Continuous batching: scenes join and leave a running batch between
+    steps; each scene's state after its own step count equals the scene run
+    alone for that many steps (the layout is rebuilt at every change, and
+    resident scenes carry their state, time step, warm-start pressure and
+    iteration history across it)."""
+    scs = small_scenes(count=4, seed=13)
+    total = {0: 10, 1: 6, 2: 8, 3: 5}
+    # (steps to take, scenes that join before them)
+    plan = [(3, [0, 1]), (3, [2]), (2, []), (2, [3]), (1, []), (2, [])]
+    mix = Batch(ctx, [], solver=solver, kernel=kernel, graph=graph, rho_0=scs[0].rho_0)
+    ids, taken, results = {}, {}, {}
+    for steps, join in plan:
+        new = mix.add([scs[k] for k in join])
+        for k, i in zip(join, new):
+            ids[k] = i
+            taken[k] = 0
+        assert sorted(mix.ids()) == sorted(ids[k] for k in taken)
+        mix.step(steps)
+        for k in list(taken):
+            taken[k] += steps
+            if taken[k] == total[k]:
+                results[k] = (mix.download(ids[k]), mix.info(ids[k]))
+                mix.remove([ids[k]])
+                del taken[k]
+    assert sorted(results) == [0, 1, 2, 3] and not mix.ids()
+    mix.step(1)  # an empty batch steps as a no-op
+    for k, sc in enumerate(scs):
+        alone = Batch(ctx, [sc], solver=solver, kernel=kernel)
+        alone.step(total[k])
+        got, info = results[k]
+        assert_same(alone.download(0), got, f"scene {k}")
+        ia = alone.info(0)
+        assert (ia.steps, ia.iterations, ia.dt, ia.time) == (info.steps, info.iterations, info.dt, info.time)
+        alone.close()
+    mix.close()
+
+
+def test_batch_remove_rejects_unknown_ids(ctx):
+    scs = small_scenes(count=2, seed=3)
+    b = Batch(ctx, scs, solver="pcisph", kernel="W6")
+    with pytest.raises(capi.ReynError):
+        b.remove([7])
+    with pytest.raises(capi.ReynError):
+        b.remove([0, 0])
+    b.remove([0])
+    assert b.ids() == [1]
+    b.step(1)
+    assert b.info(1).steps == 1
+    b.close()
