@@ -264,6 +264,8 @@ def main() -> int:
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--serve-jobs", type=int, default=32)
     ap.add_argument("--serve-load", type=float, default=0.8)
+    ap.add_argument("--only", choices=("batch", "serve"), default=None,
+                    help="run only the batch or the serving leg and print its object as one JSON line")
     ap.add_argument("--no-profile", action="store_true", help="time without per-launch events")
     ap.add_argument("--dt-policy", choices=("controller", "cfl"), default="controller",
                     help="controller: the reference main loop's simple_dt_controller from dt0 = 5e-4 "
@@ -304,6 +306,12 @@ def main() -> int:
     from reyn_amd import capi, scenes
     from reyn_amd.device import Context
     from reyn_amd.sim import IISPH, BoundarySamples, Particles, Simulation, reduce
+
+    if args.only:
+        with Context(0) as ctx:
+            leg = batch_leg(ctx, args, log) if args.only == "batch" else serve_leg(ctx, args, log)
+        print(json.dumps({"batch" if args.only == "batch" else "serving": leg}), flush=True)
+        return 0
 
     t_setup = time.perf_counter()
     sc = scenes.config3_iisph(side=args.side)

@@ -10,6 +10,8 @@
 //   Boundary, BoundarySamples, Scene bounds/grid  src/scene/scene.cuh:14-156
 //   SESPH<K,R>, PCISPH<K,R>, IISPH<K,R>  src/solvers/*.cuh
 //   cfl_time_step, simple_dt_controller  src/timestep/cfl.cuh:13-32
+//   save_to_ply                          src/scene/ply_io.cu:7-56
+//   SceneBatch: many scenes per launch (no reference counterpart)
 //
 // Device errors and misuse throw std::runtime_error (the reference throws on
 // misuse and exits on CUDA errors, src/common.h:24-36; a library must not
@@ -17,9 +19,12 @@
 #pragma once
 
 #include <algorithm>
+#include <bit>
 #include <concepts>
 #include <cstring>
 #include <cstdint>
+#include <filesystem>
+#include <fstream>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -618,6 +623,105 @@ private:
     reyn_iisph_params prm_ {};
     reyn_solve_info info_ {};
     DeviceBuffer<float>&rho_, &aii_, &si_, &ax_, &ay_, &az_, &p_;
+};
+
+// ---- save_to_ply (src/scene/ply_io.cu:7-56) ----------------------------------------
+//
+// Binary PLY of the positions in the host's byte order: one vertex element of
+// float32 x, y, z, written as x_i, y_i, z_i per particle.
+inline void save_to_ply(const std::filesystem::path& path, const DeviceBuffer<float>& xs,
+    const DeviceBuffer<float>& ys, const DeviceBuffer<float>& zs)
+{
+    const size_t n = xs.size();
+    if (ys.size() != n || zs.size() != n)
+        throw std::runtime_error("save_to_ply called on buffers with differing sizes");
+    static_assert(std::endian::native == std::endian::little || std::endian::native == std::endian::big,
+        "save_to_ply needs a little- or big-endian host");
+    const char* order = std::endian::native == std::endian::little ? "little" : "big";
+    const std::vector<float> x = xs.download(), y = ys.download(), z = zs.download();
+    std::vector<float> xyz(3 * n);
+    for (size_t i = 0; i < n; ++i) {
+        xyz[3 * i] = x[i];
+        xyz[3 * i + 1] = y[i];
+        xyz[3 * i + 2] = z[i];
+    }
+    std::ofstream out(path, std::ios::binary);
+    if (!out)
+        throw std::runtime_error("save_to_ply: cannot open " + path.string());
+    out << "ply\nformat binary_" << order << "_endian 1.0\n"
+        << "comment particle positions written by reyn_amd\n"
+        << "element vertex " << n << "\n"
+        << "property float32 x\nproperty float32 y\nproperty float32 z\nend_header\n";
+    out.write(reinterpret_cast<const char*>(xyz.data()), std::streamsize(xyz.size() * sizeof(float)));
+    if (!out)
+        throw std::runtime_error("save_to_ply: write failed for " + path.string());
+}
+
+// ---- scene batches (no reference counterpart) -------------------------------------
+//
+// Many scenes advanced together, each step being the reference's main-loop
+// step (main.cu:59-78) per scene; scenes join and leave between steps and are
+// addressed by id. A scene's result is the same alone and in any batch.
+
+struct BatchSceneState {
+    std::vector<float> x, y, z, vx, vy, vz, rho, p;
+};
+
+class SceneBatch {
+public:
+    SceneBatch(const reyn_batch_params& prm, const std::vector<reyn_batch_scene>& scenes = {},
+        Context& ctx = Context::current())
+        : ctx_(&ctx)
+    {
+        ctx.check(reyn_batch_create(ctx.get(), &prm, scenes.data(), uint32_t(scenes.size()), &b_),
+            "SceneBatch create");
+        for (size_t k = 0; k < scenes.size(); ++k)
+            n_.push_back({ uint32_t(k), scenes[k].n });
+    }
+    ~SceneBatch() { reyn_batch_destroy(b_); }
+    SceneBatch(const SceneBatch&) = delete;
+    SceneBatch& operator=(const SceneBatch&) = delete;
+
+    std::vector<uint32_t> add(const std::vector<reyn_batch_scene>& scenes)
+    {
+        std::vector<uint32_t> ids(scenes.size());
+        ctx_->check(reyn_batch_add(b_, scenes.data(), uint32_t(scenes.size()), ids.data()), "SceneBatch add");
+        for (size_t k = 0; k < scenes.size(); ++k)
+            n_.push_back({ ids[k], scenes[k].n });
+        return ids;
+    }
+    void remove(const std::vector<uint32_t>& ids)
+    {
+        ctx_->check(reyn_batch_remove(b_, ids.data(), uint32_t(ids.size())), "SceneBatch remove");
+        std::erase_if(n_, [&](const auto& e) { return std::find(ids.begin(), ids.end(), e.first) != ids.end(); });
+    }
+    void step(uint32_t steps = 1) { ctx_->check(reyn_batch_step(b_, steps), "SceneBatch step"); }
+    reyn_batch_scene_info info(uint32_t id) const
+    {
+        reyn_batch_scene_info out {};
+        ctx_->check(reyn_batch_info(b_, id, &out), "SceneBatch info");
+        return out;
+    }
+    BatchSceneState download(uint32_t id) const
+    {
+        uint32_t n = 0;
+        for (const auto& e : n_)
+            if (e.first == id)
+                n = e.second;
+        BatchSceneState s;
+        for (auto* v : { &s.x, &s.y, &s.z, &s.vx, &s.vy, &s.vz, &s.rho, &s.p })
+            v->resize(n);
+        ctx_->check(reyn_batch_download(b_, id, s.x.data(), s.y.data(), s.z.data(), s.vx.data(), s.vy.data(),
+                        s.vz.data(), s.rho.data(), s.p.data()),
+            "SceneBatch download");
+        return s;
+    }
+    reyn_batch* get() const { return b_; }
+
+private:
+    Context* ctx_;
+    reyn_batch* b_ = nullptr;
+    std::vector<std::pair<uint32_t, uint32_t>> n_; // id, particles
 };
 
 } // namespace reyn

@@ -384,6 +384,14 @@ enum { REYN_SOLVER_PCISPH = 1, REYN_SOLVER_IISPH = 2 };
 
 typedef struct reyn_batch reyn_batch;
 
+typedef struct reyn_batch_scene_info {
+    uint32_t steps;          /* steps taken */
+    uint32_t iterations;     /* pressure iterations of the last step */
+    float dt;                /* time step of the last step */
+    double time;             /* simulated time */
+    double loop_stat;        /* PCISPH: mean predicted density; IISPH: mean error */
+} reyn_batch_scene_info;
+
 typedef struct reyn_batch_scene {
     uint32_t n;                  /* fluid particles */
     const float *x, *y, *z;      /* host arrays of n: initial positions */
@@ -394,6 +402,13 @@ typedef struct reyn_batch_scene {
     reyn_float3 bound_min;       /* the fluid domain: grid extent and the */
     reyn_float3 bound_max;       /* hard_enforce_bounds box */
     float h;                     /* particle spacing; kernel support 2h */
+    const float* p;              /* host array of n: initial pressure (IISPH warm
+                                    start), or NULL for zero */
+    /* NULL for a scene that starts now; or the record of a scene downloaded
+     * from a batch (with its state in the order downloaded and its pressure):
+     * the scene continues from there exactly as if it had stayed (the time
+     * step controller resumes from `dt` and `iterations`) */
+    const reyn_batch_scene_info* resume;
 } reyn_batch_scene;
 
 typedef struct reyn_batch_params {
@@ -418,24 +433,17 @@ typedef struct reyn_batch_params {
     uint32_t check_every;
 } reyn_batch_params;
 
-typedef struct reyn_batch_scene_info {
-    uint32_t steps;          /* steps taken */
-    uint32_t iterations;     /* pressure iterations of the last step */
-    float dt;                /* time step of the last step */
-    double time;             /* simulated time */
-    double loop_stat;        /* PCISPH: mean predicted density; IISPH: mean error */
-} reyn_batch_scene_info;
-
 /* Uploads the scenes, builds their boundary grids and Akinci masses. The
  * scenes get the ids 0 .. count-1; `count` may be 0 (an empty batch that
  * scenes join later). */
 int reyn_batch_create(reyn_ctx* ctx, const reyn_batch_params* prm,
     const reyn_batch_scene* scenes, uint32_t count, reyn_batch** out);
 int reyn_batch_destroy(reyn_batch* b);
-/* Scenes join and leave between steps. Joining scenes start at rest state
- * with the batch's dt0; resident scenes keep their state, time step and loop
- * history, so a scene's result does not depend on which scenes shared its
- * steps. `ids` (may be NULL) receives the new scenes' ids. */
+/* Scenes join and leave between steps. A joining scene starts from its given
+ * state with the batch's dt0, or continues a downloaded scene (`resume`);
+ * resident scenes keep their state, time step and loop history, so a scene's
+ * result does not depend on which scenes shared its steps. `ids` (may be
+ * NULL) receives the new scenes' ids. */
 int reyn_batch_add(reyn_batch* b, const reyn_batch_scene* scenes, uint32_t count,
     uint32_t* ids);
 int reyn_batch_remove(reyn_batch* b, const uint32_t* ids, uint32_t count);

@@ -141,11 +141,14 @@ class IisphWork(C.Structure):
 SOLVER_PCISPH, SOLVER_IISPH = 1, 2
 
 
+class BatchSceneInfo(C.Structure):
+    _fields_ = [("steps", u32), ("iterations", u32), ("dt", f32), ("time", f64), ("loop_stat", f64)]
+
 class BatchScene(C.Structure):
     _fields_ = [
         ("n", u32), ("x", vp), ("y", vp), ("z", vp), ("vx", vp), ("vy", vp), ("vz", vp), ("mass", f32),
         ("n_bdy", u32), ("bx", vp), ("by", vp), ("bz", vp), ("bound_min", Float3), ("bound_max", Float3),
-        ("h", f32),
+        ("h", f32), ("p", vp), ("resume", C.POINTER(BatchSceneInfo)),
     ]
 
 
@@ -157,9 +160,6 @@ class BatchParams(C.Structure):
         ("warm_start", i32), ("graph", i32), ("check_every", u32),
     ]
 
-
-class BatchSceneInfo(C.Structure):
-    _fields_ = [("steps", u32), ("iterations", u32), ("dt", f32), ("time", f64), ("loop_stat", f64)]
 
 
 P = C.POINTER

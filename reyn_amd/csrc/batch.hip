@@ -254,6 +254,8 @@ int check_scene(reyn_ctx* ctx, const reyn_batch_scene& sc, uint32_t s)
         return set_error(ctx, REYN_ERR_INVALID_ARGUMENT, "batch: scene %u has NULL boundary arrays", s);
     if (!(sc.h > 0.f) || !(sc.mass > 0.f))
         return set_error(ctx, REYN_ERR_INVALID_ARGUMENT, "batch: scene %u needs a positive spacing and mass", s);
+    if (sc.resume && !(sc.resume->dt > 0.f))
+        return set_error(ctx, REYN_ERR_INVALID_ARGUMENT, "batch: scene %u resumes with a non-positive time step", s);
     return REYN_OK;
 }
 
@@ -356,6 +358,13 @@ int make_scene(reyn_batch* b, const reyn_batch_scene& sc, SceneRes** out)
         r->box_lo = { sc.bound_min.x, sc.bound_min.y, sc.bound_min.z };
         r->box_hi = { sc.bound_max.x, sc.bound_max.y, sc.bound_max.z };
         r->dt = prm.dt0;
+        if (sc.resume) { // the controller continues from the saved step
+            r->dt = sc.resume->dt;
+            r->iters_last = sc.resume->iterations;
+            r->info = *sc.resume;
+            r->iters = sc.resume->iterations;
+            r->stat = sc.resume->loop_stat;
+        }
         if (prm.solver == REYN_SOLVER_PCISPH)
             rc = reyn_pcisph_delta(prm.kernel, rad, h, &r->delta);
     } while (false);
@@ -456,7 +465,10 @@ int relayout(reyn_batch* b, const std::vector<SceneRes*>& next, const std::vecto
                 for (int f = 0; f < 6; ++f)
                     REYN_TRY(upload(ctx, L.st[0][f] + first, in[f] ? in[f] : zeros.data(), bytes));
                 REYN_TRY(reyn_fill_f32(ctx, L.st[0][6] + first, r.mass, n));
-                REYN_TRY(reyn_fill_f32(ctx, L.st[0][7] + first, 0.f, n));
+                if (sc.p)
+                    REYN_TRY(upload(ctx, L.st[0][7] + first, sc.p, bytes));
+                else
+                    REYN_TRY(reyn_fill_f32(ctx, L.st[0][7] + first, 0.f, n));
                 REYN_TRY(reyn_fill_f32(ctx, L.rho + first, 0.f, n));
                 REYN_HIP(ctx, hipStreamSynchronize(ctx->stream)); // `zeros` is reused
             }

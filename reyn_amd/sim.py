@@ -389,20 +389,36 @@ def batch_params(scenes=(), solver: str = "pcisph", kernel: str | int = "W6", nu
 
 
 def batch_scenes(scenes, keep: list):
-    """A `reyn_batch_scene` array over the scenes' host arrays (held in `keep`)."""
+    """A `reyn_batch_scene` array over the scenes' host arrays (held in
+    `keep`); an item is a Scene (starts at rest) or an `io.Checkpoint`
+    (continues where it was taken)."""
+    from .io import Checkpoint
+
     def ptr(a):
         a = np.ascontiguousarray(a, dtype=np.float32)
         keep.append(a)
         return a.ctypes.data
 
     arr = (capi.BatchScene * max(1, len(scenes)))()
-    for i, sc in enumerate(scenes):
-        x = np.asarray(sc.x, np.float32)
+    for i, item in enumerate(scenes):
+        sc = item.scene if isinstance(item, Checkpoint) else item
         xb = np.asarray(sc.xb, np.float32)
-        arr[i] = capi.BatchScene(sc.n, ptr(x[:, 0]), ptr(x[:, 1]), ptr(x[:, 2]), None, None, None,
-                                 float(sc.mass), sc.n_bdy, ptr(xb[:, 0]) if sc.n_bdy else None,
-                                 ptr(xb[:, 1]) if sc.n_bdy else None, ptr(xb[:, 2]) if sc.n_bdy else None,
-                                 capi.f3(sc.bound_min), capi.f3(sc.bound_max), float(sc.spacing))
+        bdy = (ptr(xb[:, 0]), ptr(xb[:, 1]), ptr(xb[:, 2])) if sc.n_bdy else (None, None, None)
+        if isinstance(item, Checkpoint):
+            st = item.state
+            pos = [ptr(st[k]) for k in ("xx", "xy", "xz")]
+            vel = [ptr(st[k]) for k in ("vx", "vy", "vz")]
+            prs = ptr(st["p"])
+            info = capi.BatchSceneInfo(item.steps, item.iterations, item.dt, item.time, item.loop_stat)
+            keep.append(info)
+            resume = C.pointer(info)
+            n = item.n
+        else:
+            x = np.asarray(sc.x, np.float32)
+            pos = [ptr(x[:, 0]), ptr(x[:, 1]), ptr(x[:, 2])]
+            vel, prs, resume, n = [None, None, None], None, None, sc.n
+        arr[i] = capi.BatchScene(n, *pos, *vel, float(sc.mass), sc.n_bdy, *bdy, capi.f3(sc.bound_min),
+                                 capi.f3(sc.bound_max), float(sc.spacing), prs, resume)
     return arr
 
 
@@ -446,11 +462,23 @@ class Batch:
             return []
         keep = []
         arr = batch_scenes(scenes, keep)
+        from .io import Checkpoint
+
         ids = (C.c_uint32 * len(scenes))()
         self.ctx.check(self.ctx._lib.reyn_batch_add(self.h, arr, len(scenes), ids), "batch add")
         out = list(ids)
-        self.scenes.update(zip(out, scenes))
+        self.scenes.update(zip(out, [s.scene if isinstance(s, Checkpoint) else s for s in scenes]))
         return out
+
+    def checkpoint(self, scene: int):
+        """The scene's state and step record (`io.Checkpoint`); adding it to
+        any batch continues the scene exactly."""
+        from .io import STATE_FIELDS, Checkpoint
+
+        d = self.download(scene)
+        inf = self.info(scene)
+        return Checkpoint(self.scenes[scene], {k: d[k] for k in STATE_FIELDS}, int(inf.steps), int(inf.iterations),
+                          float(inf.dt), float(inf.time), float(inf.loop_stat))
 
     def remove(self, ids) -> None:
         ids = [int(i) for i in ids]

@@ -2,6 +2,11 @@
 // the style of the reference's doctest cases. Run by tests/test_gpu_cpp_api.py.
 #include <cmath>
 #include <cstdio>
+#include <cstring>
+#include <filesystem>
+#include <fstream>
+#include <iterator>
+#include <algorithm>
 #include <functional>
 #include <numeric>
 #include <random>
@@ -211,6 +216,97 @@ int main()
             CHECK(iters >= 3);
             CHECK(all_finite(state.xy) && all_finite(rho));
         }
+    });
+
+    run("SceneBatch: scenes alone and in a batch that they join and leave", [] {
+        Scene a = dam_break(10, 0.05f, v3(1.f, 1.f, 1.f));
+        Scene b = dam_break(8, 0.06f, v3(1.2f, 1.f, 0.9f));
+        std::vector<float> abx, aby, abz, bbx, bby, bbz;
+        for (const float3& w : a.wall) {
+            abx.push_back(w.x);
+            aby.push_back(w.y);
+            abz.push_back(w.z);
+        }
+        for (const float3& w : b.wall) {
+            bbx.push_back(w.x);
+            bby.push_back(w.y);
+            bbz.push_back(w.z);
+        }
+        auto desc = [](const Scene& s, const std::vector<float>& bx, const std::vector<float>& by,
+                        const std::vector<float>& bz) {
+            reyn_batch_scene d {};
+            d.n = uint32_t(s.x.size());
+            d.x = s.x.data();
+            d.y = s.y.data();
+            d.z = s.z.data();
+            d.mass = 1000.f * s.h * s.h * s.h;
+            d.n_bdy = uint32_t(bx.size());
+            d.bx = bx.data();
+            d.by = by.data();
+            d.bz = bz.data();
+            d.bound_min = v3(*std::min_element(bx.begin(), bx.end()), *std::min_element(by.begin(), by.end()),
+                *std::min_element(bz.begin(), bz.end()));
+            d.bound_max = v3(*std::max_element(bx.begin(), bx.end()), *std::max_element(by.begin(), by.end()),
+                *std::max_element(bz.begin(), bz.end()));
+            d.h = s.h;
+            return d;
+        };
+        reyn_batch_params prm {};
+        prm.solver = REYN_SOLVER_IISPH;
+        prm.kernel = REYN_KERNEL_B3;
+        prm.nu = 0.001f;
+        prm.rho_0 = 1000.f;
+        prm.g = v3(0.f, -9.81f, 0.f);
+        prm.cell_ratio = 0.5f;
+        prm.dt0 = 5e-4f;
+        prm.cfl_lambda = 1.f;
+        prm.change_factor = 0.001f;
+        prm.iter_target = 10;
+        prm.eta_rho_max = 0.001f;
+        prm.omega = 0.5f;
+        prm.min_iter = 3;
+        prm.max_iter = 100;
+        prm.stop_rule = REYN_IISPH_STOP_MEAN;
+        prm.warm_start = 1;
+        const reyn_batch_scene da = desc(a, abx, aby, abz), db = desc(b, bbx, bby, bbz);
+        // a runs 6 steps; b joins after 2 and leaves after 3 of its own
+        SceneBatch mix(prm, { da });
+        mix.step(2);
+        const uint32_t ib = mix.add({ db })[0];
+        mix.step(3);
+        const BatchSceneState gb = mix.download(ib);
+        CHECK(mix.info(ib).steps == 3);
+        mix.remove({ ib });
+        mix.step(1);
+        const BatchSceneState ga = mix.download(0);
+        CHECK(mix.info(0).steps == 6);
+        SceneBatch alone_a(prm, { da }), alone_b(prm, { db });
+        alone_a.step(6);
+        alone_b.step(3);
+        const BatchSceneState ra = alone_a.download(0), rb = alone_b.download(0);
+        CHECK(ga.x == ra.x && ga.y == ra.y && ga.z == ra.z && ga.vx == ra.vx && ga.p == ra.p && ga.rho == ra.rho);
+        CHECK(gb.x == rb.x && gb.vy == rb.vy && gb.p == rb.p && gb.rho == rb.rho);
+        CHECK(mix.info(0).iterations == alone_a.info(0).iterations && mix.info(0).dt == alone_a.info(0).dt);
+    });
+
+    run("save_to_ply writes the reference's binary PLY layout", [] {
+        std::vector<float> x { 1.f, 2.f, 3.f }, y { -1.f, 0.5f, 7.f }, z { 0.f, 1e-3f, 2.f };
+        DeviceBuffer<float> xs(1), ys(1), zs(1);
+        xs.upload(x);
+        ys.upload(y);
+        zs.upload(z);
+        const std::filesystem::path path = std::filesystem::temp_directory_path() / "reyn_amd_test.ply";
+        save_to_ply(path, xs, ys, zs);
+        std::ifstream in(path, std::ios::binary);
+        const std::string all((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
+        const size_t end = all.find("end_header\n");
+        CHECK(all.rfind("ply\nformat binary_", 0) == 0);
+        CHECK(all.find("element vertex 3\n") != std::string::npos);
+        CHECK(end != std::string::npos && all.size() == end + 11 + 9 * sizeof(float));
+        float back[9];
+        std::memcpy(back, all.data() + end + 11, sizeof(back));
+        CHECK(back[0] == 1.f && back[1] == -1.f && back[2] == 0.f && back[7] == 7.f && back[8] == 2.f);
+        std::filesystem::remove(path);
     });
 
     std::printf("%d checks, %d failed\n", g_checks, g_failed);

@@ -138,3 +138,31 @@ def test_batch_remove_rejects_unknown_ids(ctx):
     b.step(1)
     assert b.info(1).steps == 1
     b.close()
+
+
+@pytest.mark.parametrize("solver,kernel", [("pcisph", "W6"), ("iisph", "B3")])
+def test_checkpoint_resume_continues_exactly(ctx, tmp_path, solver, kernel):
+    """A scene checkpointed after 5 steps (saved to disk and loaded back) and
+    resumed in another batch, next to another scene, ends where the scene run
+    for 11 steps without interruption ends, bit for bit."""
+    from reyn_amd.io import Checkpoint
+
+    scs = small_scenes(count=2, seed=17)
+    straight = Batch(ctx, [scs[0]], solver=solver, kernel=kernel)
+    straight.step(11)
+    first = Batch(ctx, [scs[0]], solver=solver, kernel=kernel)
+    first.step(5)
+    ck = first.checkpoint(0)
+    first.close()
+    ck.save(tmp_path / "scene.npz")
+    back = Checkpoint.load(tmp_path / "scene.npz")
+    second = Batch(ctx, [scs[1]], solver=solver, kernel=kernel)
+    second.step(2)
+    sid = second.add([back])[0]
+    assert second.info(sid).steps == 5
+    second.step(6)
+    assert_same(second.download(sid), straight.download(0), "resumed vs straight")
+    a, b = second.info(sid), straight.info(0)
+    assert (a.steps, a.iterations, a.dt, a.time) == (b.steps, b.iterations, b.dt, b.time)
+    second.close()
+    straight.close()
