@@ -271,6 +271,8 @@ def main() -> int:
                     help="controller: the reference main loop's simple_dt_controller from dt0 = 5e-4 "
                          "(lambda 1, 10 target iterations, 0.1%% change per step); cfl: cfl_time_step(lambda)")
     ap.add_argument("--cfl-lambda", type=float, default=None)
+    ap.add_argument("--cell-ratio", type=float, default=None,
+                    help="grid cell size / search radius (default: autotuned over 0.5 and 1.0)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run an identity gather of 2^26 floats (reads 512 MiB, writes 256 MiB), the "
                          "plain copy kernel that scales rocprofv3 FETCH_SIZE/WRITE_SIZE to bytes")
@@ -317,14 +319,31 @@ def main() -> int:
     sc = scenes.config3_iisph(side=args.side)
     log(f"scene: {sc.n} fluid, {sc.n_bdy} boundary, spacing {sc.spacing:.6f}, sphere {sc.meta.get('sphere')}")
     ctx = Context(0)
-    state = Particles(ctx, sc.x, None, sc.mass, sc.rho_0)
-    bdy = BoundarySamples(ctx, sc.xb, sc.spacing, sc.rho_0)
-    solver = IISPH(ctx, "B3", sc.n, 0.001, sc.spacing, sc.rho_0, omega=0.5, eta_rho_max=0.001, min_iter=3,
-                   max_iter=100, stop_rule=capi.IISPH_STOP_MEAN, warm_start=True, check_every=4)
     lam = args.cfl_lambda if args.cfl_lambda is not None else (1.0 if args.dt_policy == "controller" else 0.4)
-    sim = Simulation(ctx, solver, state, bdy, sc.spacing, sc.bound_min, sc.bound_max, cfl_lambda=lam,
-                     cell_ratio=0.5, extra_reorder=("p",), dt_policy=args.dt_policy, dt0=0.0005, iter_target=10,
-                     change_factor=0.001)
+
+    def make(ratio):
+        state = Particles(ctx, sc.x, None, sc.mass, sc.rho_0)
+        bdy = BoundarySamples(ctx, sc.xb, sc.spacing, sc.rho_0)
+        solver = IISPH(ctx, "B3", sc.n, 0.001, sc.spacing, sc.rho_0, omega=0.5, eta_rho_max=0.001, min_iter=3,
+                       max_iter=100, stop_rule=capi.IISPH_STOP_MEAN, warm_start=True, check_every=4)
+        sim = Simulation(ctx, solver, state, bdy, sc.spacing, sc.bound_min, sc.bound_max, cfl_lambda=lam,
+                         cell_ratio=ratio, extra_reorder=("p",), dt_policy=args.dt_policy, dt0=0.0005,
+                         iter_target=10, change_factor=0.001)
+
+        def close():
+            for o in (solver, bdy, state, sim.builder):
+                getattr(o, "close", lambda: None)()
+        sim.close = close
+        return sim
+
+    tune = None
+    ratio = args.cell_ratio
+    if ratio is None:
+        from reyn_amd.tune import autotune_step
+        ratio, trials = autotune_step(ctx, make, log=log)
+        tune = {"chosen_cell_ratio": ratio, "trials_s": trials, "trial": "fresh scene, 3 steps after 2"}
+    sim = make(ratio)
+    state, bdy, solver = sim.state, sim.bdy, sim.solver
     ncells = sim.builder.geometry().ncells
     log(f"setup {time.perf_counter() - t_setup:.1f} s; fluid grid {ncells} cells")
 
@@ -391,8 +410,7 @@ def main() -> int:
         sc.mb = bdy.m.numpy()
         sc.bdy_min, sc.bdy_max = bdy.bound_min, bdy.bound_max
         cpu = cpu_baseline(sc, host, solver.p.numpy(), sim.next_dt(), args, log)
-    for o in (solver, bdy, state, sim.builder):
-        getattr(o, "close", lambda: None)()
+    sim.close()
     batch = None if args.no_batch else batch_leg(ctx, args, log)
     serving = None if args.no_serve else serve_leg(ctx, args, log)
 
@@ -417,7 +435,8 @@ def main() -> int:
             "fluid_particles": sc.n,
             "boundary_particles": sc.n_bdy,
             "grid_cells": ncells,
-            "cell_ratio": 0.5,
+            "cell_ratio": ratio,
+            "cell_ratio_autotune": tune,
             "jacobi_iterations_mean": round(mean_iters, 2),
             "jacobi_iterations": iters,
             "mean_density_error_max": max(errs) if errs else None,

@@ -68,3 +68,29 @@ def autotune_batch(ctx: Context, scenes, ratios=RATIOS, checks=CHECKS, warmup: i
                     f"{tr.particle_steps_per_s / 1e6:.1f} M particle-steps/s")
     best = min(trials, key=lambda t: t.seconds)
     return {"cell_ratio": best.cell_ratio, "check_every": best.check_every}, trials
+
+
+def autotune_step(ctx: Context, make, ratios=RATIOS, warmup: int = 2, steps: int = 3, log=None):
+    """The single-scene form: `make(cell_ratio)` returns a fresh simulation
+    (an object with `step()` and `close()`) of the same scene; times `steps`
+    steps after `warmup` per ratio and returns the fastest ratio and the
+    trials as (ratio, seconds) pairs."""
+    trials = []
+    for r in ratios:
+        sim = make(r)
+        try:
+            for _ in range(warmup):
+                sim.step()
+            ctx.synchronize()
+            t = time.perf_counter()
+            for _ in range(steps):
+                sim.step()
+            ctx.synchronize()
+            dt = time.perf_counter() - t
+        finally:
+            sim.close()
+        trials.append((float(r), dt))
+        if log:
+            log(f"autotune: cell ratio {r}: {dt * 1e3:.2f} ms for {steps} steps")
+    best = min(trials, key=lambda t: t[1])
+    return best[0], trials
