@@ -690,6 +690,19 @@ public:
             n_.push_back({ ids[k], scenes[k].n });
         return ids;
     }
+    // scenes generated on the device from their descriptions
+    std::vector<uint32_t> add_dam_breaks(const std::vector<reyn_dam_break>& scenes)
+    {
+        std::vector<uint32_t> ids(scenes.size());
+        ctx_->check(reyn_batch_add_dam_breaks(b_, scenes.data(), uint32_t(scenes.size()), ids.data()),
+            "SceneBatch add_dam_breaks");
+        for (size_t k = 0; k < scenes.size(); ++k) {
+            uint32_t n = 0;
+            ctx_->check(reyn_batch_scene_size(b_, ids[k], &n), "SceneBatch scene size");
+            n_.push_back({ ids[k], n });
+        }
+        return ids;
+    }
     void remove(const std::vector<uint32_t>& ids)
     {
         ctx_->check(reyn_batch_remove(b_, ids.data(), uint32_t(ids.size())), "SceneBatch remove");

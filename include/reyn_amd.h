@@ -370,6 +370,37 @@ int reyn_hard_enforce_bounds(reyn_ctx* ctx, reyn_float3 bound_min,
 int reyn_reduce_f32(reyn_ctx* ctx, const float* x, uint32_t n, int32_t op,
     double* out);
 
+/* ---- lattice scenes on the device ------------------------------------------
+ * A dam break: a fluid lattice of counts[0] x counts[1] x counts[2] points at
+ * (i + 1/2) s from the origin, each coordinate moved by U(-jitter s, jitter s)
+ * from a counter hash of (seed, point, axis); the lattice shell of `layers`
+ * layers of the box [0, box] at the same spacing (the points of the
+ * box lattice outside its inner index box, x fastest); optionally `layers`
+ * Fibonacci-lattice spheres of radii radius - (k + 1/2) s around `center`,
+ * with the fluid within radius + s/2 of the centre removed. All coordinates
+ * are formed in double and rounded to float. The reference initialises fluid
+ * boxes on the device with curand jitter (src/scene/scene.cu:14-54, 239-253)
+ * and samples boundaries from meshes; no counterpart for the shell. */
+typedef struct reyn_dam_break {
+    uint32_t counts[3];
+    double spacing;
+    double box[3];
+    double jitter;           /* fraction of the spacing */
+    uint32_t seed;
+    uint32_t layers;         /* wall (and sphere) layers, at most 8 */
+    int32_t sphere;          /* 1: sphere obstacle */
+    double center[3];
+    double radius;
+} reyn_dam_break;
+
+/* The lattice size (the fluid count before the sphere is cut out) and the
+ * number of boundary samples. Host only. */
+int reyn_dam_break_sizes(const reyn_dam_break* d, uint32_t* n_lattice, uint32_t* n_bdy);
+/* Generates into device arrays: fluid x, y, z of at least n_lattice entries
+ * (n_fluid receives the count kept), boundary bx, by, bz of n_bdy. */
+int reyn_dam_break_generate(reyn_ctx* ctx, const reyn_dam_break* d, float* x, float* y, float* z,
+    uint32_t* n_fluid, float* bx, float* by, float* bz);
+
 /* ---- scene batches ---------------------------------------------------------
  * Many independent scenes advanced together: every phase of a step is one
  * launch over the particles of all scenes (workgroups aligned to scene
@@ -409,6 +440,7 @@ typedef struct reyn_batch_scene {
      * the scene continues from there exactly as if it had stayed (the time
      * step controller resumes from `dt` and `iterations`) */
     const reyn_batch_scene_info* resume;
+    int32_t device;              /* 1: every array above is a device pointer */
 } reyn_batch_scene;
 
 typedef struct reyn_batch_params {
@@ -447,6 +479,13 @@ int reyn_batch_destroy(reyn_batch* b);
 int reyn_batch_add(reyn_batch* b, const reyn_batch_scene* scenes, uint32_t count,
     uint32_t* ids);
 int reyn_batch_remove(reyn_batch* b, const uint32_t* ids, uint32_t count);
+/* Adds dam-break scenes generated on the device (`reyn_dam_break_generate`)
+ * with particle mass rho_0 s^3 (float, (s*s)*s) and the boundary samples' box
+ * as bounds; no host arrays are involved. */
+int reyn_batch_add_dam_breaks(reyn_batch* b, const reyn_dam_break* scenes, uint32_t count,
+    uint32_t* ids);
+/* The fluid particle count of a resident scene. */
+int reyn_batch_scene_size(const reyn_batch* b, uint32_t scene, uint32_t* n);
 /* The ids of the resident scenes (up to `cap`) and their number. */
 int reyn_batch_scenes(const reyn_batch* b, uint32_t* ids, uint32_t cap, uint32_t* count);
 /* Device bytes a scene occupies while resident in a batch with `prm`: state,

@@ -141,6 +141,13 @@ class IisphWork(C.Structure):
 SOLVER_PCISPH, SOLVER_IISPH = 1, 2
 
 
+class DamBreak(C.Structure):
+    _fields_ = [
+        ("counts", u32 * 3), ("spacing", f64), ("box", f64 * 3), ("jitter", f64), ("seed", u32), ("layers", u32),
+        ("sphere", i32), ("center", f64 * 3), ("radius", f64),
+    ]
+
+
 class BatchSceneInfo(C.Structure):
     _fields_ = [("steps", u32), ("iterations", u32), ("dt", f32), ("time", f64), ("loop_stat", f64)]
 
@@ -148,7 +155,7 @@ class BatchScene(C.Structure):
     _fields_ = [
         ("n", u32), ("x", vp), ("y", vp), ("z", vp), ("vx", vp), ("vy", vp), ("vz", vp), ("mass", f32),
         ("n_bdy", u32), ("bx", vp), ("by", vp), ("bz", vp), ("bound_min", Float3), ("bound_max", Float3),
-        ("h", f32), ("p", vp), ("resume", C.POINTER(BatchSceneInfo)),
+        ("h", f32), ("p", vp), ("resume", C.POINTER(BatchSceneInfo)), ("device", i32),
     ]
 
 
@@ -207,8 +214,12 @@ _SIGS = {
     "reyn_batch_create": (i32, [vp, P(BatchParams), P(BatchScene), u32, P(vp)]),
     "reyn_batch_destroy": (i32, [vp]),
     "reyn_batch_add": (i32, [vp, P(BatchScene), u32, P(u32)]),
+    "reyn_batch_add_dam_breaks": (i32, [vp, P(DamBreak), u32, P(u32)]),
+    "reyn_dam_break_sizes": (i32, [P(DamBreak), P(u32), P(u32)]),
+    "reyn_dam_break_generate": (i32, [vp, P(DamBreak), vp, vp, vp, P(u32), vp, vp, vp]),
     "reyn_batch_remove": (i32, [vp, P(u32), u32]),
     "reyn_batch_scenes": (i32, [vp, P(u32), u32, P(u32)]),
+    "reyn_batch_scene_size": (i32, [vp, u32, P(u32)]),
     "reyn_batch_scene_bytes": (i32, [P(BatchParams), P(BatchScene), P(u64)]),
     "reyn_batch_step": (i32, [vp, u32]),
     "reyn_device_memory": (i32, [vp, P(u64), P(u64)]),

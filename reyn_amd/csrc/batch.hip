@@ -213,11 +213,12 @@ template <class T> int lalloc_n(reyn_ctx* ctx, Layout& L, T** p, size_t n)
     return lalloc(ctx, L, reinterpret_cast<void**>(p), n * sizeof(T));
 }
 
+// Copies host or device data to the device.
 int upload(reyn_ctx* ctx, void* dst, const void* src, size_t bytes)
 {
     if (bytes == 0)
         return REYN_OK;
-    REYN_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    REYN_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, ctx->stream));
     return REYN_OK;
 }
 
@@ -275,7 +276,7 @@ int check_params(reyn_ctx* ctx, const reyn_batch_params* prm)
     return REYN_OK;
 }
 
-// Bounding box of a scene's boundary samples.
+// Bounding box of a scene's boundary samples (host arrays).
 void bdy_box(const reyn_batch_scene& sc, reyn_float3* lo_out, reyn_float3* hi_out)
 {
     reyn_float3 lo { sc.bx[0], sc.by[0], sc.bz[0] }, hi = lo;
@@ -322,8 +323,21 @@ int make_scene(reyn_batch* b, const reyn_batch_scene& sc, SceneRes** out)
             const float* src[3] = { sc.bx, sc.by, sc.bz };
             for (int k = 0; k < 3 && rc == REYN_OK; ++k)
                 rc = upload(ctx, fields[k], src[k], nb * sizeof(float));
-            reyn_float3 lo, hi;
-            bdy_box(sc, &lo, &hi);
+            reyn_float3 lo {}, hi {};
+            if (!sc.device) {
+                bdy_box(sc, &lo, &hi);
+            } else { // device arrays: exact FP64 min/max reductions
+                float* lv[3] = { &lo.x, &lo.y, &lo.z };
+                float* hv[3] = { &hi.x, &hi.y, &hi.z };
+                for (int k = 0; k < 3 && rc == REYN_OK; ++k) {
+                    double mn = 0.0, mx = 0.0;
+                    rc = reyn_reduce_f32(ctx, fields[k], sc.n_bdy, 1, &mn);
+                    if (rc == REYN_OK)
+                        rc = reyn_reduce_f32(ctx, fields[k], sc.n_bdy, 2, &mx);
+                    *lv[k] = static_cast<float>(mn);
+                    *hv[k] = static_cast<float>(mx);
+                }
+            }
             if (rc == REYN_OK)
                 rc = reyn_grid_builder_create(ctx, lo, hi, rad, prm.cell_ratio, &r->bgrid);
             reyn_grid g {};
@@ -884,6 +898,76 @@ int reyn_batch_remove(reyn_batch* b, const uint32_t* ids, uint32_t count)
     REYN_TRY(relayout(b, next, old_slot, std::vector<const reyn_batch_scene*>(next.size(), nullptr)));
     for (SceneRes* r : gone)
         free_scene(r);
+    return REYN_OK;
+}
+
+int reyn_batch_add_dam_breaks(reyn_batch* b, const reyn_dam_break* d, uint32_t count, uint32_t* ids)
+{
+    if (!b || (count && !d))
+        return set_error(b ? b->ctx : nullptr, REYN_ERR_INVALID_ARGUMENT, "batch add: NULL argument");
+    reyn_ctx* ctx = b->ctx;
+    REYN_HIP(ctx, hipSetDevice(ctx->device));
+    std::vector<void*> mem;
+    std::vector<reyn_batch_scene> scenes(count);
+    int rc = REYN_OK;
+    for (uint32_t k = 0; k < count && rc == REYN_OK; ++k) {
+        uint32_t nl = 0, nb = 0, nf = 0;
+        rc = dam_break_sizes(ctx, &d[k], &nl, &nb);
+        void* p = nullptr;
+        if (rc == REYN_OK)
+            rc = check_hip(ctx, hipMalloc(&p, sizeof(float) * 3 * (size_t(nl) + nb)), "hipMalloc scene");
+        if (rc != REYN_OK)
+            break;
+        mem.push_back(p);
+        float* f = static_cast<float*>(p);
+        float *x = f, *y = f + nl, *z = f + 2 * size_t(nl), *bx = f + 3 * size_t(nl), *by = bx + nb, *bz = by + nb;
+        rc = dam_break_generate(ctx, &d[k], x, y, z, &nf, bx, by, bz);
+        const float s = static_cast<float>(d[k].spacing);
+        reyn_batch_scene& sc = scenes[k];
+        sc = reyn_batch_scene {};
+        sc.n = nf;
+        sc.x = x;
+        sc.y = y;
+        sc.z = z;
+        sc.mass = b->prm.rho_0 * (s * s * s);
+        sc.n_bdy = nb;
+        sc.bx = nb ? bx : nullptr;
+        sc.by = nb ? by : nullptr;
+        sc.bz = nb ? bz : nullptr;
+        sc.h = s;
+        sc.device = 1;
+        if (rc == REYN_OK && nb) { // bounds: the boundary samples' box
+            float* lo[3] = { &sc.bound_min.x, &sc.bound_min.y, &sc.bound_min.z };
+            float* hi[3] = { &sc.bound_max.x, &sc.bound_max.y, &sc.bound_max.z };
+            const float* a[3] = { bx, by, bz };
+            for (int c = 0; c < 3 && rc == REYN_OK; ++c) {
+                double mn = 0.0, mx = 0.0;
+                rc = reyn_reduce_f32(ctx, a[c], nb, 1, &mn);
+                if (rc == REYN_OK)
+                    rc = reyn_reduce_f32(ctx, a[c], nb, 2, &mx);
+                *lo[c] = static_cast<float>(mn);
+                *hi[c] = static_cast<float>(mx);
+            }
+        } else if (rc == REYN_OK) {
+            sc.bound_min = { 0.f, 0.f, 0.f };
+            sc.bound_max = { static_cast<float>(d[k].box[0]), static_cast<float>(d[k].box[1]),
+                static_cast<float>(d[k].box[2]) };
+        }
+    }
+    if (rc == REYN_OK)
+        rc = count ? add_scenes(b, scenes.data(), count, ids) : REYN_OK;
+    (void)hipStreamSynchronize(ctx->stream);
+    for (void* p : mem)
+        (void)hipFree(p);
+    return rc;
+}
+
+int reyn_batch_scene_size(const reyn_batch* b, uint32_t scene, uint32_t* n)
+{
+    const int s = (b && n) ? slot_of(b, scene) : -1;
+    if (s < 0)
+        return REYN_ERR_INVALID_ARGUMENT;
+    *n = b->scenes[s]->n;
     return REYN_OK;
 }
 

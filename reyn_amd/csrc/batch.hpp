@@ -47,4 +47,10 @@ void batch_read_control(const void* host, uint32_t nscenes, int solver, uint32_t
 // Copies every scene's iteration count of the finished loop to `iters_dev`.
 int batch_save_iterations(reyn_ctx* ctx, const BatchTarget& bt, uint32_t* iters_dev);
 
+// Device-side dam-break scenes (scene.hip): sizes and generation into device
+// arrays.
+int dam_break_sizes(reyn_ctx* ctx, const reyn_dam_break* d, uint32_t* n_lattice, uint32_t* n_bdy);
+int dam_break_generate(reyn_ctx* ctx, const reyn_dam_break* d, float* x, float* y, float* z, uint32_t* n_fluid,
+    float* bx, float* by, float* bz);
+
 } // namespace reyn

@@ -6,6 +6,11 @@ mesh scenes are outside the per-step path. These generators produce the
 lattice scenes the workloads are defined on: a lattice of spacing s with
 points at (i + 1/2) s, an optional uniform jitter, masses rho_0 s^3
 (scene.cu:14-37), and boundary samples at the same spacing.
+
+The device generator (`reyn_dam_break_generate`, reyn_amd/csrc/scene.hip)
+produces the same float bits: coordinates are formed in double in the same
+operations, the jitter is a counter hash of (seed, point, axis) rather than a
+random stream, and the wall shell is selected by lattice index.
 """
 
 from __future__ import annotations
@@ -15,9 +20,32 @@ from dataclasses import dataclass, field
 import numpy as np
 
 
+_M32 = np.uint64(0xFFFFFFFF)
+
+
+def _mix32(x: np.ndarray) -> np.ndarray:
+    x = x & _M32
+    x ^= x >> np.uint64(16)
+    x = (x * np.uint64(0x7FEB352D)) & _M32
+    x ^= x >> np.uint64(15)
+    x = (x * np.uint64(0x846CA68B)) & _M32
+    x ^= x >> np.uint64(16)
+    return x
+
+
+def hash_uniform(seed: int, idx: np.ndarray, axis: int) -> np.ndarray:
+    """U[0, 1) with 24 bits from a counter hash of (seed, point, axis), the
+    device generator's jitter source."""
+    s = _mix32(np.array([(int(seed) * 0x9E3779B9 + 0x7F4A7C15) & 0xFFFFFFFF], dtype=np.uint64))
+    key = (np.asarray(idx, dtype=np.uint64) * np.uint64(3) + np.uint64(axis)) & _M32
+    h = _mix32(s ^ key)
+    return (h >> np.uint64(8)).astype(np.float64) * (1.0 / 16777216.0)
+
+
 def lattice(counts, spacing: float, origin=(0.0, 0.0, 0.0), jitter: float = 0.0, seed: int = 0) -> np.ndarray:
     """Points origin + (i + 1/2) s for i in [0, counts) per axis, x fastest,
-    each coordinate moved by U(-jitter s, +jitter s)."""
+    each coordinate moved by U(-jitter s, +jitter s) drawn from
+    `hash_uniform(seed, point, axis)`."""
     nx, ny, nz = (int(c) for c in counts)
     s = float(spacing)
     ix = np.arange(nx, dtype=np.float64)
@@ -27,25 +55,30 @@ def lattice(counts, spacing: float, origin=(0.0, 0.0, 0.0), jitter: float = 0.0,
     pts = np.stack([x.ravel(), y.ravel(), z.ravel()], axis=1)
     pts = (pts + 0.5) * s + np.asarray(origin, dtype=np.float64)
     if jitter:
-        rng = np.random.default_rng(seed)
-        pts += rng.uniform(-jitter * s, jitter * s, size=pts.shape)
+        lo, hi = -(jitter * s), jitter * s
+        idx = np.arange(pts.shape[0], dtype=np.uint64)
+        for a in range(3):
+            pts[:, a] += lo + (hi - lo) * hash_uniform(seed, idx, a)
     return pts.astype(np.float32)
 
 
 def box_walls(lo, hi, spacing: float, layers: int = 2) -> np.ndarray:
     """Lattice points of spacing s in the shell of `layers` layers around the
-    box [lo, hi]: (i + 1/2) s offsets from lo - layers s, keeping the points
-    outside the box in at least one axis."""
+    box [lo, hi]: (i + 1/2) s offsets from lo - layers s on a lattice of
+    rint((hi - lo) / s) + 2 layers points per axis, keeping the points outside
+    its inner index box [layers, n - layers)^3 (x fastest)."""
     lo = np.asarray(lo, dtype=np.float64)
     hi = np.asarray(hi, dtype=np.float64)
     s = float(spacing)
     start = lo - layers * s
     counts = np.rint((hi - lo) / s).astype(int) + 2 * layers
-    axes = [start[a] + (np.arange(counts[a]) + 0.5) * s for a in range(3)]
-    z, y, x = np.meshgrid(axes[2], axes[1], axes[0], indexing="ij")
-    pts = np.stack([x.ravel(), y.ravel(), z.ravel()], axis=1)
-    inside = np.all((pts > lo) & (pts < hi), axis=1)
-    return pts[~inside].astype(np.float32)
+    idx = [np.arange(counts[a]) for a in range(3)]
+    iz, iy, ix = np.meshgrid(idx[2], idx[1], idx[0], indexing="ij")
+    ii = np.stack([ix.ravel(), iy.ravel(), iz.ravel()], axis=1)
+    inside = np.all((ii >= layers) & (ii < counts - layers), axis=1)
+    ii = ii[~inside]
+    pts = np.stack([start[a] + (ii[:, a].astype(np.float64) + 0.5) * s for a in range(3)], axis=1)
+    return pts.astype(np.float32)
 
 
 def sphere_shells(center, radius: float, spacing: float, layers: int = 2) -> np.ndarray:
@@ -91,7 +124,8 @@ class Scene:
 
     @property
     def mass(self) -> float:
-        return float(np.float32(self.rho_0) * np.float32(self.spacing) ** 3)
+        s = np.float32(self.spacing)
+        return float(np.float32(self.rho_0) * (s * s * s))
 
     @property
     def bound_min(self) -> np.ndarray:
@@ -112,7 +146,7 @@ def dam_break(fluid_counts, spacing: float, box_hi, rho_0: float = 1000.0, jitte
     hi = np.asarray(box_hi, dtype=np.float64)
     xb = box_walls(lo, hi, spacing, layers)
     meta = {"fluid_counts": tuple(int(c) for c in fluid_counts), "box": tuple(float(v) for v in hi),
-            "jitter": jitter, "seed": seed}
+            "jitter": jitter, "seed": seed, "layers": layers}
     if sphere is not None:
         center, radius = sphere
         sb = sphere_shells(center, radius, spacing, layers)
@@ -122,6 +156,27 @@ def dam_break(fluid_counts, spacing: float, box_hi, rho_0: float = 1000.0, jitte
         xb = np.concatenate([xb, sb])
         meta["sphere"] = (tuple(float(v) for v in center), float(radius))
     return Scene(x, xb, float(spacing), float(rho_0), lo, hi, meta)
+
+
+def dam_break_desc(sc: Scene):
+    """The `reyn_dam_break` description of a scene made by `dam_break`: the
+    device generator makes the same scene from it."""
+    from . import capi
+
+    m = sc.meta
+    d = capi.DamBreak()
+    d.counts[:] = [int(c) for c in m["fluid_counts"]]
+    d.spacing = float(sc.spacing)
+    d.box[:] = [float(v) for v in m["box"]]
+    d.jitter = float(m["jitter"])
+    d.seed = int(m["seed"])
+    d.layers = int(m.get("layers", 2))
+    if "sphere" in m:
+        center, radius = m["sphere"]
+        d.sphere = 1
+        d.center[:] = [float(v) for v in center]
+        d.radius = float(radius)
+    return d
 
 
 def config1_sesph() -> Scene:

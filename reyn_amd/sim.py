@@ -422,6 +422,30 @@ def batch_scenes(scenes, keep: list):
     return arr
 
 
+def dam_break_sizes(desc: capi.DamBreak) -> tuple[int, int]:
+    """Lattice size and boundary count of a `reyn_dam_break` (host only)."""
+    nl, nb = C.c_uint32(), C.c_uint32()
+    rc = capi.lib().reyn_dam_break_sizes(C.byref(desc), C.byref(nl), C.byref(nb))
+    if rc != 0:
+        raise capi.ReynError(rc, "dam break sizes")
+    return int(nl.value), int(nb.value)
+
+
+def generate_dam_break(ctx: Context, desc: capi.DamBreak) -> tuple[np.ndarray, np.ndarray]:
+    """Generates a dam break on the device; returns its fluid and boundary
+    positions as host (n, 3) float32 arrays."""
+    nl, nb = dam_break_sizes(desc)
+    fl = [ctx.empty(max(1, nl)) for _ in range(3)]
+    bd = [ctx.empty(max(1, nb)) for _ in range(3)]
+    nf = C.c_uint32()
+    ctx.check(ctx._lib.reyn_dam_break_generate(ctx.h, C.byref(desc), *[a.ptr for a in fl], C.byref(nf),
+                                               *[a.ptr for a in bd]), "dam break generate")
+    n = int(nf.value)
+    x = np.stack([a.numpy()[:n] for a in fl], axis=1)
+    xb = np.stack([a.numpy()[:nb] for a in bd], axis=1) if nb else np.zeros((0, 3), np.float32)
+    return x, xb
+
+
 def scene_bytes(ctx: Context, prm: capi.BatchParams, sc) -> int:
     """Device bytes the scene occupies while resident in a batch (`reyn_batch_scene_bytes`)."""
     keep = []
@@ -468,6 +492,21 @@ class Batch:
         self.ctx.check(self.ctx._lib.reyn_batch_add(self.h, arr, len(scenes), ids), "batch add")
         out = list(ids)
         self.scenes.update(zip(out, [s.scene if isinstance(s, Checkpoint) else s for s in scenes]))
+        return out
+
+    def add_dam_breaks(self, scenes) -> list[int]:
+        """Adds scenes made by `scenes.dam_break`, generated on the device
+        from their descriptions (`reyn_batch_add_dam_breaks`)."""
+        from .scenes import dam_break_desc
+
+        scenes = list(scenes)
+        if not scenes:
+            return []
+        arr = (capi.DamBreak * len(scenes))(*[dam_break_desc(sc) for sc in scenes])
+        ids = (C.c_uint32 * len(scenes))()
+        self.ctx.check(self.ctx._lib.reyn_batch_add_dam_breaks(self.h, arr, len(scenes), ids), "batch add dam breaks")
+        out = list(ids)
+        self.scenes.update(zip(out, scenes))
         return out
 
     def checkpoint(self, scene: int):

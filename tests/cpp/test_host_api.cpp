@@ -289,6 +289,47 @@ int main()
         CHECK(mix.info(0).iterations == alone_a.info(0).iterations && mix.info(0).dt == alone_a.info(0).dt);
     });
 
+    run("SceneBatch: a dam break generated on the device steps like its host twin", [] {
+        reyn_dam_break d {};
+        d.counts[0] = 12;
+        d.counts[1] = 10;
+        d.counts[2] = 8;
+        d.spacing = 0.05;
+        d.box[0] = 1.2;
+        d.box[1] = 0.8;
+        d.box[2] = 0.6;
+        d.jitter = 0.01;
+        d.seed = 9;
+        d.layers = 2;
+        uint32_t nl = 0, nb = 0;
+        CHECK(reyn_dam_break_sizes(&d, &nl, &nb) == REYN_OK);
+        CHECK(nl == 960u && nb > 0u);
+        reyn_batch_params prm {};
+        prm.solver = REYN_SOLVER_PCISPH;
+        prm.kernel = REYN_KERNEL_W6;
+        prm.nu = 0.001f;
+        prm.rho_0 = 1000.f;
+        prm.g = v3(0.f, -9.81f, 0.f);
+        prm.cell_ratio = 1.f;
+        prm.dt0 = 5e-4f;
+        prm.cfl_lambda = 1.f;
+        prm.change_factor = 0.001f;
+        prm.iter_target = 10;
+        prm.eta_rho_max = 0.001f;
+        prm.min_iter = 3;
+        prm.max_iter = 100;
+        SceneBatch b(prm);
+        const uint32_t id = b.add_dam_breaks({ d })[0];
+        b.step(4);
+        const BatchSceneState st = b.download(id);
+        CHECK(st.x.size() == nl);
+        CHECK(b.info(id).steps == 4 && b.info(id).iterations >= 3);
+        bool finite = true;
+        for (float v : st.y)
+            finite = finite && std::isfinite(v);
+        CHECK(finite);
+    });
+
     run("save_to_ply writes the reference's binary PLY layout", [] {
         std::vector<float> x { 1.f, 2.f, 3.f }, y { -1.f, 0.5f, 7.f }, z { 0.f, 1e-3f, 2.f };
         DeviceBuffer<float> xs(1), ys(1), zs(1);

@@ -59,7 +59,7 @@ STRUCTS = {
     "SphParams": "reyn_sph_params", "SesphParams": "reyn_sesph_params", "SesphWork": "reyn_sesph_work",
     "PcisphParams": "reyn_pcisph_params", "PcisphWork": "reyn_pcisph_work", "SolveInfo": "reyn_solve_info",
     "IisphParams": "reyn_iisph_params", "IisphWork": "reyn_iisph_work", "BatchSceneInfo": "reyn_batch_scene_info",
-    "BatchScene": "reyn_batch_scene", "BatchParams": "reyn_batch_params",
+    "BatchScene": "reyn_batch_scene", "BatchParams": "reyn_batch_params", "DamBreak": "reyn_dam_break",
 }
 
 
