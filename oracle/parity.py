@@ -39,13 +39,13 @@ class GpuCase:
     """Uploads a scene, builds the boundary (GPU Akinci masses) and the
     fluid grid with reordering, and keeps host copies of every input."""
 
-    def __init__(self, ctx, sc, v, cell_ratio=0.5, with_bdy=True):
+    def __init__(self, ctx, sc, v, cell_ratio=0.5, with_bdy=True, masses=None):
         self.ctx = ctx
         self.sc = sc
         h = sc.spacing
         self.h = h
         self.r = 2.0 * h
-        self.state = Particles(ctx, sc.x, v, sc.mass, sc.rho_0)
+        self.state = Particles(ctx, sc.x, v, sc.mass if masses is None else masses, sc.rho_0)
         self.bdy = BoundarySamples(ctx, sc.xb, h, sc.rho_0, cell_ratio) if with_bdy else None
         self.builder = UniformGridBuilder(ctx, sc.bound_min, sc.bound_max, self.r, cell_ratio)
         self.cell_ratio = cell_ratio
@@ -86,9 +86,16 @@ def sph_oracle(kernel: int, h: float, rho0: float):
 
 
 def run_iisph_case(ctx, kernel="B3", dt=2e-3, side=14, warm=True, stop_rule=capi.IISPH_STOP_ANY, eta=0.005,
-                   min_iter=3, max_iter=0, seed=7, p0_scale=200.0, cell_ratio=0.5, compress=0.97):
+                   min_iter=3, max_iter=0, seed=7, p0_scale=200.0, cell_ratio=0.5, compress=0.97, mass_spread=0.0):
+    """One IISPH step on the GPU against the FP64 oracle from the same inputs;
+    `mass_spread` > 0 gives every particle its own mass, m (1 + U(-spread,
+    spread))."""
     sc, v = make_case(side=side, seed=seed, compress=compress)
-    case = GpuCase(ctx, sc, v, cell_ratio)
+    masses = None
+    if mass_spread:
+        u = np.random.default_rng(seed + 5).uniform(-mass_spread, mass_spread, sc.n)
+        masses = (sc.mass * (1.0 + u)).astype(np.float32)
+    case = GpuCase(ctx, sc, v, cell_ratio, masses=masses)
     solver = IISPH(ctx, kernel, sc.n, NU, sc.spacing, sc.rho_0, eta_rho_max=eta, min_iter=min_iter,
                    max_iter=max_iter, stop_rule=stop_rule, warm_start=warm)
     p0 = np.abs(np.random.default_rng(seed + 2).normal(0.0, p0_scale, sc.n)).astype(np.float32)

@@ -25,7 +25,7 @@ struct SolveCtl {
     uint32_t iter;
     uint32_t done;
     uint32_t n_violating;
-    uint32_t pad;
+    uint32_t mass_mixed; // 1: the scene's particle masses are not all equal
     double mean_rel_error;
     double avg_density;
 };
@@ -162,18 +162,26 @@ __global__ __launch_bounds__(kBlock) void k_pack_bdy(BdyDev b, float4* __restric
 
 // P4[i] = {x_i, y_i, z_i, m_i}, the records the density pass walks, and the
 // interleaved records X of the later loops with u = v_i (see `aux4`).
+// mflag[workgroup] = 1 when one of its particles has a mass other than the
+// scene's first particle's.
 template <bool BT>
 __global__ __launch_bounds__(kBlock) void k_pack(SceneArg sa, FluidDev f, float4* __restrict__ P4,
-    float4* __restrict__ X)
+    float4* __restrict__ X, uint32_t* __restrict__ mflag)
 {
     REYN_SCENE(BT, sa);
-    if (!live)
-        return;
-    const uint32_t i = t;
-    const float4 r = make_float4(f.x[i], f.y[i], f.z[i], f.m[i]);
-    P4[i] = r;
-    X[2 * size_t(i)] = r;
-    X[2 * size_t(i) + 1] = make_float4(f.vx[i], f.vy[i], f.vz[i], 0.f);
+    bool mixed = false;
+    if (live) {
+        const uint32_t i = t;
+        const float m = f.m[i];
+        const float4 r = make_float4(f.x[i], f.y[i], f.z[i], m);
+        P4[i] = r;
+        X[2 * size_t(i)] = r;
+        X[2 * size_t(i) + 1] = make_float4(f.vx[i], f.vy[i], f.vz[i], 0.f);
+        mixed = !(m == f.m[sc.first]);
+    }
+    const int any = __syncthreads_or(mixed ? 1 : 0);
+    if (threadIdx.x == 0)
+        mflag[lb_] = any ? 1u : 0u;
 }
 
 // Densities; with GB also the boundary sums, and with a list in L the
@@ -501,13 +509,15 @@ __device__ __forceinline__ float jacobi_update(float omega, float aii, float Ap,
     return (aii <= lim && aii >= -lim) ? 0.f : fmaxf(0.f, omega * (s - Ap) / aii);
 }
 
-// PK: u of X holds the updated velocities.
+// PK: u of X holds the updated velocities, and the pressure goes to w of P4
+// as well (after the density pass the records P4 serve the pressure
+// accelerations only, which read {x, p} from them).
 template <int K, bool IND, bool PK, bool BT>
 __global__ __launch_bounds__(kBlock) REYN_OCC(REYN_OCC_LOOP) void k_iisph_aii_s(SceneArg sa, FluidDev f,
     float omega, int warm, const float* __restrict__ rho,
     const float* __restrict__ gbx, const float* __restrict__ gby, const float* __restrict__ gbz,
     float* __restrict__ aii, float* __restrict__ si, float* __restrict__ p, NbrDev L,
-    float4* __restrict__ X)
+    float4* __restrict__ X, float4* __restrict__ P4)
 {
     REYN_SCENE(BT, sa);
     if (!live)
@@ -541,19 +551,24 @@ __global__ __launch_bounds__(kBlock) REYN_OCC(REYN_OCC_LOOP) void k_iisph_aii_s(
     const float s = (rho0 - rho[i]) / dt - (div.value() + dot(vi, gb));
     aii[i] = a;
     si[i] = s;
-    p[i] = warm ? p[i] * 0.5f : jacobi_update(omega, a, 0.f, s);
+    const float pn = warm ? p[i] * 0.5f : jacobi_update(omega, a, 0.f, s);
+    p[i] = pn;
+    if (PK)
+        reinterpret_cast<float*>(P4 + i)[3] = pn;
 }
 
-// Pressure accelerations. PK: the loop reads the dense records P4 and the
-// pressure array (4 + 16 bytes per neighbour in two dense arrays touch fewer
-// cache lines than the interleaved records) and, inside the solver loop
-// (ctl != NULL), writes the result to u of X for the Jacobi pass; the final
-// pass and the SoA mode write the work arrays.
+// Pressure accelerations. PK: the listed loop reads {x, y, z, p} of each
+// neighbour from the records P4 (w holds the pressure after the a_ii pass),
+// one 16-byte load per neighbour when the scene's masses are all equal
+// (`mctl`'s flag; m_j is then the particle's own mass) and a second load of
+// m_j otherwise; inside the solver loop (ctl != NULL) it writes the result to
+// u of X for the Jacobi pass; the final pass and the SoA mode write the work
+// arrays.
 template <int K, bool IND, bool PK, bool BT>
 __global__ __launch_bounds__(kBlock) REYN_OCC(REYN_OCC_PACC) void k_iisph_pacc(SceneArg sa, FluidDev f,
     const float* __restrict__ p, const float* __restrict__ gbx,
     const float* __restrict__ gby, const float* __restrict__ gbz, float* __restrict__ ax,
-    float* __restrict__ ay, float* __restrict__ az, const SolveCtl* ctl, NbrDev L,
+    float* __restrict__ ay, float* __restrict__ az, const SolveCtl* ctl, const SolveCtl* mctl, NbrDev L,
     const float4* __restrict__ P4, float4* __restrict__ X)
 {
     REYN_SCENE(BT, sa);
@@ -567,9 +582,37 @@ __global__ __launch_bounds__(kBlock) REYN_OCC(REYN_OCC_PACC) void k_iisph_pacc(S
     const f3 xi = ld3(f.x, f.y, f.z, i);
     const float pi = p[i];
     f3 acc { 0.f, 0.f, 0.f };
-    nbrs4<IND, kNbrBatch<K>>(g, L, t, xi, P4, f.x, f.y, f.z, f.m, [&](uint32_t j, f3 xij, float d2, float mj) {
-        acc += (mj * (pi + gat(p, j))) * kdW<K>(kp, xij, d2);
-    });
+    const uint32_t c = (PK && L.cnt) ? L.cnt[t] : 0xffffffffu;
+    if (PK && c <= L.cap) {
+        const bool uniform = !mctl[sc.idx].mass_mixed; // the same for the whole scene
+        const float mi = f.m[i];
+        constexpr uint32_t B = kNbrBatch<K>;
+        uint32_t off = t;
+        for (uint32_t k = 0; k < c; k += B, off += B * L.n) {
+            uint32_t j[B];
+            float4 q[B];
+            float m[B];
+#pragma unroll
+            for (uint32_t u = 0; u < B; ++u) // past the count: row k again (valid)
+                j[u] = gat(L.idx, (k + u < c) ? off + u * L.n : off);
+#pragma unroll
+            for (uint32_t u = 0; u < B; ++u)
+                q[u] = gat(P4, j[u]);
+#pragma unroll
+            for (uint32_t u = 0; u < B; ++u)
+                m[u] = uniform ? mi : gat(f.m, j[u]);
+#pragma unroll
+            for (uint32_t u = 0; u < B; ++u) {
+                const f3 xij = xi - f3 { q[u].x, q[u].y, q[u].z };
+                const float mj = (k + u < c) ? m[u] : 0.f;
+                acc += (mj * (pi + q[u].w)) * kdW<K>(kp, xij, dot(xij, xij));
+            }
+        }
+    } else {
+        walk<IND>(g, xi, f.x, f.y, f.z, [&](uint32_t j, f3 xij, float d2) {
+            acc += (gat(f.m, j) * (pi + gat(p, j))) * kdW<K>(kp, xij, d2);
+        });
+    }
     if (bdy)
         acc += pi * ld3(gbx, gby, gbz, i);
     const f3 a = -rho0_sq_inv * acc;
@@ -579,14 +622,16 @@ __global__ __launch_bounds__(kBlock) REYN_OCC(REYN_OCC_PACC) void k_iisph_pacc(S
         st3(ax, ay, az, i, a);
 }
 
-// PK: the accelerations come from u of X; otherwise from the SoA work arrays.
+// PK: the accelerations come from u of X, and the new pressure goes to w of
+// P4 as well; otherwise the SoA work arrays.
 template <int K, bool IND, bool PK, bool BT>
 __global__ __launch_bounds__(kBlock) REYN_OCC(REYN_OCC_LOOP) void k_iisph_jacobi(SceneArg sa, FluidDev f,
     float omega, const float* __restrict__ ax, const float* __restrict__ ay,
     const float* __restrict__ az, const float* __restrict__ aii,
     const float* __restrict__ si, const float* __restrict__ gbx, const float* __restrict__ gby,
     const float* __restrict__ gbz, float* __restrict__ p, double* __restrict__ part_err,
-    uint32_t* __restrict__ part_cnt, const SolveCtl* ctl, NbrDev L, float4* __restrict__ X)
+    uint32_t* __restrict__ part_cnt, const SolveCtl* ctl, NbrDev L, float4* __restrict__ X,
+    float4* __restrict__ P4)
 {
     __shared__ double lds[kBlock / kWave];
     REYN_SCENE(BT, sa);
@@ -619,7 +664,10 @@ __global__ __launch_bounds__(kBlock) REYN_OCC(REYN_OCC_LOOP) void k_iisph_jacobi
             lap += dot(ai, ld3(gbx, gby, gbz, i));
         const float Ap = dt * lap;
         const float s = si[i];
-        p[i] = p[i] + jacobi_update(omega, aii[i], Ap, s);
+        const float pn = p[i] + jacobi_update(omega, aii[i], Ap, s);
+        p[i] = pn;
+        if (PK)
+            reinterpret_cast<float*>(P4 + i)[3] = pn;
         const float err = (Ap - s) * dt;
         viol = err > thr ? 1u : 0u;
         err_pos = err > 0.f ? double(err) : 0.0;
@@ -669,15 +717,25 @@ __global__ __launch_bounds__(kBlock) void k_ctl_iisph(SceneArg sa, SolveCtl* ctl
     }
 }
 
-__global__ void k_ctl_reset(SolveCtl* ctl, uint32_t count)
+// One workgroup per scene: resets the scene's loop control and reduces the
+// per-workgroup mass flags of the pack pass (NULL: treated as mixed).
+template <bool BT>
+__global__ __launch_bounds__(kBlock) void k_ctl_reset(SceneArg sa, SolveCtl* ctl, const uint32_t* __restrict__ mflag)
 {
-    for (uint32_t k = threadIdx.x; k < count; k += blockDim.x) {
-        ctl[k].iter = 0;
-        ctl[k].done = 0;
-        ctl[k].n_violating = 0;
-        ctl[k].pad = 0;
-        ctl[k].mean_rel_error = 0.0;
-        ctl[k].avg_density = 0.0;
+    const SceneDev& sc = scene_of_ctl<BT>(sa);
+    uint32_t mixed = mflag ? 0u : 1u;
+    if (mflag)
+        for (uint32_t k = threadIdx.x; k < sc.nblk; k += blockDim.x)
+            mixed |= mflag[sc.blk_first + k];
+    const int any = __syncthreads_or(mixed ? 1 : 0);
+    if (threadIdx.x == 0) {
+        SolveCtl& c = ctl[sc.idx];
+        c.iter = 0;
+        c.done = 0;
+        c.n_violating = 0;
+        c.mass_mixed = any ? 1u : 0u;
+        c.mean_rel_error = 0.0;
+        c.avg_density = 0.0;
     }
 }
 
@@ -756,7 +814,7 @@ int launch_density(reyn_ctx* ctx, int kernel, const Common& c0, float* rho, cons
     if (s && s->P4) {
         {
             ProfScope _prof(ctx, "k_pack");
-            k_pack<BT><<<c.nb, kBlock, 0, ctx->stream>>>(c.a, c.f, s->P4, s->X);
+            k_pack<BT><<<c.nb, kBlock, 0, ctx->stream>>>(c.a, c.f, s->P4, s->X, s->part_u);
         }
         REYN_LAUNCHED(ctx, "k_pack");
     }
@@ -816,14 +874,15 @@ int launch_integrate(reyn_ctx* ctx, const Common& c, const float* ax, const floa
 // Enqueue loop iterations in chunks until every scene's control block
 // reports done; `nscenes` control blocks at `ctl`, copied into `out`.
 template <class Iter>
-int run_loop(reyn_ctx* ctx, SolveCtl* ctl, uint32_t nscenes, uint32_t min_iter, uint32_t max_iter,
-    uint32_t check_every, Iter&& iter, SolveCtl* out)
+int run_loop(reyn_ctx* ctx, const Common& c, const Scratch& s, uint32_t nscenes, uint32_t min_iter,
+    uint32_t max_iter, uint32_t check_every, Iter&& iter, SolveCtl* out)
 {
+    SolveCtl* ctl = s.ctl;
     SolveCtl* host = nullptr;
     REYN_TRY(ctx_pinned(ctx, sizeof(SolveCtl) * nscenes, reinterpret_cast<void**>(&host)));
     {
         ProfScope _prof(ctx, "k_ctl_reset");
-        k_ctl_reset<<<1, kBlock, 0, ctx->stream>>>(ctl, nscenes);
+        k_ctl_reset<false><<<1, kBlock, 0, ctx->stream>>>(c.a, ctl, s.P4 ? s.part_u : nullptr);
     }
     REYN_LAUNCHED(ctx, "k_ctl_reset");
     const uint32_t chunk = check_every ? check_every : 4;
@@ -940,7 +999,7 @@ int pcisph_run(reyn_ctx* ctx, int kernel, const Common& c, const reyn_pcisph_wor
 {
     REYN_TRY(pcisph_begin<BT>(ctx, kernel, c, work, s));
     REYN_TRY(run_loop(
-        ctx, s.ctl, nscenes, min_iter, max_iter, 4,
+        ctx, c, s, nscenes, min_iter, max_iter, 4,
         [&]() -> int { return pcisph_iteration<BT>(ctx, kernel, c, work, s, nscenes, min_iter, max_iter); },
         res));
     return pcisph_end<BT>(ctx, c, work);
@@ -967,7 +1026,7 @@ int iisph_pacc(reyn_ctx* ctx, int kind, const Common& c, const reyn_iisph_work* 
                 {
                     ProfScope _prof(ctx, "k_iisph_pacc");
                     k_iisph_pacc<K, I, PK, BT><<<c.nb, kBlock, 0, ctx->stream>>>(c.a, c.f, work->p, s.gbx, s.gby,
-                        s.gbz, work->ax, work->ay, work->az, ctl, s.L, s.P4, s.X);
+                        s.gbz, work->ax, work->ay, work->az, ctl, s.ctl, s.L, s.P4, s.X);
                 }
                 REYN_LAUNCHED(ctx, "k_iisph_pacc");
                 return REYN_OK;
@@ -994,7 +1053,7 @@ int iisph_begin(reyn_ctx* ctx, int kind, const Common& c, const IisphLoop& lp, c
                 {
                     ProfScope _prof(ctx, "k_iisph_aii_s");
                     k_iisph_aii_s<K, I, PK, BT><<<c.nb, kBlock, 0, ctx->stream>>>(c.a, c.f, lp.omega, lp.warm,
-                        work->rho, s.gbx, s.gby, s.gbz, work->a_ii, work->s_i, work->p, s.L, s.X);
+                        work->rho, s.gbx, s.gby, s.gbz, work->a_ii, work->s_i, work->p, s.L, s.X, s.P4);
                 }
                 REYN_LAUNCHED(ctx, "k_iisph_aii_s");
                 return REYN_OK;
@@ -1022,7 +1081,7 @@ int iisph_iteration(reyn_ctx* ctx, int kind, const Common& c, const IisphLoop& l
                     ProfScope _prof(ctx, "k_iisph_jacobi");
                     k_iisph_jacobi<K, I, PK, BT><<<c.nb, kBlock, 0, ctx->stream>>>(c.a, c.f, lp.omega, work->ax,
                         work->ay, work->az, work->a_ii, work->s_i, s.gbx, s.gby, s.gbz, work->p, s.part_d,
-                        s.part_u, s.ctl, s.L, s.X);
+                        s.part_u, s.ctl, s.L, s.X, s.P4);
                 }
                 REYN_LAUNCHED(ctx, "k_iisph_jacobi");
                 return REYN_OK;
@@ -1051,7 +1110,7 @@ int iisph_run(reyn_ctx* ctx, int kind, const Common& c, const IisphLoop& lp, con
 {
     REYN_TRY(iisph_begin<BT>(ctx, kind, c, lp, work, s));
     REYN_TRY(run_loop(
-        ctx, s.ctl, nscenes, lp.min_iter, lp.max_iter, lp.check_every,
+        ctx, c, s, nscenes, lp.min_iter, lp.max_iter, lp.check_every,
         [&]() -> int { return iisph_iteration<BT>(ctx, kind, c, lp, work, s, nscenes); }, res));
     return iisph_end<BT>(ctx, kind, c, work, s);
 }
@@ -1136,7 +1195,7 @@ int batch_phase(reyn_ctx* ctx, const BatchTarget& bt, const BatchSolver& bs, int
         REYN_TRY(pc ? pcisph_begin<true>(ctx, bs.kernel, c, &pw, s) : iisph_begin<true>(ctx, bs.kernel, c, lp, &iw, s));
         {
             ProfScope _prof(ctx, "k_ctl_reset");
-            k_ctl_reset<<<1, kBlock, 0, ctx->stream>>>(s.ctl, bt.nscenes);
+            k_ctl_reset<true><<<bt.nscenes, kBlock, 0, ctx->stream>>>(c.a, s.ctl, s.part_u);
         }
         REYN_LAUNCHED(ctx, "k_ctl_reset");
         [[fallthrough]];

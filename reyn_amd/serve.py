@@ -17,6 +17,10 @@ grid, time-step controller and bounds (main.cu:59-78).
 `run_fifo` is the one-job-at-a-time baseline on the same arrivals: each job
 alone through the same kernels, in arrival order, with its service time
 measured and its time-to-result taken on the arrival clock.
+
+With `device_setup`, a job is a description (`scenes.dam_break_desc` of its
+scene) and both the server and the baseline build it on the GPU when it is
+admitted, so its set-up is part of its time-to-result.
 """
 
 from __future__ import annotations
@@ -101,8 +105,10 @@ class Server:
     server starts; `max_resident` bounds the resident jobs."""
 
     def __init__(self, ctx: Context, budget_bytes: int | None = None, reserve_bytes: int = RESERVE_BYTES,
-                 max_resident: int = 512, keep_results: bool = True, **batch_options):
+                 max_resident: int = 512, keep_results: bool = True, device_setup: bool = False,
+                 **batch_options):
         self.ctx = ctx
+        self.device_setup = device_setup
         self.options = dict(batch_options)
         self.batch = Batch(ctx, [], **self.options)
         self.prm = self.batch.prm
@@ -150,7 +156,8 @@ class Server:
                 self.used += j.bytes
                 admit.append(j)
             if admit:
-                ids = self.batch.add([j.scene for j in admit])
+                scs = [j.scene for j in admit]
+                ids = self.batch.add_dam_breaks(scs) if self.device_setup else self.batch.add(scs)
                 self.layout_changes += 1
                 t_adm = clock() - t0
                 for j, bid in zip(admit, ids):
@@ -198,7 +205,7 @@ class Server:
 
 
 def run_fifo(ctx: Context, jobs: list[Job], keep_results: bool = True, clock=time.perf_counter,
-             **batch_options) -> dict:
+             device_setup: bool = False, **batch_options) -> dict:
     """One job at a time in arrival order, each alone (a batch of one: the
     same bits as the single-scene entry points) for all its steps. Service
     times are measured; a job starts when it has arrived and the previous job
@@ -208,7 +215,11 @@ def run_fifo(ctx: Context, jobs: list[Job], keep_results: bool = True, clock=tim
     busy = 0.0
     for j in jobs:
         t = clock()
-        b = Batch(ctx, [j.scene], **batch_options)
+        if device_setup:
+            b = Batch(ctx, [], rho_0=j.scene.rho_0, **{k: v for k, v in batch_options.items() if k != "rho_0"})
+            b.add_dam_breaks([j.scene])
+        else:
+            b = Batch(ctx, [j.scene], **batch_options)
         b.step(j.steps)
         if keep_results:
             j.result = b.download(0)

@@ -103,6 +103,17 @@ def test_iisph_step_parity_cell_ratios(ctx, ratio):
     assert errs["a_ii"] <= TOL_DIFF and errs["s_i"] <= TOL_DIFF and errs["v"] <= TOL_DIFF, errs
 
 
+@pytest.mark.parametrize("ratio", [0.5, 1.0])
+def test_iisph_step_parity_mixed_masses(ctx, ratio):
+    # particle masses spread by 10%: the pressure accelerations gather m_j per
+    # neighbour instead of taking the scene's one mass
+    errs, gpu, ref = parity.run_iisph_case(ctx, cell_ratio=ratio, mass_spread=0.1)
+    assert gpu["iterations"] >= 3
+    assert errs["rho"] <= TOL and errs["p"] <= TOL and errs["x"] <= TOL, errs
+    assert errs["a_ii"] <= TOL_DIFF and errs["s_i"] <= TOL_DIFF and errs["v"] <= TOL_DIFF, errs
+    assert errs["dx"] <= TOL_DIFF, errs
+
+
 def test_list_cap_overflow_parity(ctx):
     # at 0.8 s spacing a particle has about 64 neighbours, around the list
     # capacity: part of the particles keep a list, the rest walk the grid in
