@@ -228,11 +228,11 @@ def serve_leg(ctx, args, log) -> dict:
     arrivals = poisson_arrivals(len(spec), rate, seed=5)
     log(f"serving: capacity {cap / 1e6:.1f} M particle-steps/s, {rate:.2f} arrivals/s")
     served = make_jobs(spec, arrivals)
-    srv = Server(ctx, **opts)
+    srv = Server(ctx, device_setup=True, **opts)
     rep = srv.run(served)
     srv.close()
     alone = make_jobs(spec, arrivals)
-    fifo = run_fifo(ctx, alone, **opts)
+    fifo = run_fifo(ctx, alone, device_setup=True, **opts)
     same, diff = same_results(served, alone)
     log(f"serving: p50 {rep['p50_s']:.2f} s / p99 {rep['p99_s']:.2f} s served, p50 {fifo['p50_s']:.2f} s / "
         f"p99 {fifo['p99_s']:.2f} s FIFO; {rep['particle_steps_per_s'] / 1e6:.1f} vs "
@@ -240,7 +240,8 @@ def serve_leg(ctx, args, log) -> dict:
     return {
         "workload": f"{len(spec)} IISPH dam-break jobs (seed 5): N log-uniform in [5k, 500k], box sides U[1,3], "
                     "sphere obstacle with p 0.5, steps U{200..2000}, B3, stop at mean error <= 0.1% or 100 "
-                    f"iterations; Poisson arrivals at {args.serve_load:.0%} of the measured batch capacity",
+                    f"iterations; Poisson arrivals at {args.serve_load:.0%} of the measured batch capacity; each "
+                    "job is built on the GPU from its description when admitted (in its time-to-result)",
         "capacity_particle_steps_per_s": round(cap, 1), "arrival_rate_per_s": round(rate, 4),
         "served": rep, "fifo_one_at_a_time": fifo,
         "p50_speedup": round(fifo["p50_s"] / rep["p50_s"], 3),

@@ -130,14 +130,17 @@ __device__ __forceinline__ bool loop_done(const SolveCtl* ctl, const SceneDev& s
 //   k_pack                      u = v,  w = 0
 //   k_density                   w = rho
 //   k_integrate (velocity only) u = the updated velocity
-//   k_predict_rho (PCISPH)      w = the predicted density
 //   k_iisph_pacc (loop)         u = the pressure acceleration
 // A neighbour's record and auxiliary slot share one 32-byte block, so the two
 // gathers of a neighbour touch one cache line. A phase writes only the
 // component that no concurrent thread of the same phase reads.
+//
+// The dense records P4 = {x, y, z, m} serve the density pass; after it, w of
+// P4 carries the pressure (IISPH: k_iisph_aii_s, k_iisph_jacobi) or the
+// predicted density (PCISPH: k_predict_rho), so the pressure-acceleration
+// passes read one 16-byte record per neighbour.
 __device__ __forceinline__ float4 aux4(const float4* __restrict__ X, uint32_t j) { return gat(X, 2 * j + 1); }
 __device__ __forceinline__ f3 aux3(const float4* __restrict__ X, uint32_t j) { return gatb<f3>(X, 32 * j + 16); }
-__device__ __forceinline__ float auxw(const float4* __restrict__ X, uint32_t j) { return gatb<float>(X, 32 * j + 28); }
 __device__ __forceinline__ void put_aux3(float4* __restrict__ X, uint32_t i, f3 u)
 {
     float* q = reinterpret_cast<float*>(X + 2 * size_t(i) + 1);
@@ -393,12 +396,14 @@ __global__ __launch_bounds__(kBlock) void k_sesph_acc(SceneArg sa, FluidDev f, f
 // ---- PCISPH ----------------------------------------------------------------------
 
 // predicted density; per-block FP64 partial sums feed the mean-density test
-// PK: u of X holds the updated velocities and w receives the prediction.
+// PK: u of X holds the updated velocities, and w of P4 receives the
+// prediction (after the density pass P4 serves the pressure accelerations
+// only, which read {x, rho} from it).
 template <int K, bool IND, bool PK, bool BT>
 __global__ __launch_bounds__(kBlock) void k_predict_rho(SceneArg sa, FluidDev f,
     const float* __restrict__ gbx, const float* __restrict__ gby,
     const float* __restrict__ gbz, const float* __restrict__ rb, float* __restrict__ rho,
-    double* __restrict__ part, const SolveCtl* ctl, NbrDev L, float4* __restrict__ X)
+    double* __restrict__ part, const SolveCtl* ctl, NbrDev L, float4* __restrict__ X, float4* __restrict__ P4)
 {
     __shared__ double lds[kBlock / kWave];
     REYN_SCENE(BT, sa);
@@ -430,7 +435,7 @@ __global__ __launch_bounds__(kBlock) void k_predict_rho(SceneArg sa, FluidDev f,
         const float r = acc.value();
         rho[i] = r;
         if constexpr (PK)
-            put_auxw(X, i, r);
+            reinterpret_cast<float*>(P4 + i)[3] = r;
         mine = r;
     }
     const double s = block_reduce<0>(mine, lds);
@@ -438,12 +443,14 @@ __global__ __launch_bounds__(kBlock) void k_predict_rho(SceneArg sa, FluidDev f,
         part[lb_] = s;
 }
 
-// PK: w of X holds the predicted densities.
+// PK: w of P4 holds the predicted densities; the listed loop reads {x, rho}
+// of each neighbour from one record when the scene's masses are all equal
+// (m_j is then the particle's own mass) and loads m_j as well otherwise.
 template <int K, bool IND, bool PK, bool BT>
 __global__ __launch_bounds__(kBlock) void k_pcisph_prs_acc(SceneArg sa, FluidDev f,
     const float* __restrict__ rho, const float* __restrict__ gbx, const float* __restrict__ gby,
     const float* __restrict__ gbz, float* __restrict__ ax, float* __restrict__ ay, float* __restrict__ az,
-    const SolveCtl* ctl, NbrDev L, const float4* __restrict__ X)
+    const SolveCtl* ctl, NbrDev L, const float4* __restrict__ P4)
 {
     REYN_SCENE(BT, sa);
     if (loop_done(ctl, sc) || !live)
@@ -459,15 +466,38 @@ __global__ __launch_bounds__(kBlock) void k_pcisph_prs_acc(SceneArg sa, FluidDev
     const float pri = pi / (rhoi * rhoi);
     const float pr0 = pi * rho0_sq_inv;
     f3 acc { 0.f, 0.f, 0.f };
-    nbrs4<IND, kNbrBatch<K>, 2>(g, L, t, xi, X, f.x, f.y, f.z, f.m, [&](uint32_t j, f3 xij, float d2, float mj) {
-        float rhoj;
-        if constexpr (PK)
-            rhoj = auxw(X, j);
-        else
-            rhoj = gat(rho, j);
+    auto term = [&](f3 xij, float d2, float mj, float rhoj) {
         const float pj = fmaxf(0.f, k * (rhoj - rho0));
         acc += (mj * (pri + pj / (rhoj * rhoj))) * kdW<K>(kp, xij, d2);
-    });
+    };
+    const uint32_t c = (PK && L.cnt) ? L.cnt[t] : 0xffffffffu;
+    if (PK && c <= L.cap) {
+        const bool uniform = !ctl[sc.idx].mass_mixed; // the same for the whole scene
+        const float mi = f.m[i];
+        constexpr uint32_t B = kNbrBatch<K>;
+        uint32_t off = t;
+        for (uint32_t kk = 0; kk < c; kk += B, off += B * L.n) {
+            uint32_t j[B];
+            float4 q[B];
+            float m[B];
+#pragma unroll
+            for (uint32_t u = 0; u < B; ++u) // past the count: row kk again (valid)
+                j[u] = gat(L.idx, (kk + u < c) ? off + u * L.n : off);
+#pragma unroll
+            for (uint32_t u = 0; u < B; ++u)
+                q[u] = gat(P4, j[u]);
+#pragma unroll
+            for (uint32_t u = 0; u < B; ++u)
+                m[u] = uniform ? mi : gat(f.m, j[u]);
+#pragma unroll
+            for (uint32_t u = 0; u < B; ++u) {
+                const f3 xij = xi - f3 { q[u].x, q[u].y, q[u].z };
+                term(xij, dot(xij, xij), (kk + u < c) ? m[u] : 0.f, q[u].w);
+            }
+        }
+    } else {
+        walk<IND>(g, xi, f.x, f.y, f.z, [&](uint32_t j, f3 xij, float d2) { term(xij, d2, gat(f.m, j), gat(rho, j)); });
+    }
     f3 a = -acc;
     if (bdy)
         a += -(0.5f * (pri + pr0)) * ld3(gbx, gby, gbz, i);
@@ -966,13 +996,13 @@ int pcisph_iteration(reyn_ctx* ctx, int kernel, const Common& c, const reyn_pcis
                 {
                     ProfScope _prof(ctx, "k_predict_rho");
                     k_predict_rho<K, I, PK, BT><<<c.nb, kBlock, 0, ctx->stream>>>(c.a, c.f, s.gbx, s.gby, s.gbz,
-                        s.rb, work->rho, s.part_d, s.ctl, s.L, s.X);
+                        s.rb, work->rho, s.part_d, s.ctl, s.L, s.X, s.P4);
                 }
                 REYN_LAUNCHED(ctx, "k_predict_rho");
                 {
                     ProfScope _prof(ctx, "k_pcisph_prs_acc");
                     k_pcisph_prs_acc<K, I, PK, BT><<<c.nb, kBlock, 0, ctx->stream>>>(c.a, c.f, work->rho, s.gbx,
-                        s.gby, s.gbz, work->ax, work->ay, work->az, s.ctl, s.L, s.X);
+                        s.gby, s.gbz, work->ax, work->ay, work->az, s.ctl, s.L, s.P4);
                 }
                 REYN_LAUNCHED(ctx, "k_pcisph_prs_acc");
                 return REYN_OK;
@@ -1263,7 +1293,7 @@ int reyn_predict_rho(reyn_ctx* ctx, const reyn_sph_params* prm, const reyn_parti
         {
             ProfScope _prof(ctx, "k_predict_rho");
             k_predict_rho<K, I, false, false><<<c.nb, kBlock, 0, ctx->stream>>>(c.a, c.f, s.gbx, s.gby, s.gbz,
-                s.rb, rho, nullptr, nullptr, NbrDev {}, nullptr);
+                s.rb, rho, nullptr, nullptr, NbrDev {}, nullptr, nullptr);
         }
         REYN_LAUNCHED(ctx, "k_predict_rho");
         return REYN_OK;
