@@ -25,6 +25,8 @@ Two more objects in the same line:
   continuous batching under the HBM budget, against one job at a time in
   arrival order (FIFO); p50/p99 time-to-result, particle-steps/s, and whether
   every served result equals its alone result bit for bit.
+- "grid": the 1M-particle grid and SESPH configuration at cell ratios 0.5, 1
+  and 2: grid build, build + reorder, neighbour query and SESPH step times.
 """
 
 from __future__ import annotations
@@ -148,6 +150,81 @@ def cpu_baseline(sc, host_state, p, dt_steps, args, log) -> dict | None:
                       f"{wall:.2f} s wall"}
 
 
+def grid_leg(ctx, args, log) -> dict:
+    """The grid and SESPH configuration: 1,000,000 particles (100^3 at s = 0.02,
+    jitter 5%, seed 2) in [0,2]^3, at cell ratios 0.5, 1 and 2: grid build
+    alone, build + reorder of the 7 state fields, the neighbour query of the
+    reference's micro-benchmark (sum_j x_i x_j W_ij with W6, uniformgrid.cu:231-245)
+    over the reordered and the unreordered grid, and SESPH steps with the grid
+    rebuilt every step and the state reordered every 10 steps. Wall clock per
+    call, `grid_reps` calls each after one untimed call."""
+    import ctypes as C
+
+    from reyn_amd import capi, scenes
+    from reyn_amd.sim import SESPH, BoundarySamples, Particles, Simulation, UniformGridBuilder
+
+    sc = scenes.config2_grid()
+    h = sc.spacing
+    r = 2.0 * h
+    K = args.grid_reps
+    out = {"workload": f"{sc.n} particles (100^3 lattice, s = 0.02, jitter 5%, seed 2) in [0,2]^3, "
+                       f"{sc.n_bdy} boundary samples; W6 query; SESPH k = 1000, C2",
+           "particles": sc.n, "ratios": {}}
+
+    def timed(fn):
+        fn()
+        ctx.synchronize()
+        t = time.perf_counter()
+        for _ in range(K):
+            fn()
+        ctx.synchronize()
+        return (time.perf_counter() - t) / K
+
+    for ratio in (0.5, 1.0, 2.0):
+        state = Particles(ctx, sc.x, None, sc.mass, sc.rho_0)
+        builder = UniformGridBuilder(ctx, sc.bound_min, sc.bound_max, r, ratio)
+        ncells = builder.geometry().ncells
+        t_build = timed(lambda: builder.construct(r, state.xx, state.xy, state.xz))
+        fields = [(state, f) for f in Particles.FIELDS]
+        t_both = timed(lambda: builder.construct_and_reorder(r, fields))
+        res = ctx.empty(sc.n)
+        g_sorted = builder.construct_and_reorder(r, fields)
+
+        def query(g):
+            ctx.check(ctx._lib.reyn_grid_query_bench(ctx.h, C.byref(g), capi.KERNEL_W6, float(r), state.xx.ptr,
+                                                     state.xy.ptr, state.xz.ptr, sc.n, res.ptr), "query")
+        t_q_sorted = timed(lambda: query(g_sorted))
+        # the unreordered view: a shuffled copy of the state, built without reorder
+        perm = np.random.default_rng(2).permutation(sc.n)
+        shuffled = Particles(ctx, state.positions()[perm], None, sc.mass, sc.rho_0)
+        g_unsorted = builder.construct(r, shuffled.xx, shuffled.xy, shuffled.xz)
+        t_q_unsorted = timed(lambda: ctx.check(ctx._lib.reyn_grid_query_bench(
+            ctx.h, C.byref(g_unsorted), capi.KERNEL_W6, float(r), shuffled.xx.ptr, shuffled.xy.ptr,
+            shuffled.xz.ptr, sc.n, res.ptr), "query"))
+        builder.close()
+        st2 = Particles(ctx, sc.x, None, sc.mass, sc.rho_0)
+        bdy = BoundarySamples(ctx, sc.xb, h, sc.rho_0, ratio)
+        solver = SESPH(ctx, "C2", sc.n, 0.001, h, sc.rho_0, k=1000.0)
+        sim = Simulation(ctx, solver, st2, bdy, h, sc.bound_min, sc.bound_max, cfl_lambda=0.4, cell_ratio=ratio,
+                         reorder_every=10, dt_policy="cfl")
+        for _ in range(2):
+            sim.step()
+        ctx.synchronize()
+        t = time.perf_counter()
+        for _ in range(2 * K):
+            sim.step()
+        ctx.synchronize()
+        t_step = (time.perf_counter() - t) / (2 * K)
+        sim.builder.close()
+        row = {"cells": int(ncells), "build_us": round(t_build * 1e6, 1),
+               "build_and_reorder_us": round(t_both * 1e6, 1),
+               "query_reordered_us": round(t_q_sorted * 1e6, 1), "query_unreordered_us": round(t_q_unsorted * 1e6, 1),
+               "sesph_step_us": round(t_step * 1e6, 1)}
+        out["ratios"][str(ratio)] = row
+        log(f"grid ratio {ratio}: {row}")
+    return out
+
+
 def batch_leg(ctx, args, log) -> dict:
     """64 PCISPH scenes in one batch (autotuned) against 64 sequential plain
     solves (the reference's main loop per scene through the single-scene
@@ -265,8 +342,10 @@ def main() -> int:
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--serve-jobs", type=int, default=32)
     ap.add_argument("--serve-load", type=float, default=0.8)
-    ap.add_argument("--only", choices=("batch", "serve"), default=None,
-                    help="run only the batch or the serving leg and print its object as one JSON line")
+    ap.add_argument("--only", choices=("batch", "serve", "grid"), default=None,
+                    help="run only the batch, serving or grid leg and print its object as one JSON line")
+    ap.add_argument("--grid-reps", type=int, default=10)
+    ap.add_argument("--no-grid", action="store_true", help="skip the grid and SESPH leg")
     ap.add_argument("--no-profile", action="store_true", help="time without per-launch events")
     ap.add_argument("--dt-policy", choices=("controller", "cfl"), default="controller",
                     help="controller: the reference main loop's simple_dt_controller from dt0 = 5e-4 "
@@ -311,9 +390,11 @@ def main() -> int:
     from reyn_amd.sim import IISPH, BoundarySamples, Particles, Simulation, reduce
 
     if args.only:
+        legs = {"batch": ("batch", batch_leg), "serve": ("serving", serve_leg), "grid": ("grid", grid_leg)}
+        key, fn = legs[args.only]
         with Context(0) as ctx:
-            leg = batch_leg(ctx, args, log) if args.only == "batch" else serve_leg(ctx, args, log)
-        print(json.dumps({"batch" if args.only == "batch" else "serving": leg}), flush=True)
+            leg = fn(ctx, args, log)
+        print(json.dumps({key: leg}), flush=True)
         return 0
 
     t_setup = time.perf_counter()
@@ -414,6 +495,7 @@ def main() -> int:
     sim.close()
     batch = None if args.no_batch else batch_leg(ctx, args, log)
     serving = None if args.no_serve else serve_leg(ctx, args, log)
+    grid = None if args.no_grid else grid_leg(ctx, args, log)
 
     out = {
         "metric": METRIC,
@@ -454,6 +536,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "batch": batch,
         "serving": serving,
+        "grid": grid,
     }
     print(json.dumps(out), flush=True)
     ctx.close()
