@@ -29,7 +29,7 @@ extern "C" {
 #pragma GCC visibility push(default)
 #endif
 
-#define REYN_ABI_VERSION 1
+#define REYN_ABI_VERSION 2
 
 /* ---- status codes ------------------------------------------------------ */
 enum {

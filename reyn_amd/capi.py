@@ -11,6 +11,8 @@ import ctypes as C
 from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "libreyn_amd.so"
+#: the C ABI version this binding is written for (REYN_ABI_VERSION of include/reyn_amd.h)
+ABI_VERSION = 2
 
 # status codes
 OK = 0
@@ -246,7 +248,7 @@ def lib() -> C.CDLL:
             fn = getattr(handle, name)
             fn.restype = res
             fn.argtypes = args
-        if handle.reyn_abi_version() != 1:
+        if handle.reyn_abi_version() != ABI_VERSION:
             raise RuntimeError("reyn_amd ABI version mismatch")
         _lib = handle
     return _lib

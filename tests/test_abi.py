@@ -27,7 +27,7 @@ def exported_symbols():
 
 def test_library_loads_without_gpu():
     lib = capi.lib()
-    assert lib.reyn_abi_version() == 1
+    assert lib.reyn_abi_version() == capi.ABI_VERSION
 
 
 def test_exports_match_header():
@@ -93,3 +93,8 @@ def test_struct_layouts_match_header(tmp_path):
         assert got[(py, "size")] == C.sizeof(cls), py
         for name, _ in cls._fields_:
             assert got[(py, name)] == getattr(cls, name).offset, (py, name)
+
+
+def test_abi_version_matches_header():
+    text = (ROOT / "include" / "reyn_amd.h").read_text()
+    assert int(re.search(r"#define REYN_ABI_VERSION (\d+)", text).group(1)) == capi.ABI_VERSION
