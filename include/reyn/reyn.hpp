@@ -11,6 +11,7 @@
 //   SESPH<K,R>, PCISPH<K,R>, IISPH<K,R>  src/solvers/*.cuh
 //   cfl_time_step, simple_dt_controller  src/timestep/cfl.cuh:13-32
 //   save_to_ply                          src/scene/ply_io.cu:7-56
+//   StepTrace: the GUI's dt / iteration plots as data   src/gui.cu:956-957
 //   SceneBatch: many scenes per launch (no reference counterpart)
 //
 // Device errors and misuse throw std::runtime_error (the reference throws on
@@ -656,6 +657,27 @@ inline void save_to_ply(const std::filesystem::path& path, const DeviceBuffer<fl
     if (!out)
         throw std::runtime_error("save_to_ply: write failed for " + path.string());
 }
+
+// ---- step telemetry (the GUI's dt and iteration plots, src/gui.cu:956-957) ---------
+
+struct StepTrace {
+    std::vector<float> dt_ms; // time step of each step in milliseconds, as plotted
+    std::vector<unsigned> iters; // pressure iterations of each step
+    void record(float dt, unsigned iterations)
+    {
+        dt_ms.push_back(dt * 1000.f);
+        iters.push_back(iterations);
+    }
+    void write_csv(const std::filesystem::path& path) const
+    {
+        std::ofstream out(path);
+        if (!out)
+            throw std::runtime_error("StepTrace: cannot open " + path.string());
+        out << "step,dt_ms,iterations\n";
+        for (size_t k = 0; k < dt_ms.size(); ++k)
+            out << k + 1 << ',' << dt_ms[k] << ',' << iters[k] << '\n';
+    }
+};
 
 // ---- scene batches (no reference counterpart) -------------------------------------
 //

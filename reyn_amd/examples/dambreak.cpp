@@ -2,10 +2,14 @@
 // reference's main loop (src/main.cu:60-78): time step, grid + reorder of the
 // state and the warm-start pressure, IISPH step, bounds.
 //
-//   dambreak [side=32] [steps=50]
+//   dambreak [side=32] [steps=50] [out]
+//
+// With `out`, the time-step / iteration trace goes to out.csv and the final
+// positions to out.ply.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <string>
 #include <vector>
 
 #include "reyn/reyn.hpp"
@@ -16,6 +20,7 @@ int main(int argc, char** argv)
 {
     const int side = argc > 1 ? std::atoi(argv[1]) : 32;
     const int steps = argc > 2 ? std::atoi(argv[2]) : 50;
+    const char* out = argc > 3 ? argv[3] : nullptr;
     const float h = 1.0f / side, rho_0 = 1000.f;
     const float3 box = v3(2.f, 1.25f, 2.f);
 
@@ -55,14 +60,21 @@ int main(int argc, char** argv)
 
     float dt = 0.f;
     double t = 0.0;
+    StepTrace trace;
     for (int s = 0; s < steps; ++s) {
         dt = cfl_time_step(0.4f, h, state, v3(0.f, -9.81f, 0.f));
         const auto grid = builder.construct_and_reorder(2.f * h, tmp, state, &prs);
         const uint32_t iters = solver.step(state, grid, bdy, dt);
         hard_enforce_bounds(bdy.bound_min, bdy.bound_max, state);
+        trace.record(dt, iters);
         t += dt;
         if (s % 10 == 0 || s + 1 == steps)
             std::printf("step %4d  t %.4f  dt %.3e  iters %u  mean rho %.3f\n", s, t, dt, iters, rho.avg());
+    }
+    if (out) { // the dt / iteration trace and the final positions
+        trace.write_csv(std::string(out) + ".csv");
+        save_to_ply(std::string(out) + ".ply", state.xx, state.xy, state.xz);
+        std::printf("wrote %s.csv and %s.ply\n", out, out);
     }
     return 0;
 }

@@ -313,7 +313,7 @@ class Simulation:
     def __init__(self, ctx: Context, solver, state: Particles, bdy: BoundarySamples | None, h: float,
                  bound_min, bound_max, cfl_lambda: float = 0.4, cell_ratio: float = 0.5, reorder_every: int = 1,
                  extra_reorder: tuple[str, ...] = (), dt_policy: str = "cfl", dt0: float = 0.0005,
-                 iter_target: int = 10, change_factor: float = 0.001):
+                 iter_target: int = 10, change_factor: float = 0.001, trace=None):
         if dt_policy not in ("cfl", "controller"):
             raise ValueError(f"unknown dt policy {dt_policy!r}")
         self.ctx = ctx
@@ -334,6 +334,7 @@ class Simulation:
         self.last_iters = 0
         self.steps = 0
         self.time = 0.0
+        self.trace = trace  # a telemetry.StepTrace, or None
 
     def get_grid(self) -> capi.Grid:
         """Scene::get_grid (src/scene/scene.cuh:149-156): build the grid and,
@@ -363,8 +364,11 @@ class Simulation:
         self.time += dt
         self.last_iters = int(iters)
         info = getattr(self.solver, "info", None)
-        return StepStats(dt, int(iters), float(info.mean_rel_error) if info else 0.0,
-                         int(info.n_violating) if info else 0)
+        stats = StepStats(dt, int(iters), float(info.mean_rel_error) if info else 0.0,
+                          int(info.n_violating) if info else 0)
+        if self.trace is not None:
+            self.trace.record(self.steps, self.time, dt, stats.iterations, stats.mean_rel_error)
+        return stats
 
 
 def batch_params(scenes=(), solver: str = "pcisph", kernel: str | int = "W6", nu: float = 0.001,

@@ -166,3 +166,31 @@ def test_checkpoint_resume_continues_exactly(ctx, tmp_path, solver, kernel):
     assert (a.steps, a.iterations, a.dt, a.time) == (b.steps, b.iterations, b.dt, b.time)
     second.close()
     straight.close()
+
+
+def test_step_traces(ctx):
+    """The telemetry traces: one record per step with the step's time step and
+    iteration count, for a single-scene simulation and for batch scenes."""
+    from reyn_amd.telemetry import StepTrace, record_batch_step
+
+    sc = small_scenes(count=1, seed=19)[0]
+    h = sc.spacing
+    state = Particles(ctx, sc.x, None, sc.mass, sc.rho_0)
+    bdy = BoundarySamples(ctx, sc.xb, h, sc.rho_0)
+    solver = PCISPH(ctx, "W6", sc.n, 0.001, h, sc.rho_0, eta_rho_max=0.001, min_iter=3, max_iter=100)
+    trace = StepTrace()
+    sim = Simulation(ctx, solver, state, bdy, h, sc.bound_min, sc.bound_max, cfl_lambda=1.0,
+                     dt_policy="controller", dt0=0.0005, iter_target=10, change_factor=0.001, trace=trace)
+    stats = [sim.step() for _ in range(5)]
+    assert trace.steps == [1, 2, 3, 4, 5]
+    assert trace.iterations == [s.iterations for s in stats] and trace.dt == [s.dt for s in stats]
+    b = Batch(ctx, [sc], solver="pcisph", kernel="W6")
+    traces = {}
+    for _ in range(5):
+        b.step(1)
+        record_batch_step(b, traces)
+    assert traces[0].steps == [1, 2, 3, 4, 5]
+    # the batch of one takes the same steps as the single-scene path
+    assert traces[0].iterations == trace.iterations
+    assert [np.float32(v) for v in traces[0].dt] == [np.float32(v) for v in trace.dt]
+    b.close()

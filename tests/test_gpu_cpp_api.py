@@ -2,6 +2,8 @@
 reyn_amd/examples/build.sh) and the headless dam-break example on the GPU."""
 
 import subprocess
+
+import numpy as np
 from pathlib import Path
 
 import pytest
@@ -25,8 +27,16 @@ def test_cpp_host_api():
     assert "0 failed" in p.stdout
 
 
-def test_cpp_dambreak_example():
-    p = _run(["dambreak", "20", "30"], 300)
+def test_cpp_dambreak_example(tmp_path):
+    from reyn_amd.io import read_ply
+
+    out = tmp_path / "run"
+    p = _run(["dambreak", "20", "30", str(out)], 300)
     print(p.stdout)
     assert p.returncode == 0, p.stdout + p.stderr
     assert "step   29" in p.stdout
+    rows = (tmp_path / "run.csv").read_text().splitlines()
+    assert rows[0] == "step,dt_ms,iterations" and len(rows) == 31
+    assert all(int(r.split(",")[2]) >= 3 for r in rows[1:])
+    x = read_ply(tmp_path / "run.ply")
+    assert x.shape == (20 ** 3, 3) and np.isfinite(x).all()

@@ -36,3 +36,17 @@ def test_checkpoint_round_trip(tmp_path):
         assert np.array_equal(back.state[k], state[k])
     assert np.array_equal(back.scene.xb, sc.xb) and back.scene.spacing == sc.spacing
     assert back.scene.mass == sc.mass and np.array_equal(back.scene.bound_min, sc.bound_min)
+
+
+def test_step_trace_round_trip(tmp_path):
+    from reyn_amd.telemetry import StepTrace
+
+    t = StepTrace()
+    for k in range(1, 6):
+        t.record(k, 1e-3 * k, 5e-4 + 1e-7 * k, 3 + k % 2, 1e-3 / k)
+    t.write_csv(tmp_path / "t.csv")
+    back = StepTrace.read_csv(tmp_path / "t.csv")
+    assert back == t
+    s = t.summary()
+    assert s["steps"] == 5 and s["iterations_max"] == 4 and s["dt_max"] == 5e-4 + 5e-7
+    assert abs(t.dt_ms[0] - 0.5001) < 1e-12
