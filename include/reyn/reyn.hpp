@@ -454,6 +454,9 @@ public:
         builder_ = std::make_unique<UniformGridBuilder>(bound_min, bound_max, 2.f * h, 0.5f, ctx);
         DeviceBuffer<float> tmp(pts.size(), ctx);
         grid_ = builder_->construct_and_reorder(2.f * h, tmp, xs, ys, zs);
+        const reyn_grid& g = grid_.grid;
+        near_ = std::make_unique<DeviceBuffer<uint8_t>>(size_t(g.nx) * size_t(g.ny) * size_t(g.nz), ctx);
+        ctx.check(reyn_grid_occupancy(ctx.get(), &g, near_->ptr()), "boundary occupancy");
         reyn_boundary v = get().b;
         ctx.check(reyn_boundary_masses(ctx.get(), &v, m.ptr(), h, rho_0, mass_iterations), "boundary masses");
     }
@@ -461,12 +464,13 @@ public:
     Boundary get() const
     {
         return Boundary { reyn_boundary { xs.ptr(), ys.ptr(), zs.ptr(), m.ptr(), static_cast<uint32_t>(xs.size()),
-            grid_.grid } };
+            grid_.grid, near_ ? near_->ptr() : nullptr } };
     }
 
 private:
     std::unique_ptr<UniformGridBuilder> builder_;
     UniformGrid<Resort::yes> grid_ {};
+    std::unique_ptr<DeviceBuffer<uint8_t>> near_; // cells with samples within reach
 };
 
 // Scene::hard_enforce_bounds (src/scene/scene.cu:466-491)

@@ -92,6 +92,11 @@ typedef struct reyn_boundary {
     const float *xx, *xy, *xz, *m;
     uint32_t n;
     reyn_grid grid;
+    /* optional: per cell of `grid`, nonzero when a sample lies in the
+     * (2 range + 1)^3 cells around it (`reyn_grid_occupancy`); a fluid
+     * particle in a cell marked zero has no boundary neighbour and skips the
+     * boundary walk. NULL: every particle walks the boundary grid. */
+    const uint8_t* near;
 } reyn_boundary;
 
 /* Constants shared by every solver (the constructor arguments of SESPH,
@@ -209,6 +214,11 @@ int reyn_gather_f32(reyn_ctx* ctx, const uint32_t* sorted, uint32_t n,
 int reyn_grid_query_stats(reyn_ctx* ctx, const reyn_grid* grid,
     const float* xx, const float* xy, const float* xz, uint32_t n,
     uint32_t* count, float* len2, float* vx, float* vy, float* vz);
+/* near[c] = 1 when a point of `grid` lies in the (2 range + 1)^3 cells
+ * around cell c, else 0, for all nx ny nz cells (the boundary view's
+ * `near`). No reference counterpart: the reference walks the boundary grid
+ * for every fluid particle (solver.cuh:18-38). */
+int reyn_grid_occupancy(reyn_ctx* ctx, const reyn_grid* grid, uint8_t* near);
 /* res[i] = sum_j x_i.x * x_j.x * W(x_ij) with the given kernel
  * (`_benchmark_kernel`, uniformgrid.cu:231-245). */
 int reyn_grid_query_bench(reyn_ctx* ctx, const reyn_grid* grid,

@@ -75,6 +75,7 @@ class Boundary(C.Structure):
         ("xx", vp), ("xy", vp), ("xz", vp), ("m", vp),
         ("n", u32),
         ("grid", Grid),
+        ("near", vp),
     ]
 
 
@@ -224,6 +225,7 @@ _SIGS = {
     "reyn_batch_scene_size": (i32, [vp, u32, P(u32)]),
     "reyn_batch_scene_bytes": (i32, [P(BatchParams), P(BatchScene), P(u64)]),
     "reyn_batch_step": (i32, [vp, u32]),
+    "reyn_grid_occupancy": (i32, [vp, P(Grid), vp]),
     "reyn_device_memory": (i32, [vp, P(u64), P(u64)]),
     "reyn_batch_info": (i32, [vp, u32, P(BatchSceneInfo)]),
     "reyn_batch_download": (i32, [vp, u32, vp, vp, vp, vp, vp, vp, vp, vp]),

@@ -122,6 +122,7 @@ struct SceneRes {
     float mass = 0.f;
     reyn_grid_builder* bgrid = nullptr;
     void* bmem = nullptr;
+    uint8_t* near = nullptr; // boundary cells with samples within reach
     BdyDev bd {};
     GridDev g {}; // prefix set by the layout
     size_t ncells = 0;
@@ -143,6 +144,8 @@ void free_scene(SceneRes* r)
     reyn_grid_builder_destroy(r->bgrid);
     if (r->bmem)
         (void)hipFree(r->bmem);
+    if (r->near)
+        (void)hipFree(r->near);
     delete r;
 }
 
@@ -345,7 +348,13 @@ int make_scene(reyn_batch* b, const reyn_batch_scene& sc, SceneRes** out)
                 rc = reyn_grid_construct_and_reorder(ctx, r->bgrid, rad, fields, tmp, 3, sc.n_bdy, &g);
             if (rc != REYN_OK)
                 break;
-            reyn_boundary view { fields[0], fields[1], fields[2], bm, sc.n_bdy, g };
+            const size_t bcells = size_t(g.nx) * size_t(g.ny) * size_t(g.nz);
+            rc = check_hip(ctx, hipMalloc(reinterpret_cast<void**>(&r->near), bcells ? bcells : 1), "hipMalloc near");
+            if (rc == REYN_OK)
+                rc = reyn_grid_occupancy(ctx, &g, r->near);
+            if (rc != REYN_OK)
+                break;
+            reyn_boundary view { fields[0], fields[1], fields[2], bm, sc.n_bdy, g, r->near };
             rc = reyn_boundary_masses(ctx, &view, bm, h, prm.rho_0, 5);
             if (rc != REYN_OK)
                 break;

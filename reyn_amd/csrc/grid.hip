@@ -256,6 +256,27 @@ __global__ __launch_bounds__(kBlock) void k_gather(uint32_t n, const uint32_t* _
         a.dst[f][i] = a.src[f][s];
 }
 
+// near[c] = 1 when a point lies in the (2R+1)^3 cells around cell c: the
+// strips a walk from cell c would visit, each tested with two cell starts.
+__global__ __launch_bounds__(kBlock) void k_occupancy(GridDev g, uint64_t ncells, uint8_t* __restrict__ near)
+{
+    const uint64_t c = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (c >= ncells)
+        return;
+    const int x = static_cast<int>(c % uint64_t(g.nx));
+    const int y = static_cast<int>((c / uint64_t(g.nx)) % uint64_t(g.ny));
+    const int z = static_cast<int>(c / uint64_t(g.nxny));
+    const int R = g.range;
+    const int x0 = max(x - R, 0), x1 = min(x + R, g.nx - 1);
+    uint8_t any = 0;
+    for (int zz = max(z - R, 0); zz <= min(z + R, g.nz - 1) && !any; ++zz)
+        for (int yy = max(y - R, 0); yy <= min(y + R, g.ny - 1) && !any; ++yy) {
+            const uint64_t base = uint64_t(zz) * uint64_t(g.nxny) + uint64_t(yy) * uint64_t(g.nx);
+            any = g.prefix[base + x1 + 1] > g.prefix[base + x0] ? 1 : 0;
+        }
+    near[c] = any;
+}
+
 template <bool IND>
 __global__ __launch_bounds__(kBlock) void k_query_stats(GridDev g, const float* __restrict__ xs,
     const float* __restrict__ ys, const float* __restrict__ zs, uint32_t n,
@@ -643,6 +664,25 @@ int reyn_grid_query_stats(reyn_ctx* ctx, const reyn_grid* grid, const float* xx,
             k_query_stats<false><<<blocks_for(n), kBlock, 0, ctx->stream>>>(g, xx, xy, xz, n, count, len2, vx, vy, vz);
     }
     REYN_LAUNCHED(ctx, "k_query_stats");
+    return REYN_OK;
+}
+
+int reyn_grid_occupancy(reyn_ctx* ctx, const reyn_grid* grid, uint8_t* near)
+{
+    if (!ctx || !grid || !near || !grid->prefix)
+        return set_error(ctx, REYN_ERR_INVALID_ARGUMENT, "grid occupancy: NULL argument");
+    if (grid->nx <= 0 || grid->ny <= 0 || grid->nz <= 0 || grid->range < 0)
+        return set_error(ctx, REYN_ERR_INVALID_ARGUMENT, "grid occupancy: bad grid");
+    const uint64_t nc = uint64_t(grid->nx) * uint64_t(grid->ny) * uint64_t(grid->nz);
+    const uint64_t nb = (nc + kBlock - 1) / kBlock;
+    if (nb >= (uint64_t(1) << 31))
+        return set_error(ctx, REYN_ERR_CAPACITY, "grid occupancy: too many cells");
+    REYN_HIP(ctx, hipSetDevice(ctx->device));
+    {
+        ProfScope _prof(ctx, "k_occupancy");
+        k_occupancy<<<static_cast<uint32_t>(nb), kBlock, 0, ctx->stream>>>(to_dev(*grid), nc, near);
+    }
+    REYN_LAUNCHED(ctx, "k_occupancy");
     return REYN_OK;
 }
 

@@ -274,7 +274,10 @@ __global__ __launch_bounds__(kBlock) REYN_OCC(REYN_OCC_DENSITY) void k_density(S
     }
     ksum accb;
     ksum3 grad;
-    if (b.present) {
+    // a particle whose boundary cell has no sample within reach has no
+    // boundary neighbour: its walk would only visit empty strips
+    const bool bnear = b.present && (!b.near || b.near[cell_key(b.grid, cell_of(b.grid, xi))]);
+    if (bnear) {
         auto visit = [&](f3 xik, float d2, float mk) {
             accb.add(mk * kW<K>(kp, d2));
             if (GB)

@@ -183,6 +183,11 @@ __device__ __forceinline__ int3c cell_of(const GridDev& g, f3 x)
         cell_axis(x.z, g.bmin.z, g.cell, g.nz) };
 }
 
+__device__ __forceinline__ uint32_t cell_key(const GridDev& g, int3c c)
+{
+    return static_cast<uint32_t>(c.z) * static_cast<uint32_t>(g.nxny) + static_cast<uint32_t>(c.y * g.nx + c.x);
+}
+
 constexpr uint32_t kWalkBudget = 1u << 15;
 
 // Calls r(s, e) for the candidate range [s, e) (cell order) of each of the
@@ -402,6 +407,7 @@ inline FluidDev to_dev(const reyn_particles& p)
 struct BdyDev {
     const float *x, *y, *z, *m;
     const float4* p4; // packed {x, y, z, m} of the same samples, or NULL
+    const uint8_t* near; // per grid cell: samples within reach, or NULL
     uint32_t n;
     GridDev grid;
     int present;
@@ -416,6 +422,7 @@ inline BdyDev to_dev(const reyn_boundary* b)
         d.m = b->m;
         d.n = b->n;
         d.grid = to_dev(b->grid);
+        d.near = b->near;
         d.present = 1;
     }
     return d;

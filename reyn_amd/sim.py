@@ -169,12 +169,15 @@ class BoundarySamples:
         self.builder = UniformGridBuilder(ctx, self.bound_min, self.bound_max, 2.0 * h, cell_ratio)
         # the boundary grid is built once and the samples are stored in cell order
         self.grid = self.builder.construct_and_reorder(2.0 * h, [(self, "xs"), (self, "ys"), (self, "zs")])
+        g = self.grid
+        self.near = ctx.empty(int(g.nx) * int(g.ny) * int(g.nz), np.uint8)
+        ctx.check(ctx._lib.reyn_grid_occupancy(ctx.h, C.byref(self.grid), self.near._ptr), "boundary occupancy")
         view = self.view()
         ctx.check(ctx._lib.reyn_boundary_masses(ctx.h, C.byref(view), self.m._ptr, float(h), float(rho_0),
                                                 int(mass_iterations)), "boundary masses")
 
     def view(self) -> capi.Boundary:
-        return capi.Boundary(self.xs.ptr, self.ys.ptr, self.zs.ptr, self.m.ptr, self.n, self.grid)
+        return capi.Boundary(self.xs.ptr, self.ys.ptr, self.zs.ptr, self.m.ptr, self.n, self.grid, self.near.ptr)
 
     def positions(self) -> np.ndarray:
         return np.stack([self.xs.numpy(), self.ys.numpy(), self.zs.numpy()], axis=1)

@@ -182,3 +182,44 @@ def test_kernel_properties_reference_inputs(ctx, kernel):
     np.testing.assert_allclose(-Gg, Gg[::-1], rtol=1e-4, atol=1e-9)
     assert Wg.astype(np.float64).sum() == pytest.approx(1.0 / sp ** 3, rel=0.005)
     assert np.all(np.abs(Gg.astype(np.float64).sum(axis=0)) <= 0.01)
+
+
+@pytest.mark.parametrize("ratio", [0.5, 1.0])
+def test_boundary_occupancy(ctx, ratio):
+    """near[c] is 1 exactly when a boundary sample lies in the (2R+1)^3 cells
+    around c (numpy over the same cell starts), and a density pass with the
+    flags gives the same bits as one that walks the boundary grid for every
+    particle."""
+    import ctypes as C
+
+    from oracle.parity import GpuCase, make_case
+    from reyn_amd.sim import sph_params
+
+    sc, v = make_case(side=12, seed=3)
+    case = GpuCase(ctx, sc, v, ratio)
+    g = case.bdy.grid
+    nx, ny, nz, R = g.nx, g.ny, g.nz, g.range
+    prefix, _ = grid_tables(ctx, g)
+    counts = np.diff(prefix.astype(np.int64)).reshape(nz, ny, nx) > 0
+    pad = np.pad(counts, R)
+    want = np.zeros_like(counts)
+    for dz in range(2 * R + 1):
+        for dy in range(2 * R + 1):
+            for dx in range(2 * R + 1):
+                want |= pad[dz:dz + nz, dy:dy + ny, dx:dx + nx]
+    near = case.bdy.near.numpy().reshape(nz, ny, nx).astype(bool)
+    assert np.array_equal(near, want)
+    assert (~near).any() and near.any()
+    gf = case.grid()
+    prm = sph_params("B3", sc.spacing, 0.001, sc.rho_0)
+    out = []
+    for use_near in (True, False):
+        view = case.bdy.view()
+        if not use_near:
+            view.near = None
+        rho = ctx.empty(sc.n)
+        fl = case.state.view()
+        ctx.check(ctx._lib.reyn_density(ctx.h, C.byref(prm), C.byref(fl), C.byref(gf), C.byref(view), rho.ptr),
+                  "density")
+        out.append(rho.numpy())
+    assert np.array_equal(out[0], out[1])
