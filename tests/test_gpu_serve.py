@@ -37,3 +37,24 @@ def test_served_results_equal_results_alone(ctx, solver, kernel):
     run_fifo(ctx, alone, **opts)
     same, diff = same_results(served, alone)
     assert not diff, f"jobs {diff} differ between the mix and alone"
+
+
+def test_size_class_server_results_equal_results_alone(ctx):
+    """Size classes on their own streams and host threads: every result is
+    still the job alone, bit for bit."""
+    from reyn_amd.serve import SizeClassServer
+
+    spec = tiny_jobs(count=8, seed=23)
+    opts = dict(solver="iisph", kernel="B3", rho_0=1000.0)
+    arrivals = np.linspace(0.0, 0.01, len(spec))
+    served = make_jobs(spec, arrivals)
+    sizes = sorted(sc.n for sc, _ in spec)
+    srv = SizeClassServer(ctx, thresholds=(sizes[3] + 1,), device_setup=True, **opts)
+    rep = srv.run(served)
+    srv.close()
+    assert [c["jobs"] for c in rep["classes"]] == [4, 4]
+    assert all(j.info["steps"] == j.steps for j in served)
+    alone = make_jobs(spec, arrivals)
+    run_fifo(ctx, alone, device_setup=True, **opts)
+    same, diff = same_results(served, alone)
+    assert not diff, f"jobs {diff} differ between the size classes and alone"

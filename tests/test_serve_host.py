@@ -112,3 +112,28 @@ def test_poisson_arrivals():
     t = serve.poisson_arrivals(20000, 50.0, seed=1)
     assert np.all(np.diff(t) > 0)
     assert abs(np.diff(t).mean() - 1 / 50.0) < 0.05 / 50.0
+
+
+def test_size_classes_split_and_budget(fake, monkeypatch):
+    """Jobs go to their size class; every class's lane admits under the one
+    budget and each job leaves after exactly its steps."""
+    monkeypatch.setattr(serve, "Context", lambda device: None)
+    spec = scenes.config5_jobs(10, seed=6, n_lo=1000, n_hi=20000, steps_lo=3, steps_hi=8)
+    jobs = serve.make_jobs(spec, np.zeros(len(spec)))
+    srv = serve.SizeClassServer(types_ns_ctx(), thresholds=(5000,), reserve_bytes=0)
+    srv.budget = 3 * max(sc.n for sc, _ in spec) * 1000
+    rep = srv.run(jobs)
+    small = sum(1 for sc, _ in spec if sc.n < 5000)
+    assert [c["jobs"] for c in rep["classes"]] == [small, len(spec) - small]
+    assert rep["peak_bytes"] <= srv.budget
+    assert all(j.taken == j.steps and j.finished is not None for j in jobs)
+    assert [srv.class_of(j) for j in jobs] == [0 if j.scene.n < 5000 else 1 for j in jobs]
+
+
+def types_ns_ctx():
+    class Ctx:
+        device = 0
+
+        def close(self):
+            pass
+    return Ctx()
