@@ -10,6 +10,7 @@
 //   Boundary, BoundarySamples, Scene bounds/grid  src/scene/scene.cuh:14-156
 //   SESPH<K,R>, PCISPH<K,R>, IISPH<K,R>  src/solvers/*.cuh
 //   cfl_time_step, simple_dt_controller  src/timestep/cfl.cuh:13-32
+//   Mesh, sample_mesh                    src/scene/loader.h, sample_boundary.cu:391-440
 //   save_to_ply                          src/scene/ply_io.cu:7-56
 //   StepTrace: the GUI's dt / iteration plots as data   src/gui.cu:956-957
 //   SceneBatch: many scenes per launch (no reference counterpart)
@@ -629,6 +630,70 @@ private:
     reyn_solve_info info_ {};
     DeviceBuffer<float>&rho_, &aii_, &si_, &ax_, &ay_, &az_, &p_;
 };
+
+// ---- meshes: Mesh / load_mesh_from_obj (src/scene/loader.h), sample_mesh ----------
+// (sample_boundary.cu:391-440; no relaxation)
+
+class Mesh {
+public:
+    static Mesh load_obj(const std::filesystem::path& path, const std::vector<std::string>& ignore = {},
+        const char* only = nullptr)
+    {
+        std::vector<const char*> ig;
+        for (const auto& s : ignore)
+            ig.push_back(s.c_str());
+        char err[512] = {};
+        reyn_mesh* m = nullptr;
+        const int rc = reyn_mesh_load_obj(path.string().c_str(), ig.data(), uint32_t(ig.size()), only, &m, err,
+            sizeof(err));
+        if (rc != REYN_OK)
+            throw std::runtime_error(std::string("reyn: load_obj: ") + err);
+        return Mesh(m);
+    }
+    Mesh(Mesh&& o) noexcept : m_(std::exchange(o.m_, nullptr)) { }
+    Mesh& operator=(Mesh&& o) noexcept
+    {
+        std::swap(m_, o.m_);
+        return *this;
+    }
+    Mesh(const Mesh&) = delete;
+    Mesh& operator=(const Mesh&) = delete;
+    ~Mesh() { reyn_mesh_destroy(m_); }
+    uint32_t face_count() const
+    {
+        uint32_t nf = 0;
+        reyn_mesh_info(m_, nullptr, &nf, nullptr, nullptr);
+        return nf;
+    }
+    double get_volume() const
+    {
+        double v = 0.0;
+        reyn_mesh_info(m_, nullptr, nullptr, nullptr, &v);
+        return v;
+    }
+    const reyn_mesh* get() const { return m_; }
+
+private:
+    explicit Mesh(reyn_mesh* m) : m_(m) { }
+    reyn_mesh* m_ = nullptr;
+};
+
+// boundary samples of the mesh surface with Akinci masses
+inline BoundarySamples sample_mesh(const Mesh& mesh, float h, float rho_0, float oversampling = 1.f,
+    uint32_t seed = 1614201312u, Context& ctx = Context::current())
+{
+    uint32_t n = 0;
+    ctx.check(reyn_mesh_sample_count(mesh.get(), h, oversampling, &n), "sample_mesh count");
+    DeviceBuffer<float> xs(n, ctx), ys(n, ctx), zs(n, ctx);
+    uint32_t got = 0;
+    ctx.check(reyn_mesh_sample(ctx.get(), mesh.get(), h, oversampling, seed, xs.ptr(), ys.ptr(), zs.ptr(), n, &got),
+        "sample_mesh");
+    const std::vector<float> x = xs.download(), y = ys.download(), z = zs.download();
+    std::vector<float3> pts(n);
+    for (uint32_t i = 0; i < n; ++i)
+        pts[i] = v3(x[i], y[i], z[i]);
+    return BoundarySamples(pts, h, rho_0, 5, ctx);
+}
 
 // ---- save_to_ply (src/scene/ply_io.cu:7-56) ----------------------------------------
 //

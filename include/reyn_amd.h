@@ -411,6 +411,40 @@ int reyn_dam_break_sizes(const reyn_dam_break* d, uint32_t* n_lattice, uint32_t*
 int reyn_dam_break_generate(reyn_ctx* ctx, const reyn_dam_break* d, float* x, float* y, float* z,
     uint32_t* n_fluid, float* bx, float* by, float* bz);
 
+/* ---- triangle meshes ---------------------------------------------------------
+ * `Mesh`, `load_mesh_from_obj` (src/scene/loader.h, loader.cpp:13-107): vertices
+ * in double, triangles as vertex indices. OBJ: `v` and `f` lines (corners "v",
+ * "v/t", "v//n", "v/t/n"; negative indices count back), polygons triangulated
+ * as fans from their first corner (the default of tinyobjloader v1.0.6, which
+ * the reference uses), shapes (`o`/`g`) skipped when their lower-case name
+ * contains an `ignore` entry or lacks `only`. On failure `err` (may be NULL)
+ * receives the reason. */
+typedef struct reyn_mesh reyn_mesh;
+int reyn_mesh_load_obj(const char* path, const char* const* ignore, uint32_t n_ignore,
+    const char* only, reyn_mesh** out, char* err, size_t err_len);
+int reyn_mesh_create(const double* x, const double* y, const double* z, uint32_t n_vertices,
+    const uint32_t* faces, uint32_t n_faces, reyn_mesh** out);
+int reyn_mesh_destroy(reyn_mesh* m);
+/* Sizes, total area and signed volume (sum of a . (b x c) / 6,
+ * `DeviceMesh::get_volume`, sample_boundary.cuh:33-60); any output may be NULL. */
+int reyn_mesh_info(const reyn_mesh* m, uint32_t* n_vertices, uint32_t* n_faces, double* area,
+    double* volume);
+int reyn_mesh_data(const reyn_mesh* m, double* x, double* y, double* z, uint32_t* faces);
+/* Boundary samples of the surface (`uniform_sample_mesh`,
+ * sample_boundary.cu:179-249, 299-389): n = ceil(area / (1.5 sqrt(3) h_bdy^2))
+ * with h_bdy = h / oversampling; sample i takes the area coordinate
+ * (i / n + u / n) * area, the face whose inclusive area CDF is first >= it, and
+ * barycentric (s, t) folded into the triangle, u, s, t from a counter hash of
+ * (seed, i); in double, rounded to float. Device arrays of `capacity`. */
+int reyn_mesh_sample_count(const reyn_mesh* m, float h, float oversampling, uint32_t* count);
+int reyn_mesh_sample(reyn_ctx* ctx, const reyn_mesh* m, float h, float oversampling,
+    uint32_t seed, float* xs, float* ys, float* zs, uint32_t capacity, uint32_t* count);
+/* inside[k] = 1 when the generalized winding number of the mesh at point k
+ * (device arrays) exceeds 1/2: the reference's inside predicate for culling
+ * fluid (scene.cu:59-140), evaluated over every face instead of an LBVH. */
+int reyn_mesh_inside(reyn_ctx* ctx, const reyn_mesh* m, const float* xs, const float* ys,
+    const float* zs, uint32_t n, uint8_t* inside);
+
 /* ---- scene batches ---------------------------------------------------------
  * Many independent scenes advanced together: every phase of a step is one
  * launch over the particles of all scenes (workgroups aligned to scene

@@ -225,6 +225,14 @@ _SIGS = {
     "reyn_batch_scene_size": (i32, [vp, u32, P(u32)]),
     "reyn_batch_scene_bytes": (i32, [P(BatchParams), P(BatchScene), P(u64)]),
     "reyn_batch_step": (i32, [vp, u32]),
+    "reyn_mesh_load_obj": (i32, [C.c_char_p, P(C.c_char_p), u32, C.c_char_p, P(vp), C.c_char_p, C.c_size_t]),
+    "reyn_mesh_create": (i32, [vp, vp, vp, u32, vp, u32, P(vp)]),
+    "reyn_mesh_destroy": (i32, [vp]),
+    "reyn_mesh_info": (i32, [vp, P(u32), P(u32), P(f64), P(f64)]),
+    "reyn_mesh_data": (i32, [vp, vp, vp, vp, vp]),
+    "reyn_mesh_sample_count": (i32, [vp, f32, f32, P(u32)]),
+    "reyn_mesh_sample": (i32, [vp, vp, f32, f32, u32, vp, vp, vp, u32, P(u32)]),
+    "reyn_mesh_inside": (i32, [vp, vp, vp, vp, vp, u32, vp]),
     "reyn_grid_occupancy": (i32, [vp, P(Grid), vp]),
     "reyn_device_memory": (i32, [vp, P(u64), P(u64)]),
     "reyn_batch_info": (i32, [vp, u32, P(BatchSceneInfo)]),

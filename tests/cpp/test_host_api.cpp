@@ -2,6 +2,7 @@
 // the style of the reference's doctest cases. Run by tests/test_gpu_cpp_api.py.
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <filesystem>
 #include <fstream>
@@ -328,6 +329,18 @@ int main()
         for (float v : st.y)
             finite = finite && std::isfinite(v);
         CHECK(finite);
+    });
+
+    run("Mesh loads the reference's cube and samples it as a boundary", [] {
+        const char* root = std::getenv("REYN_TEST_GOLDEN");
+        const std::string dir = root ? root : "tests/golden";
+        const Mesh cube = Mesh::load_obj(dir + "/cube.obj");
+        CHECK(cube.face_count() == 12u);
+        CHECK(cube.get_volume() == 8.0);
+        BoundarySamples bdy = sample_mesh(cube, 0.1f, 1000.f);
+        CHECK(bdy.xs.size() > 0);
+        CHECK(bdy.bound_min.x >= -1.f && bdy.bound_max.x <= 1.f);
+        CHECK(bdy.m.min() > 0.0);
     });
 
     run("save_to_ply writes the reference's binary PLY layout", [] {

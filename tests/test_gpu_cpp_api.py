@@ -21,6 +21,9 @@ def _run(args, timeout):
 
 
 def test_cpp_host_api():
+    import os
+
+    os.environ["REYN_TEST_GOLDEN"] = str(Path(__file__).resolve().parent / "golden")
     p = _run(["test_host_api"], 300)
     print(p.stdout)
     assert p.returncode == 0, p.stdout + p.stderr

