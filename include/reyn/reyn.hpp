@@ -632,7 +632,7 @@ private:
 };
 
 // ---- meshes: Mesh / load_mesh_from_obj (src/scene/loader.h), sample_mesh ----------
-// (sample_boundary.cu:391-440; no relaxation)
+// (sample_boundary.cu:391-440, with the optional relaxation of :100-166)
 
 class Mesh {
 public:
@@ -678,9 +678,13 @@ private:
     reyn_mesh* m_ = nullptr;
 };
 
-// boundary samples of the mesh surface with Akinci masses
+// boundary samples of the mesh surface with Akinci masses; `relaxation_iters`
+// rounds of relaxation follow the sampling (the reference defaults to 50 with
+// factor 1; here the default is none). `ratio`, when given, receives the
+// max/min number density of the last relaxation round.
 inline BoundarySamples sample_mesh(const Mesh& mesh, float h, float rho_0, float oversampling = 1.f,
-    uint32_t seed = 1614201312u, Context& ctx = Context::current())
+    uint32_t seed = 1614201312u, Context& ctx = Context::current(), uint32_t relaxation_iters = 0,
+    float relaxation_factor = 1.f, float* ratio = nullptr)
 {
     uint32_t n = 0;
     ctx.check(reyn_mesh_sample_count(mesh.get(), h, oversampling, &n), "sample_mesh count");
@@ -688,6 +692,10 @@ inline BoundarySamples sample_mesh(const Mesh& mesh, float h, float rho_0, float
     uint32_t got = 0;
     ctx.check(reyn_mesh_sample(ctx.get(), mesh.get(), h, oversampling, seed, xs.ptr(), ys.ptr(), zs.ptr(), n, &got),
         "sample_mesh");
+    if (relaxation_iters)
+        ctx.check(reyn_mesh_relax(ctx.get(), mesh.get(), h, oversampling, xs.ptr(), ys.ptr(), zs.ptr(), n,
+                      relaxation_iters, relaxation_factor, ratio),
+            "sample_mesh relaxation");
     const std::vector<float> x = xs.download(), y = ys.download(), z = zs.download();
     std::vector<float3> pts(n);
     for (uint32_t i = 0; i < n; ++i)

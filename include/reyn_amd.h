@@ -439,6 +439,16 @@ int reyn_mesh_data(const reyn_mesh* m, double* x, double* y, double* z, uint32_t
 int reyn_mesh_sample_count(const reyn_mesh* m, float h, float oversampling, uint32_t* count);
 int reyn_mesh_sample(reyn_ctx* ctx, const reyn_mesh* m, float h, float oversampling,
     uint32_t seed, float* xs, float* ys, float* zs, uint32_t capacity, uint32_t* count);
+/* Relaxes samples of the mesh surface (device arrays, any order; left in the
+ * cell order of their grid) `relax_sampling` (sample_boundary.cu:100-166):
+ * per iteration a grid over the samples, their B3 number densities (support
+ * 2 h_bdy), a shift -factor h_bdy^2 sum_j (n_i + n_j) / n_0^2 grad W_ij with n_0
+ * the hexagonal-plane rest density, and the projection of every shifted sample
+ * to the closest point of the mesh (all faces, in double; the reference uses
+ * an LBVH). `ratio` (may be NULL) receives max/min number density of the last
+ * iteration. */
+int reyn_mesh_relax(reyn_ctx* ctx, const reyn_mesh* m, float h, float oversampling,
+    float* xs, float* ys, float* zs, uint32_t n, uint32_t iterations, float factor, float* ratio);
 /* inside[k] = 1 when the generalized winding number of the mesh at point k
  * (device arrays) exceeds 1/2: the reference's inside predicate for culling
  * fluid (scene.cu:59-140), evaluated over every face instead of an LBVH. */

@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "sph.hpp"
 
 struct reyn_mesh {
     std::vector<double> x, y, z;
@@ -114,6 +115,118 @@ __global__ __launch_bounds__(kBlock) void k_mesh_inside(MeshDev m, const float* 
         w += 2.0 * atan2(det, den);
     }
     inside[k] = w / (4.0 * M_PI) > 0.5 ? 1 : 0;
+}
+
+// ---- relaxation of the samples -------------------------------------------------
+
+// Number density of sample i: sum_j W(x_ij) over the samples (B3 of support
+// 2 h_bdy; the grid's radius covers it).
+__global__ __launch_bounds__(kBlock) void k_relax_density(GridDev g, KParams kp, const float* __restrict__ xs,
+    const float* __restrict__ ys, const float* __restrict__ zs, uint32_t n, float* __restrict__ nd)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n)
+        return;
+    float acc = 0.f;
+    walk<false>(g, ld3(xs, ys, zs, i), xs, ys, zs, [&](uint32_t, f3, float d2) { acc += kW<KB3>(kp, d2); });
+    nd[i] = acc;
+}
+
+// The shift of `_shift_neg_rho_grad` (sample_boundary.cu:46-77): down the
+// symmetric number-density gradient, -k h_bdy^2 sum_j (n_i + n_j) / n_0^2
+// grad W_ij, written to separate arrays (the reference shifts in place while
+// neighbours read).
+__global__ __launch_bounds__(kBlock) void k_relax_shift(GridDev g, KParams kp, const float* __restrict__ xs,
+    const float* __restrict__ ys, const float* __restrict__ zs, uint32_t n, const float* __restrict__ nd, float k,
+    float h_bdy, float n0_sq_inv, float* __restrict__ ox, float* __restrict__ oy, float* __restrict__ oz)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n)
+        return;
+    const f3 xi = ld3(xs, ys, zs, i);
+    const float ti = n0_sq_inv * nd[i];
+    f3 acc { 0.f, 0.f, 0.f };
+    walk<false>(g, xi, xs, ys, zs, [&](uint32_t j, f3 xij, float d2) {
+        acc += (ti + n0_sq_inv * nd[j]) * kdW<KB3>(kp, xij, d2);
+    });
+    st3(ox, oy, oz, i, xi + (-k * h_bdy * h_bdy) * acc);
+}
+
+struct D3 {
+    double x, y, z;
+};
+__device__ inline D3 dsub(D3 a, D3 b) { return { a.x - b.x, a.y - b.y, a.z - b.z }; }
+__device__ inline double ddot(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ inline D3 dmad(D3 a, double s, D3 b) { return { a.x + s * b.x, a.y + s * b.y, a.z + s * b.z }; }
+
+// Closest point of triangle (a, b, c) to p, by the Voronoi regions of its
+// vertices, edges and face.
+__device__ D3 closest_on_triangle(D3 p, D3 a, D3 b, D3 c)
+{
+    const D3 ab = dsub(b, a), ac = dsub(c, a), ap = dsub(p, a);
+    const double d1 = ddot(ab, ap), d2 = ddot(ac, ap);
+    if (d1 <= 0.0 && d2 <= 0.0)
+        return a;
+    const D3 bp = dsub(p, b);
+    const double d3 = ddot(ab, bp), d4 = ddot(ac, bp);
+    if (d3 >= 0.0 && d4 <= d3)
+        return b;
+    const double vc = d1 * d4 - d3 * d2;
+    if (vc <= 0.0 && d1 >= 0.0 && d3 <= 0.0)
+        return dmad(a, d1 / (d1 - d3), ab);
+    const D3 cp = dsub(p, c);
+    const double d5 = ddot(ab, cp), d6 = ddot(ac, cp);
+    if (d6 >= 0.0 && d5 <= d6)
+        return c;
+    const double vb = d5 * d2 - d1 * d6;
+    if (vb <= 0.0 && d2 >= 0.0 && d6 <= 0.0)
+        return dmad(a, d2 / (d2 - d6), ac);
+    const double va = d3 * d6 - d5 * d4;
+    if (va <= 0.0 && (d4 - d3) >= 0.0 && (d5 - d6) >= 0.0)
+        return dmad(b, (d4 - d3) / ((d4 - d3) + (d5 - d6)), dsub(c, b));
+    const double den = 1.0 / (va + vb + vc);
+    return dmad(dmad(a, vb * den, ab), vc * den, ac);
+}
+
+// Moves every point to the closest point of the mesh (first face on ties).
+__global__ __launch_bounds__(kBlock) void k_project(MeshDev m, const float* __restrict__ ix,
+    const float* __restrict__ iy, const float* __restrict__ iz, uint32_t n, float* __restrict__ xs,
+    float* __restrict__ ys, float* __restrict__ zs)
+{
+    const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= n)
+        return;
+    const D3 p { ix[k], iy[k], iz[k] };
+    D3 best = p;
+    double bd = INFINITY;
+    for (uint32_t f = 0; f < m.nf; ++f) {
+        const uint32_t i1 = m.f[3 * f], i2 = m.f[3 * f + 1], i3 = m.f[3 * f + 2];
+        const D3 q = closest_on_triangle(p, D3 { m.x[i1], m.y[i1], m.z[i1] }, D3 { m.x[i2], m.y[i2], m.z[i2] },
+            D3 { m.x[i3], m.y[i3], m.z[i3] });
+        const D3 d = dsub(q, p);
+        const double dd = ddot(d, d);
+        if (dd < bd) {
+            bd = dd;
+            best = q;
+        }
+    }
+    xs[k] = static_cast<float>(best.x);
+    ys[k] = static_cast<float>(best.y);
+    zs[k] = static_cast<float>(best.z);
+}
+
+// Rest number density of a hexagonal lattice of spacing h_bdy in a plane,
+// points q a + r b for q, r in [-2, 2] (`_hex_plane_rest_density`,
+// sample_boundary.cu:80-98).
+__global__ void k_hex_rest(KParams kp, float h_bdy, float* out)
+{
+    float rho = 0.f;
+    for (int q = -2; q <= 2; ++q)
+        for (int r = -2; r <= 2; ++r) {
+            const f3 x { h_bdy * (float(q) + 0.5f * float(r)), h_bdy * (0.8660254037844386f * float(r)), 0.f };
+            rho += kW<KB3>(kp, dot(x, x));
+        }
+    *out = rho;
 }
 
 std::string lower(std::string s)
@@ -394,6 +507,90 @@ int reyn_mesh_sample(reyn_ctx* ctx, const reyn_mesh* m, float h, float oversampl
         (void)hipFree(dcdf);
     if (dm.mem)
         (void)hipFree(dm.mem);
+    return rc;
+}
+
+int reyn_mesh_relax(reyn_ctx* ctx, const reyn_mesh* m, float h, float oversampling, float* xs, float* ys,
+    float* zs, uint32_t n, uint32_t iterations, float factor, float* ratio)
+{
+    if (!ctx || !m || (n && (!xs || !ys || !zs)) || !(h > 0.f) || !(oversampling > 0.f))
+        return set_error(ctx, REYN_ERR_INVALID_ARGUMENT, "mesh relax: bad argument");
+    if (n == 0 || iterations == 0)
+        return REYN_OK;
+    REYN_HIP(ctx, hipSetDevice(ctx->device));
+    // the grid reaches 2 h_grid >= the kernel support 2 h_bdy (the reference
+    // takes h_grid = h_bdy * oversampling = h, too short for oversampling < 1)
+    const float h_bdy = static_cast<float>(double(h) / double(oversampling)), h_grid = fmaxf(h, h_bdy);
+    const KParams kp = make_kparams(REYN_KERNEL_B3, 2.f * h_bdy);
+    // bounds of the samples, padded by 5 h_grid as in the reference
+    reyn_float3 lo {}, hi {};
+    float* lv[3] = { &lo.x, &lo.y, &lo.z };
+    float* hv[3] = { &hi.x, &hi.y, &hi.z };
+    const float* arr[3] = { xs, ys, zs };
+    for (int a = 0; a < 3; ++a) {
+        double mn = 0.0, mx = 0.0;
+        REYN_TRY(reyn_reduce_f32(ctx, arr[a], n, 1, &mn));
+        REYN_TRY(reyn_reduce_f32(ctx, arr[a], n, 2, &mx));
+        *lv[a] = static_cast<float>(mn) - 5.f * h_grid;
+        *hv[a] = static_cast<float>(mx) + 5.f * h_grid;
+    }
+    reyn_grid_builder* gb = nullptr;
+    REYN_TRY(reyn_grid_builder_create(ctx, lo, hi, 2.f * h_grid, 0.5f, &gb));
+    DevMesh dm;
+    float* work = nullptr; // tmp x y z, shifted x y z, number density, rest density
+    int rc = upload_mesh(ctx, m, &dm);
+    if (rc == REYN_OK)
+        rc = check_hip(ctx, hipMalloc(reinterpret_cast<void**>(&work), sizeof(float) * (7 * size_t(n) + 1)),
+            "hipMalloc relax");
+    float* tmp[3] = { work, work + n, work + 2 * size_t(n) };
+    float* sh[3] = { work + 3 * size_t(n), work + 4 * size_t(n), work + 5 * size_t(n) };
+    float* nd = work + 6 * size_t(n);
+    float* rest = work + 7 * size_t(n);
+    float n0 = 0.f;
+    if (rc == REYN_OK) {
+        k_hex_rest<<<1, 1, 0, ctx->stream>>>(kp, h_bdy, rest);
+        rc = check_hip(ctx, hipGetLastError(), "k_hex_rest");
+    }
+    if (rc == REYN_OK)
+        rc = check_hip(ctx, hipMemcpyAsync(&n0, rest, sizeof(float), hipMemcpyDeviceToHost, ctx->stream), "copy");
+    if (rc == REYN_OK)
+        rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "sync");
+    const float n0_sq_inv = 1.f / (n0 * n0);
+    // the reorder swaps `cur` with `tmp`; the last positions are copied back
+    // into the caller's arrays when they ended up in the work buffers
+    float* cur[3] = { xs, ys, zs };
+    for (uint32_t it = 0; it < iterations && rc == REYN_OK; ++it) {
+        reyn_grid g {};
+        rc = reyn_grid_construct_and_reorder(ctx, gb, 2.f * h_grid, cur, tmp, 3, n, &g);
+        if (rc != REYN_OK)
+            break;
+        const GridDev gd = to_dev(g);
+        k_relax_density<<<blocks_for(n), kBlock, 0, ctx->stream>>>(gd, kp, cur[0], cur[1], cur[2], n, nd);
+        k_relax_shift<<<blocks_for(n), kBlock, 0, ctx->stream>>>(gd, kp, cur[0], cur[1], cur[2], n, nd, factor,
+            h_bdy, n0_sq_inv, sh[0], sh[1], sh[2]);
+        k_project<<<blocks_for(n), kBlock, 0, ctx->stream>>>(dm.d, sh[0], sh[1], sh[2], n, cur[0], cur[1], cur[2]);
+        rc = check_hip(ctx, hipGetLastError(), "relax kernels");
+    }
+    float* const mine[3] = { xs, ys, zs };
+    for (int a = 0; a < 3 && rc == REYN_OK; ++a)
+        if (cur[a] != mine[a])
+            rc = check_hip(ctx,
+                hipMemcpyAsync(mine[a], cur[a], sizeof(float) * n, hipMemcpyDeviceToDevice, ctx->stream), "copy");
+    if (rc == REYN_OK && ratio) { // spread of the number density before the last shift
+        double mn = 0.0, mx = 0.0;
+        rc = reyn_reduce_f32(ctx, nd, n, 1, &mn);
+        if (rc == REYN_OK)
+            rc = reyn_reduce_f32(ctx, nd, n, 2, &mx);
+        *ratio = static_cast<float>(mx / mn);
+    }
+    if (rc == REYN_OK)
+        rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "sync");
+    (void)hipStreamSynchronize(ctx->stream);
+    if (work)
+        (void)hipFree(work);
+    if (dm.mem)
+        (void)hipFree(dm.mem);
+    reyn_grid_builder_destroy(gb);
     return rc;
 }
 

@@ -1,6 +1,7 @@
 // Headless dam break driven through the C++ host API, in the order of the
-// reference's main loop (src/main.cu:60-78): time step, grid + reorder of the
-// state and the warm-start pressure, IISPH step, bounds.
+// reference's main loop (src/main.cu:59-78): the time-step controller (dt
+// starts at 0.5 ms; lambda 1, 10 target iterations, 0.1 % change per step),
+// grid + reorder of the state and the warm-start pressure, IISPH step, bounds.
 //
 //   dambreak [side=32] [steps=50] [out]
 //
@@ -58,13 +59,14 @@ int main(int argc, char** argv)
     IISPH<B3, Resort::yes> solver(W, uint32_t(N), 0.001f, h, rho_0, rho, aii, si, ax, ay, az, prs, bits);
     UniformGridBuilder builder(bdy.bound_min, bdy.bound_max, 2.f * h);
 
-    float dt = 0.f;
+    float dt = 0.0005f;
+    uint32_t iters = 0;
     double t = 0.0;
     StepTrace trace;
     for (int s = 0; s < steps; ++s) {
-        dt = cfl_time_step(0.4f, h, state, v3(0.f, -9.81f, 0.f));
+        dt = simple_dt_controller(iters, dt, 1.f, h, state, v3(0.f, -9.81f, 0.f), 10, 0.001f);
         const auto grid = builder.construct_and_reorder(2.f * h, tmp, state, &prs);
-        const uint32_t iters = solver.step(state, grid, bdy, dt);
+        iters = solver.step(state, grid, bdy, dt);
         hard_enforce_bounds(bdy.bound_min, bdy.bound_max, state);
         trace.record(dt, iters);
         t += dt;

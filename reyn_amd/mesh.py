@@ -68,14 +68,36 @@ class Mesh:
             raise capi.ReynError(rc, "mesh sample count")
         return int(n.value)
 
-    def sample(self, ctx: Context, h: float, oversampling: float = 1.0, seed: int = 1614201312) -> np.ndarray:
-        """Boundary samples of the surface as a host (n, 3) float32 array."""
+    def sample(self, ctx: Context, h: float, oversampling: float = 1.0, seed: int = 1614201312,
+               relaxation_iters: int = 0, relaxation_factor: float = 1.0) -> np.ndarray:
+        """Boundary samples of the surface as a host (n, 3) float32 array,
+        relaxed `relaxation_iters` times (then in the cell order of the last
+        relaxation grid)."""
         n = self.sample_count(h, oversampling)
         xs, ys, zs = (ctx.empty(max(1, n)) for _ in range(3))
         got = C.c_uint32()
         ctx.check(ctx._lib.reyn_mesh_sample(ctx.h, self.h, float(h), float(oversampling), int(seed) & 0xFFFFFFFF,
                                             xs.ptr, ys.ptr, zs.ptr, n, C.byref(got)), "mesh sample")
+        if relaxation_iters:
+            ctx.check(ctx._lib.reyn_mesh_relax(ctx.h, self.h, float(h), float(oversampling), xs.ptr, ys.ptr, zs.ptr,
+                                               n, int(relaxation_iters), float(relaxation_factor), None),
+                      "mesh relax")
         return np.stack([a.numpy()[:n] for a in (xs, ys, zs)], axis=1)
+
+    def relax(self, ctx: Context, points: np.ndarray, h: float, oversampling: float = 1.0, iterations: int = 1,
+              factor: float = 1.0) -> tuple[np.ndarray, float]:
+        """`relax_sampling` (sample_boundary.cu:100-166) of surface samples:
+        returns the relaxed points (in the cell order of the last relaxation
+        grid) and the max/min number density of the last iteration."""
+        p = np.asarray(points, np.float32)
+        n = p.shape[0]
+        if n == 0:
+            return p.reshape(0, 3), 1.0
+        cols = [ctx.array(np.ascontiguousarray(p[:, a])) for a in range(3)]
+        ratio = C.c_float(0.0)
+        ctx.check(ctx._lib.reyn_mesh_relax(ctx.h, self.h, float(h), float(oversampling), *[c.ptr for c in cols], n,
+                                           int(iterations), float(factor), C.byref(ratio)), "mesh relax")
+        return np.stack([c.numpy() for c in cols], axis=1), float(ratio.value)
 
     def inside(self, ctx: Context, points: np.ndarray) -> np.ndarray:
         """Whether each point lies inside the (closed) mesh."""

@@ -341,6 +341,15 @@ int main()
         CHECK(bdy.xs.size() > 0);
         CHECK(bdy.bound_min.x >= -1.f && bdy.bound_max.x <= 1.f);
         CHECK(bdy.m.min() > 0.0);
+        // relaxed samples stay on the cube's surface and spread more evenly
+        float r1 = 0.f, r8 = 0.f;
+        BoundarySamples one = sample_mesh(cube, 0.1f, 1000.f, 1.f, 1614201312u, Context::current(), 1, 1.f, &r1);
+        BoundarySamples relaxed = sample_mesh(cube, 0.1f, 1000.f, 1.f, 1614201312u, Context::current(), 8, 1.f, &r8);
+        CHECK(relaxed.xs.size() == bdy.xs.size());
+        CHECK(r8 < r1);
+        CHECK(relaxed.bound_min.x >= -1.f && relaxed.bound_max.x <= 1.f);
+        CHECK(relaxed.bound_min.y >= -1.f && relaxed.bound_max.y <= 1.f);
+        CHECK(relaxed.m.min() > 0.0);
     });
 
     run("save_to_ply writes the reference's binary PLY layout", [] {

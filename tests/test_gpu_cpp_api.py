@@ -41,5 +41,10 @@ def test_cpp_dambreak_example(tmp_path):
     rows = (tmp_path / "run.csv").read_text().splitlines()
     assert rows[0] == "step,dt_ms,iterations" and len(rows) == 31
     assert all(int(r.split(",")[2]) >= 3 for r in rows[1:])
+    # the controller's steps: from 0.5 ms, at most 0.1 % larger each step, and
+    # the fluid settles without blowing up (CFL never cuts dt)
+    dts = [float(r.split(",")[1]) for r in rows[1:]]
+    assert dts[0] == pytest.approx(0.5005, rel=1e-5)
+    assert all(b <= a * 1.001 + 1e-9 and b >= a * 0.999 - 1e-9 for a, b in zip(dts, dts[1:]))
     x = read_ply(tmp_path / "run.ply")
     assert x.shape == (20 ** 3, 3) and np.isfinite(x).all()

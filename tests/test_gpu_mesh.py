@@ -1,7 +1,8 @@
 """Meshes on the GPU: surface samples equal the numpy restatement bit for bit
 (oracle/mesh_sampling.py) and lie on the surface, the inside test separates
-the icosphere's interior from its exterior, and a dam break with a mesh
-obstacle steps."""
+the icosphere's interior from its exterior, a dam break with a mesh
+obstacle steps, and the relaxation of the samples matches its float64
+restatement, keeps them on the surface and evens out their number density."""
 
 import math
 from pathlib import Path
@@ -62,3 +63,42 @@ def test_mesh_obstacle_scene_steps(ctx):
     assert all(np.isfinite(d[k]).all() for k in d)
     assert b.info(0).steps == 5
     b.close()
+
+
+@pytest.mark.parametrize("name,h,over,iters,tol", [("cube.obj", 0.1, 1.0, 1, 1e-5), ("cube.obj", 0.1, 1.0, 3, 1e-4),
+                                                    ("ico_sphere.obj", 0.05, 1.5, 2, 1e-4)])
+def test_relax_matches_restatement(ctx, name, h, over, iters, tol):
+    """`relax_sampling` against the float64 restatement: the same points (the
+    library returns them in grid order; matched one to one by nearest point)
+    and the same number-density ratio."""
+    from scipy.spatial import cKDTree
+
+    m = Mesh.load_obj(GOLD / name)
+    v, f = m.arrays()
+    pts = m.sample(ctx, h, over, seed=21)
+    print(f"{name}: relax {pts.shape[0]} samples x {iters}", flush=True)
+    got, ratio = m.relax(ctx, pts, h, over, iters)
+    want, ratio_w = mesh_sampling.relax(pts, v, f, h, over, iters)
+    d, idx = cKDTree(want).query(got.astype(np.float64))
+    print(f"max deviation {d.max() / h:.2e} h, ratio {ratio} vs {ratio_w}", flush=True)
+    assert np.unique(idx).size == got.shape[0]
+    assert d.max() < tol * h
+    assert ratio == pytest.approx(ratio_w, rel=1e-5)
+    # the shift moved the samples at all
+    assert cKDTree(pts).query(got)[0].max() > 1e-3 * h
+
+
+def test_relax_spreads_and_stays_on_surface(ctx):
+    m = Mesh.load_obj(GOLD / "cube.obj")
+    pts = m.sample(ctx, 0.1, 1.0, seed=5)
+    _, r1 = m.relax(ctx, pts, 0.1, 1.0, 1)
+    got, r10 = m.relax(ctx, pts, 0.1, 1.0, 10)
+    print(f"number-density ratio {r1:.3f} -> {r10:.3f}", flush=True)
+    assert r10 < 0.8 * r1
+    assert np.allclose(np.abs(got).max(axis=1), 1.0, atol=1e-6)
+    assert np.abs(got).max() <= 1.0
+    # deterministic: the same input gives the same bits
+    again, r10b = m.relax(ctx, pts, 0.1, 1.0, 10)
+    assert np.array_equal(got, again) and r10 == r10b
+    # through sample(): the same as sampling, then relaxing
+    assert np.array_equal(m.sample(ctx, 0.1, 1.0, seed=5, relaxation_iters=10), got)
