@@ -11,6 +11,7 @@
 //   SESPH<K,R>, PCISPH<K,R>, IISPH<K,R>  src/solvers/*.cuh
 //   cfl_time_step, simple_dt_controller  src/timestep/cfl.cuh:13-32
 //   Mesh, sample_mesh                    src/scene/loader.h, sample_boundary.cu:391-440
+//   LBVH (project, inside)               src/datastructure/lbvh.cuh:614-758, scene.cu:88-140
 //   save_to_ply                          src/scene/ply_io.cu:7-56
 //   StepTrace: the GUI's dt / iteration plots as data   src/gui.cu:956-957
 //   SceneBatch: many scenes per launch (no reference counterpart)
@@ -676,6 +677,50 @@ public:
 private:
     explicit Mesh(reyn_mesh* m) : m_(m) { }
     reyn_mesh* m_ = nullptr;
+};
+
+// `LBVH` (src/datastructure/lbvh.cuh:614-758) of a mesh on the context's GPU:
+// closest-point projection and the inside test of the fluid culling.
+class LBVH {
+public:
+    explicit LBVH(const Mesh& mesh, Context& ctx = Context::current()) : ctx_(&ctx)
+    {
+        ctx.check(reyn_lbvh_create(ctx.get(), mesh.get(), &b_), "LBVH");
+    }
+    LBVH(LBVH&& o) noexcept : ctx_(o.ctx_), b_(std::exchange(o.b_, nullptr)) { }
+    LBVH(const LBVH&) = delete;
+    LBVH& operator=(const LBVH&) = delete;
+    LBVH& operator=(LBVH&&) = delete;
+    ~LBVH() { reyn_lbvh_destroy(b_); }
+    uint32_t height() const
+    {
+        uint32_t h = 0;
+        reyn_lbvh_info(b_, nullptr, &h);
+        return h;
+    }
+    // every point to the closest point of the mesh, in place
+    void project(DeviceBuffer<float>& xs, DeviceBuffer<float>& ys, DeviceBuffer<float>& zs) const
+    {
+        if (ys.size() != xs.size() || zs.size() != xs.size())
+            throw std::runtime_error("LBVH::project called with coordinate buffers of differing sizes");
+        ctx_->check(reyn_lbvh_project(ctx_->get(), b_, xs.ptr(), ys.ptr(), zs.ptr(), uint32_t(xs.size())),
+            "LBVH::project");
+    }
+    // inside[k] = 1 for points inside the closed mesh
+    void inside(const DeviceBuffer<float>& xs, const DeviceBuffer<float>& ys, const DeviceBuffer<float>& zs,
+        DeviceBuffer<uint8_t>& out) const
+    {
+        if (ys.size() != xs.size() || zs.size() != xs.size() || out.size() != xs.size())
+            throw std::runtime_error("LBVH::inside called with buffers of differing sizes");
+        ctx_->check(
+            reyn_lbvh_inside(ctx_->get(), b_, xs.ptr(), ys.ptr(), zs.ptr(), uint32_t(xs.size()), out.ptr()),
+            "LBVH::inside");
+    }
+    const reyn_lbvh* get() const { return b_; }
+
+private:
+    Context* ctx_;
+    reyn_lbvh* b_ = nullptr;
 };
 
 // boundary samples of the mesh surface with Akinci masses; `relaxation_iters`

@@ -444,15 +444,54 @@ int reyn_mesh_sample(reyn_ctx* ctx, const reyn_mesh* m, float h, float oversampl
  * per iteration a grid over the samples, their B3 number densities (support
  * 2 h_bdy), a shift -factor h_bdy^2 sum_j (n_i + n_j) / n_0^2 grad W_ij with n_0
  * the hexagonal-plane rest density, and the projection of every shifted sample
- * to the closest point of the mesh (all faces, in double; the reference uses
- * an LBVH). `ratio` (may be NULL) receives max/min number density of the last
+ * to the closest point of the mesh (through an LBVH built for the call).
+ * `ratio` (may be NULL) receives max/min number density of the last
  * iteration. */
 int reyn_mesh_relax(reyn_ctx* ctx, const reyn_mesh* m, float h, float oversampling,
     float* xs, float* ys, float* zs, uint32_t n, uint32_t iterations, float factor, float* ratio);
+/* Moves every point (device arrays, in place) to the closest point of the
+ * mesh by testing every face in double; among faces at the same distance the
+ * lowest index. The brute-force counterpart of `reyn_lbvh_project` (the
+ * reference tests its LBVH against such a kernel, lbvh.cu:278-311). */
+int reyn_mesh_project(reyn_ctx* ctx, const reyn_mesh* m, float* xs, float* ys, float* zs,
+    uint32_t n);
 /* inside[k] = 1 when the generalized winding number of the mesh at point k
- * (device arrays) exceeds 1/2: the reference's inside predicate for culling
- * fluid (scene.cu:59-140), evaluated over every face instead of an LBVH. */
+ * (device arrays) exceeds 1/2, summed over every face: exact for any closed
+ * mesh, O(points x faces). `reyn_lbvh_inside` is the fast form. */
 int reyn_mesh_inside(reyn_ctx* ctx, const reyn_mesh* m, const float* xs, const float* ys,
+    const float* zs, uint32_t n, uint8_t* inside);
+
+/* Linear BVH over the faces of a mesh, built on the GPU of `ctx` (`LBVH`,
+ * src/datastructure/lbvh.cuh:614-758): face centroids' 30-bit Morton codes
+ * in the mesh's box, made unique by the face index (code << 32 | face), a
+ * radix sort, the binary radix tree of the sorted keys and the boxes
+ * bottom-up (double, exact). It keeps its own copy of the mesh. */
+typedef struct reyn_lbvh reyn_lbvh;
+int reyn_lbvh_create(reyn_ctx* ctx, const reyn_mesh* m, reyn_lbvh** out);
+int reyn_lbvh_destroy(reyn_lbvh* b);
+/* Number of faces (leaves) and the tree height (levels from the root to the
+ * deepest leaf, the leaf included). */
+int reyn_lbvh_info(const reyn_lbvh* b, uint32_t* n_faces, uint32_t* height);
+/* Host copies of the tree (any may be NULL): for internal node i (n_faces - 1
+ * of them, the root is 0) its children left[i], right[i] as index << 1 | 1 for
+ * an internal node and index << 1 for a leaf (the reference's `ChildNode`,
+ * lbvh.cuh:162-193); leaf k's face; boxes as lo x, y, z, hi x, y, z, 6 doubles
+ * per internal node and per leaf. */
+int reyn_lbvh_nodes(reyn_ctx* ctx, const reyn_lbvh* b, uint32_t* left, uint32_t* right,
+    uint32_t* leaf_face, double* internal_box, double* leaf_box);
+/* `LBVH::project` (lbvh.cuh:728-742): every point (device arrays, in place)
+ * to the closest point of the mesh; the same result as `reyn_mesh_project`.
+ * Asynchronous on the context's stream. */
+int reyn_lbvh_project(reyn_ctx* ctx, const reyn_lbvh* b, float* xs, float* ys, float* zs,
+    uint32_t n);
+/* The reference's `InsidePredicate` (scene.cu:88-140) through the LBVH:
+ * inside[k] = 1 when point k lies behind the closest feature of the surface,
+ * judged by the angle-weighted pseudo-normal of the face, edge or vertex that
+ * holds the closest point (the reference reads the closest face's normal);
+ * points on the surface count as outside. For closed, consistently oriented
+ * meshes with shared vertices it agrees with `reyn_mesh_inside`.
+ * Asynchronous on the context's stream. */
+int reyn_lbvh_inside(reyn_ctx* ctx, const reyn_lbvh* b, const float* xs, const float* ys,
     const float* zs, uint32_t n, uint8_t* inside);
 
 /* ---- scene batches ---------------------------------------------------------

@@ -234,6 +234,13 @@ _SIGS = {
     "reyn_mesh_sample": (i32, [vp, vp, f32, f32, u32, vp, vp, vp, u32, P(u32)]),
     "reyn_mesh_inside": (i32, [vp, vp, vp, vp, vp, u32, vp]),
     "reyn_mesh_relax": (i32, [vp, vp, f32, f32, vp, vp, vp, u32, u32, f32, P(f32)]),
+    "reyn_mesh_project": (i32, [vp, vp, vp, vp, vp, u32]),
+    "reyn_lbvh_create": (i32, [vp, vp, P(vp)]),
+    "reyn_lbvh_destroy": (i32, [vp]),
+    "reyn_lbvh_info": (i32, [vp, P(u32), P(u32)]),
+    "reyn_lbvh_nodes": (i32, [vp, vp, vp, vp, vp, vp, vp]),
+    "reyn_lbvh_project": (i32, [vp, vp, vp, vp, vp, u32]),
+    "reyn_lbvh_inside": (i32, [vp, vp, vp, vp, vp, u32, vp]),
     "reyn_grid_occupancy": (i32, [vp, P(Grid), vp]),
     "reyn_device_memory": (i32, [vp, P(u64), P(u64)]),
     "reyn_batch_info": (i32, [vp, u32, P(BatchSceneInfo)]),

@@ -22,16 +22,12 @@
 #include <vector>
 
 #include "common.hpp"
+#include "mesh.hpp"
 #include "sph.hpp"
 
-struct reyn_mesh {
-    std::vector<double> x, y, z;
-    std::vector<uint32_t> f; // 3 per face
-};
-
-// No fused multiply-adds anywhere in this file: the samples and the area CDF
-// are formed operation by operation, as a restatement in numpy forms them.
-#pragma clang fp contract(off)
+// No fused multiply-adds anywhere in this file (mesh.hpp turns contraction
+// off): the samples and the area CDF are formed operation by operation, as a
+// restatement in numpy forms them.
 
 namespace reyn {
 namespace {
@@ -52,12 +48,6 @@ __device__ inline double mhash(uint32_t seed, uint32_t idx, uint32_t axis)
     const uint32_t h = mmix32(mmix32(seed * 0x9e3779b9u + 0x7f4a7c15u) ^ (idx * 3u + axis));
     return double(h >> 8) * (1.0 / 16777216.0);
 }
-
-struct MeshDev {
-    const double *x, *y, *z;
-    const uint32_t* f;
-    uint32_t nf;
-};
 
 // Sample i of n: a stratified area coordinate, the face whose inclusive CDF
 // entry is the first >= it, and a uniform point of that face (s, t folded into
@@ -101,19 +91,8 @@ __global__ __launch_bounds__(kBlock) void k_mesh_inside(MeshDev m, const float* 
         return;
     const double px = xs[k], py = ys[k], pz = zs[k];
     double w = 0.0;
-    for (uint32_t f = 0; f < m.nf; ++f) {
-        const uint32_t i1 = m.f[3 * f], i2 = m.f[3 * f + 1], i3 = m.f[3 * f + 2];
-        const double ax = m.x[i1] - px, ay = m.y[i1] - py, az = m.z[i1] - pz;
-        const double bx = m.x[i2] - px, by = m.y[i2] - py, bz = m.z[i2] - pz;
-        const double cx = m.x[i3] - px, cy = m.y[i3] - py, cz = m.z[i3] - pz;
-        const double la = sqrt(ax * ax + ay * ay + az * az);
-        const double lb = sqrt(bx * bx + by * by + bz * bz);
-        const double lc = sqrt(cx * cx + cy * cy + cz * cz);
-        const double det = ax * (by * cz - bz * cy) - ay * (bx * cz - bz * cx) + az * (bx * cy - by * cx);
-        const double den = la * lb * lc + (ax * bx + ay * by + az * bz) * lc + (ax * cx + ay * cy + az * cz) * lb
-            + (bx * cx + by * cy + bz * cz) * la;
-        w += 2.0 * atan2(det, den);
-    }
+    for (uint32_t f = 0; f < m.nf; ++f)
+        w += solid_angle2(m, f, px, py, pz);
     inside[k] = w / (4.0 * M_PI) > 0.5 ? 1 : 0;
 }
 
@@ -152,42 +131,6 @@ __global__ __launch_bounds__(kBlock) void k_relax_shift(GridDev g, KParams kp, c
     st3(ox, oy, oz, i, xi + (-k * h_bdy * h_bdy) * acc);
 }
 
-struct D3 {
-    double x, y, z;
-};
-__device__ inline D3 dsub(D3 a, D3 b) { return { a.x - b.x, a.y - b.y, a.z - b.z }; }
-__device__ inline double ddot(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-__device__ inline D3 dmad(D3 a, double s, D3 b) { return { a.x + s * b.x, a.y + s * b.y, a.z + s * b.z }; }
-
-// Closest point of triangle (a, b, c) to p, by the Voronoi regions of its
-// vertices, edges and face.
-__device__ D3 closest_on_triangle(D3 p, D3 a, D3 b, D3 c)
-{
-    const D3 ab = dsub(b, a), ac = dsub(c, a), ap = dsub(p, a);
-    const double d1 = ddot(ab, ap), d2 = ddot(ac, ap);
-    if (d1 <= 0.0 && d2 <= 0.0)
-        return a;
-    const D3 bp = dsub(p, b);
-    const double d3 = ddot(ab, bp), d4 = ddot(ac, bp);
-    if (d3 >= 0.0 && d4 <= d3)
-        return b;
-    const double vc = d1 * d4 - d3 * d2;
-    if (vc <= 0.0 && d1 >= 0.0 && d3 <= 0.0)
-        return dmad(a, d1 / (d1 - d3), ab);
-    const D3 cp = dsub(p, c);
-    const double d5 = ddot(ab, cp), d6 = ddot(ac, cp);
-    if (d6 >= 0.0 && d5 <= d6)
-        return c;
-    const double vb = d5 * d2 - d1 * d6;
-    if (vb <= 0.0 && d2 >= 0.0 && d6 <= 0.0)
-        return dmad(a, d2 / (d2 - d6), ac);
-    const double va = d3 * d6 - d5 * d4;
-    if (va <= 0.0 && (d4 - d3) >= 0.0 && (d5 - d6) >= 0.0)
-        return dmad(b, (d4 - d3) / ((d4 - d3) + (d5 - d6)), dsub(c, b));
-    const double den = 1.0 / (va + vb + vc);
-    return dmad(dmad(a, vb * den, ab), vc * den, ac);
-}
-
 // Moves every point to the closest point of the mesh (first face on ties).
 __global__ __launch_bounds__(kBlock) void k_project(MeshDev m, const float* __restrict__ ix,
     const float* __restrict__ iy, const float* __restrict__ iz, uint32_t n, float* __restrict__ xs,
@@ -200,11 +143,9 @@ __global__ __launch_bounds__(kBlock) void k_project(MeshDev m, const float* __re
     D3 best = p;
     double bd = INFINITY;
     for (uint32_t f = 0; f < m.nf; ++f) {
-        const uint32_t i1 = m.f[3 * f], i2 = m.f[3 * f + 1], i3 = m.f[3 * f + 2];
-        const D3 q = closest_on_triangle(p, D3 { m.x[i1], m.y[i1], m.z[i1] }, D3 { m.x[i2], m.y[i2], m.z[i2] },
-            D3 { m.x[i3], m.y[i3], m.z[i3] });
-        const D3 d = dsub(q, p);
-        const double dd = ddot(d, d);
+        double dd;
+        int feature;
+        const D3 q = closest_on_face(m, f, p, &dd, &feature);
         if (dd < bd) {
             bd = dd;
             best = q;
@@ -323,10 +264,7 @@ int load_obj(const char* path, const char* const* ignore, uint32_t n_ignore, con
     return REYN_OK;
 }
 
-struct DevMesh {
-    void* mem = nullptr;
-    MeshDev d {};
-};
+} // namespace
 
 int upload_mesh(reyn_ctx* ctx, const reyn_mesh* m, DevMesh* out)
 {
@@ -343,6 +281,8 @@ int upload_mesh(reyn_ctx* ctx, const reyn_mesh* m, DevMesh* out)
     out->d = MeshDev { x, x + nv, x + 2 * nv, f, static_cast<uint32_t>(nf) };
     return REYN_OK;
 }
+
+namespace {
 
 void face_areas(const reyn_mesh* m, std::vector<double>* cdf)
 {
@@ -534,11 +474,11 @@ int reyn_mesh_relax(reyn_ctx* ctx, const reyn_mesh* m, float h, float oversampli
         *lv[a] = static_cast<float>(mn) - 5.f * h_grid;
         *hv[a] = static_cast<float>(mx) + 5.f * h_grid;
     }
+    reyn_lbvh* bvh = nullptr;
+    REYN_TRY(reyn_lbvh_create(ctx, m, &bvh));
     reyn_grid_builder* gb = nullptr;
-    REYN_TRY(reyn_grid_builder_create(ctx, lo, hi, 2.f * h_grid, 0.5f, &gb));
-    DevMesh dm;
+    int rc = reyn_grid_builder_create(ctx, lo, hi, 2.f * h_grid, 0.5f, &gb);
     float* work = nullptr; // tmp x y z, shifted x y z, number density, rest density
-    int rc = upload_mesh(ctx, m, &dm);
     if (rc == REYN_OK)
         rc = check_hip(ctx, hipMalloc(reinterpret_cast<void**>(&work), sizeof(float) * (7 * size_t(n) + 1)),
             "hipMalloc relax");
@@ -556,8 +496,9 @@ int reyn_mesh_relax(reyn_ctx* ctx, const reyn_mesh* m, float h, float oversampli
     if (rc == REYN_OK)
         rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "sync");
     const float n0_sq_inv = 1.f / (n0 * n0);
-    // the reorder swaps `cur` with `tmp`; the last positions are copied back
-    // into the caller's arrays when they ended up in the work buffers
+    // the reorder swaps `cur` with `tmp` and the projection `cur` with `sh`;
+    // the last positions are copied back into the caller's arrays when they
+    // ended up in the work buffers
     float* cur[3] = { xs, ys, zs };
     for (uint32_t it = 0; it < iterations && rc == REYN_OK; ++it) {
         reyn_grid g {};
@@ -568,8 +509,11 @@ int reyn_mesh_relax(reyn_ctx* ctx, const reyn_mesh* m, float h, float oversampli
         k_relax_density<<<blocks_for(n), kBlock, 0, ctx->stream>>>(gd, kp, cur[0], cur[1], cur[2], n, nd);
         k_relax_shift<<<blocks_for(n), kBlock, 0, ctx->stream>>>(gd, kp, cur[0], cur[1], cur[2], n, nd, factor,
             h_bdy, n0_sq_inv, sh[0], sh[1], sh[2]);
-        k_project<<<blocks_for(n), kBlock, 0, ctx->stream>>>(dm.d, sh[0], sh[1], sh[2], n, cur[0], cur[1], cur[2]);
         rc = check_hip(ctx, hipGetLastError(), "relax kernels");
+        if (rc == REYN_OK)
+            rc = reyn_lbvh_project(ctx, bvh, sh[0], sh[1], sh[2], n);
+        for (int a = 0; a < 3; ++a)
+            std::swap(cur[a], sh[a]);
     }
     float* const mine[3] = { xs, ys, zs };
     for (int a = 0; a < 3 && rc == REYN_OK; ++a)
@@ -588,9 +532,40 @@ int reyn_mesh_relax(reyn_ctx* ctx, const reyn_mesh* m, float h, float oversampli
     (void)hipStreamSynchronize(ctx->stream);
     if (work)
         (void)hipFree(work);
+    reyn_lbvh_destroy(bvh);
+    reyn_grid_builder_destroy(gb);
+    return rc;
+}
+
+int reyn_mesh_project(reyn_ctx* ctx, const reyn_mesh* m, float* xs, float* ys, float* zs, uint32_t n)
+{
+    if (!ctx || !m || (n && (!xs || !ys || !zs)))
+        return set_error(ctx, REYN_ERR_INVALID_ARGUMENT, "mesh project: NULL argument");
+    if (n == 0)
+        return REYN_OK;
+    if (m->f.empty())
+        return set_error(ctx, REYN_ERR_INVALID_ARGUMENT, "mesh project: the mesh has no faces");
+    REYN_HIP(ctx, hipSetDevice(ctx->device));
+    DevMesh dm;
+    float* in = nullptr;
+    int rc = upload_mesh(ctx, m, &dm);
+    if (rc == REYN_OK)
+        rc = check_hip(ctx, hipMalloc(reinterpret_cast<void**>(&in), sizeof(float) * 3 * size_t(n)), "hipMalloc");
+    float* const io[3] = { xs, ys, zs };
+    for (int a = 0; a < 3 && rc == REYN_OK; ++a)
+        rc = check_hip(ctx, hipMemcpyAsync(in + a * size_t(n), io[a], sizeof(float) * n, hipMemcpyDeviceToDevice,
+                                ctx->stream), "copy");
+    if (rc == REYN_OK) {
+        k_project<<<blocks_for(n), kBlock, 0, ctx->stream>>>(dm.d, in, in + n, in + 2 * size_t(n), n, xs, ys, zs);
+        rc = check_hip(ctx, hipGetLastError(), "k_project");
+    }
+    if (rc == REYN_OK)
+        rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    (void)hipStreamSynchronize(ctx->stream);
+    if (in)
+        (void)hipFree(in);
     if (dm.mem)
         (void)hipFree(dm.mem);
-    reyn_grid_builder_destroy(gb);
     return rc;
 }
 

@@ -1,10 +1,11 @@
-"""Triangle meshes (`reyn_mesh_*`): OBJ loading, surface sampling on the GPU
-and the inside test, and a dam-break scene with a mesh obstacle.
+"""Triangle meshes (`reyn_mesh_*`, `reyn_lbvh_*`): OBJ loading, surface
+sampling and relaxation on the GPU, the LBVH with its closest-point and
+inside queries, and a dam-break scene with a mesh obstacle.
 
 The reference loads meshes with `load_mesh_from_obj` (src/scene/loader.cpp:
-13-107), samples boundaries with `sample_mesh` (sample_boundary.cu:391-440)
-and culls fluid with its `CullPredicate` / `InsidePredicate`
-(scene.cu:55-140).
+13-107), samples boundaries with `sample_mesh` (sample_boundary.cu:391-440),
+builds an `LBVH` (src/datastructure/lbvh.cuh:614-758) and culls fluid with
+its `CullPredicate` / `InsidePredicate` (scene.cu:55-140).
 """
 
 from __future__ import annotations
@@ -99,8 +100,18 @@ class Mesh:
                                            int(iterations), float(factor), C.byref(ratio)), "mesh relax")
         return np.stack([c.numpy() for c in cols], axis=1), float(ratio.value)
 
+    def project(self, ctx: Context, points: np.ndarray) -> np.ndarray:
+        """Closest points of the mesh, every face tested (`reyn_mesh_project`)."""
+        p = np.asarray(points, np.float32)
+        if p.shape[0] == 0:
+            return p.reshape(0, 3)
+        cols = [ctx.array(np.ascontiguousarray(p[:, a])) for a in range(3)]
+        ctx.check(ctx._lib.reyn_mesh_project(ctx.h, self.h, *[c.ptr for c in cols], p.shape[0]), "mesh project")
+        return np.stack([c.numpy() for c in cols], axis=1)
+
     def inside(self, ctx: Context, points: np.ndarray) -> np.ndarray:
-        """Whether each point lies inside the (closed) mesh."""
+        """Whether each point lies inside the (closed) mesh: the generalized
+        winding number over every face."""
         p = np.asarray(points, np.float32)
         n = p.shape[0]
         if n == 0:
@@ -113,6 +124,67 @@ class Mesh:
     def close(self) -> None:
         if self.h is not None and self.h.value:
             self._lib.reyn_mesh_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class LBVH:
+    """`LBVH` (lbvh.cuh:614-758) of a mesh on the context's GPU."""
+
+    def __init__(self, ctx: Context, mesh: Mesh):
+        self.ctx = ctx
+        self._lib = ctx._lib
+        self.h = C.c_void_p()
+        ctx.check(self._lib.reyn_lbvh_create(ctx.h, mesh.h, C.byref(self.h)), "lbvh create")
+
+    def info(self) -> dict:
+        nf, height = C.c_uint32(), C.c_uint32()
+        self._lib.reyn_lbvh_info(self.h, C.byref(nf), C.byref(height))
+        return {"faces": nf.value, "height": height.value}
+
+    def nodes(self) -> dict:
+        """The tree on the host: children (index << 1 | 1 for internal nodes),
+        leaf faces, and boxes (lo, hi) of the internal nodes and leaves."""
+        nf = self.info()["faces"]
+        ni = max(nf - 1, 0)
+        left, right = np.zeros(ni, np.uint32), np.zeros(ni, np.uint32)
+        leaf = np.zeros(nf, np.uint32)
+        ibox, lbox = np.zeros((ni, 6)), np.zeros((nf, 6))
+        self.ctx.check(self._lib.reyn_lbvh_nodes(self.ctx.h, self.h, left.ctypes.data, right.ctypes.data,
+                                                 leaf.ctypes.data, ibox.ctypes.data, lbox.ctypes.data), "lbvh nodes")
+        return {"left": left, "right": right, "leaf_face": leaf, "internal_box": ibox, "leaf_box": lbox}
+
+    def project(self, points: np.ndarray) -> np.ndarray:
+        """Closest points of the mesh (`LBVH::project`)."""
+        p = np.asarray(points, np.float32)
+        if p.shape[0] == 0:
+            return p.reshape(0, 3)
+        cols = [self.ctx.array(np.ascontiguousarray(p[:, a])) for a in range(3)]
+        self.ctx.check(self._lib.reyn_lbvh_project(self.ctx.h, self.h, *[c.ptr for c in cols], p.shape[0]),
+                       "lbvh project")
+        return np.stack([c.numpy() for c in cols], axis=1)
+
+    def inside(self, points: np.ndarray) -> np.ndarray:
+        """Whether each point lies inside the (closed) mesh, from the
+        pseudo-normal of its closest feature."""
+        p = np.asarray(points, np.float32)
+        n = p.shape[0]
+        if n == 0:
+            return np.zeros(0, bool)
+        cols = [self.ctx.array(np.ascontiguousarray(p[:, a])) for a in range(3)]
+        out = self.ctx.empty(n, np.uint8)
+        self.ctx.check(self._lib.reyn_lbvh_inside(self.ctx.h, self.h, *[c.ptr for c in cols], n, out.ptr),
+                       "lbvh inside")
+        return out.numpy().astype(bool)
+
+    def close(self) -> None:
+        if self.h is not None and self.h.value:
+            self._lib.reyn_lbvh_destroy(self.h)
         self.h = None
 
     def __del__(self):
@@ -137,7 +209,9 @@ def obstacle_scene(ctx: Context, mesh: Mesh, fluid_counts, spacing: float, box_h
     s = float(spacing)
     surf = placed.sample(ctx, s, oversampling)
     x = lattice(fluid_counts, s, (0.0, 0.0, 0.0), jitter, seed)
-    keep = ~placed.inside(ctx, x)
+    bvh = LBVH(ctx, placed)
+    keep = ~bvh.inside(x)
+    bvh.close()
     if surf.shape[0]:
         d, _ = cKDTree(surf.astype(np.float64)).query(x.astype(np.float64), k=1)
         keep &= d > cull_radius * s
@@ -149,4 +223,4 @@ def obstacle_scene(ctx: Context, mesh: Mesh, fluid_counts, spacing: float, box_h
                        "mesh_samples": int(surf.shape[0]), "culled": int((~keep).sum())})
 
 
-__all__ = ["Mesh", "obstacle_scene"]
+__all__ = ["LBVH", "Mesh", "obstacle_scene"]

@@ -350,6 +350,22 @@ int main()
         CHECK(relaxed.bound_min.x >= -1.f && relaxed.bound_max.x <= 1.f);
         CHECK(relaxed.bound_min.y >= -1.f && relaxed.bound_max.y <= 1.f);
         CHECK(relaxed.m.min() > 0.0);
+        // the LBVH: projection to the cube's faces and the inside test
+        const LBVH bvh(cube);
+        CHECK(bvh.height() >= 4u && bvh.height() <= 12u);
+        DeviceBuffer<float> px(3), py(3), pz(3);
+        px.upload(std::vector<float> { 2.f, 0.25f, -0.5f });
+        py.upload(std::vector<float> { 0.5f, 0.1f, 0.f });
+        pz.upload(std::vector<float> { -0.5f, 0.2f, 3.f });
+        DeviceBuffer<uint8_t> in(3);
+        bvh.inside(px, py, pz, in);
+        const std::vector<uint8_t> inv = in.download();
+        CHECK(inv[0] == 0 && inv[1] == 1 && inv[2] == 0);
+        bvh.project(px, py, pz);
+        const std::vector<float> qx = px.download(), qy = py.download(), qz = pz.download();
+        CHECK(qx[0] == 1.f && qy[0] == 0.5f && qz[0] == -0.5f);
+        CHECK(qx[1] == 1.f && qy[1] == 0.1f && qz[1] == 0.2f);
+        CHECK(qx[2] == -0.5f && qy[2] == 0.f && qz[2] == 1.f);
     });
 
     run("save_to_ply writes the reference's binary PLY layout", [] {

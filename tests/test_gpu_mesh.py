@@ -102,3 +102,123 @@ def test_relax_spreads_and_stays_on_surface(ctx):
     assert np.array_equal(got, again) and r10 == r10b
     # through sample(): the same as sampling, then relaxing
     assert np.array_equal(m.sample(ctx, 0.1, 1.0, seed=5, relaxation_iters=10), got)
+
+
+# ---- the LBVH ---------------------------------------------------------------------
+
+
+def torus(nu: int, nv: int, big: float = 1.0, small: float = 0.35) -> tuple[np.ndarray, np.ndarray]:
+    """A closed, outward-oriented torus of 2 nu nv faces."""
+    u = 2 * np.pi * np.arange(nu) / nu
+    v = 2 * np.pi * np.arange(nv) / nv
+    uu, vv = np.meshgrid(u, v, indexing="ij")
+    verts = np.stack([(big + small * np.cos(vv)) * np.cos(uu), (big + small * np.cos(vv)) * np.sin(uu),
+                      small * np.sin(vv)], axis=-1).reshape(-1, 3)
+    i, j = np.meshgrid(np.arange(nu), np.arange(nv), indexing="ij")
+    a = i * nv + j
+    b = ((i + 1) % nu) * nv + j
+    c = ((i + 1) % nu) * nv + (j + 1) % nv
+    d = i * nv + (j + 1) % nv
+    faces = np.concatenate([np.stack([a, b, c], -1).reshape(-1, 3), np.stack([a, c, d], -1).reshape(-1, 3)])
+    m = Mesh.from_arrays(verts, faces)
+    if m.info()["volume"] < 0:
+        faces = faces[:, ::-1]
+    m.close()
+    return verts, np.ascontiguousarray(faces)
+
+
+def _meshes():
+    v, f = torus(120, 60)
+    return {"cube": Mesh.load_obj(GOLD / "cube.obj"), "ico_sphere": Mesh.load_obj(GOLD / "ico_sphere.obj"),
+            "torus": Mesh.from_arrays(v, f)}
+
+
+def _check_tree(nodes, v, f):
+    """The reference's LBVH checks (lbvh.cu:314-384): every box holds its
+    children's boxes and each leaf box its face; every leaf and every internal
+    node but the root is referenced exactly once."""
+    left, right, leaf = nodes["left"], nodes["right"], nodes["leaf_face"]
+    ib, lb = nodes["internal_box"], nodes["leaf_box"]
+    nf = leaf.size
+    assert np.array_equal(np.sort(leaf), np.arange(nf))
+    ch = np.concatenate([left, right])
+    is_int = (ch & 1) == 1
+    assert np.array_equal(np.sort(ch[~is_int] >> 1), np.arange(nf))
+    assert np.array_equal(np.sort(ch[is_int] >> 1), np.arange(1, nf - 1))
+    for side in (left, right):
+        box = np.where(((side & 1) == 1)[:, None], ib[np.minimum(side >> 1, max(nf - 2, 0))], lb[side >> 1])
+        assert np.all(ib[:, :3] <= box[:, :3]) and np.all(ib[:, 3:] >= box[:, 3:])
+    tri = v[f[leaf]]  # leaf k's vertices
+    assert np.array_equal(lb[:, :3], tri.min(axis=1)) and np.array_equal(lb[:, 3:], tri.max(axis=1))
+
+
+@pytest.mark.parametrize("name", ["cube", "ico_sphere", "torus"])
+def test_lbvh_structure(ctx, name):
+    from reyn_amd.mesh import LBVH
+
+    m = _meshes()[name]
+    v, f = m.arrays()
+    bvh = LBVH(ctx, m)
+    inf = bvh.info()
+    print(f"{name}: {inf['faces']} faces, height {inf['height']}", flush=True)
+    assert inf["faces"] == f.shape[0]
+    assert math.ceil(math.log2(f.shape[0])) + 1 <= inf["height"] <= 64
+    _check_tree(bvh.nodes(), v, f)
+
+
+def test_lbvh_degenerate_trees(ctx):
+    """One face, two faces, and many faces with the same centroid (equal
+    Morton codes, told apart by the face index)."""
+    from reyn_amd.mesh import LBVH
+
+    tri = np.array([[0.0, 0, 0], [1, 0, 0], [0, 1, 0]])
+    for faces in (np.array([[0, 1, 2]]), np.array([[0, 1, 2], [0, 2, 1]]), np.tile([[0, 1, 2]], (300, 1))):
+        m = Mesh.from_arrays(tri, faces)
+        bvh = LBVH(ctx, m)
+        if faces.shape[0] > 1:
+            _check_tree(bvh.nodes(), tri, faces)
+        p = np.array([[0.2, 0.2, 1.0], [2.0, 2.0, 0.0]], np.float32)
+        assert np.array_equal(bvh.project(p), m.project(ctx, p))
+
+
+@pytest.mark.parametrize("name", ["cube", "ico_sphere", "torus"])
+def test_lbvh_project_matches_brute_force(ctx, name):
+    """`LBVH::project` against every face tested in order (lbvh.cu:386-461,
+    10,000 points in the mesh's box): the same bits; and against the float64
+    restatement within rounding."""
+    from reyn_amd.mesh import LBVH
+
+    m = _meshes()[name]
+    v, f = m.arrays()
+    rng = np.random.default_rng(420)
+    lo, hi = v.min(0), v.max(0)
+    p = (lo + (hi - lo) * rng.uniform(-0.1, 1.1, (10000, 3))).astype(np.float32)
+    bvh = LBVH(ctx, m)
+    got = bvh.project(p)
+    assert np.array_equal(got, m.project(ctx, p))
+    k = 2000 if f.shape[0] < 2000 else 300  # the restatement loops over the faces
+    want = mesh_sampling.closest_points(p[:k], v, f)
+    assert np.abs(got[:k] - want).max() <= 4e-7 * np.abs(v).max()
+    # distances are never larger than to any vertex
+    dv = np.min(np.linalg.norm(p[:500, None, :].astype(np.float64) - v[None], axis=2), axis=1)
+    assert np.all(np.linalg.norm(got[:500] - p[:500], axis=1) <= dv + 1e-6)
+
+
+@pytest.mark.parametrize("name", ["cube", "ico_sphere", "torus"])
+def test_lbvh_inside_matches_winding_number(ctx, name):
+    from reyn_amd.mesh import LBVH
+
+    m = _meshes()[name]
+    v, f = m.arrays()
+    rng = np.random.default_rng(9)
+    lo, hi = v.min(0), v.max(0)
+    p = (lo + (hi - lo) * rng.uniform(-0.2, 1.2, (20000, 3))).astype(np.float32)
+    bvh = LBVH(ctx, m)
+    near = np.linalg.norm(bvh.project(p) - p, axis=1) < 1e-5
+    a, b = bvh.inside(p), m.inside(ctx, p)
+    print(f"{name}: {a.sum()} of {p.shape[0]} inside, {near.sum()} on the surface", flush=True)
+    assert a.sum() > 1000 and (~a).sum() > 1000
+    assert np.array_equal(a[~near], b[~near])
+    if name == "torus":  # the inscribed polyhedron lies within the smooth torus
+        r = np.hypot(np.hypot(p[:, 0], p[:, 1]) - 1.0, p[:, 2])
+        assert np.all(a[r < 0.34]) and not np.any(a[r > 0.36])
