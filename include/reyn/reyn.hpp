@@ -796,6 +796,7 @@ struct StepTrace {
         if (!out)
             throw std::runtime_error("StepTrace: cannot open " + path.string());
         out << "step,dt_ms,iterations\n";
+        out.precision(9); // float values round-trip
         for (size_t k = 0; k < dt_ms.size(); ++k)
             out << k + 1 << ',' << dt_ms[k] << ',' << iters[k] << '\n';
     }
