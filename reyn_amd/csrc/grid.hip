@@ -23,6 +23,11 @@ constexpr uint32_t kScanTile = kBlock * kScanItems; // 4096 cells per scan block
 // cells holding more points than this are left in slot order (and reported):
 // their O(c^2) ranking would be unbounded for degenerate inputs
 constexpr uint32_t kMaxStableCell = 4096;
+// records per staging window (two per wavefront) of the query walks
+#ifndef REYN_QSTAGE
+#define REYN_QSTAGE 96
+#endif
+constexpr uint32_t kQueryStage = REYN_QSTAGE;
 
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
 {
@@ -283,19 +288,30 @@ __global__ __launch_bounds__(kBlock) void k_query_stats(GridDev g, const float* 
     uint32_t* __restrict__ count, float* __restrict__ len2, float* __restrict__ vx,
     float* __restrict__ vy, float* __restrict__ vz)
 {
+    // every lane stays until the walk is done (the staged walk reduces over
+    // the wavefront); lanes past the end visit nothing
     const uint32_t t = global_index();
-    if (t >= n)
-        return;
-    const uint32_t i = particle_of<IND>(g, t);
-    const f3 xi = ld3(xs, ys, zs, i);
+    const bool live = t < n;
+    const uint32_t i = live ? particle_of<IND>(g, t) : 0u;
+    const f3 xi = live ? ld3(xs, ys, zs, i) : g.bmin;
     uint32_t c = 0;
     float l2 = 0.f;
     f3 v { 0.f, 0.f, 0.f };
-    walk<IND>(g, xi, xs, ys, zs, [&](uint32_t j, f3 xij, float d2) {
+    auto body = [&](uint32_t j, f3 xij, float d2) {
         c += (j == i) ? 0u : 1u;
         l2 += d2;
         v += xij;
-    });
+    };
+    if constexpr (IND) {
+        if (live)
+            walk<IND>(g, xi, xs, ys, zs, body);
+    } else {
+        __shared__ float4 stage[kBlock / kWave][2 * kQueryStage];
+        walk_staged<kQueryStage, 2>(g, xi, xs, ys, zs, stage[threadIdx.x / kWave], live,
+            [&](uint32_t j, f3 xij, float d2, f3) { body(j, xij, d2); });
+    }
+    if (!live)
+        return;
     if (count)
         count[i] = c;
     if (len2)
@@ -314,14 +330,20 @@ __global__ __launch_bounds__(kBlock) void k_query_bench(GridDev g, KParams kp,
     uint32_t n, float* __restrict__ res)
 {
     const uint32_t t = global_index();
-    if (t >= n)
-        return;
-    const uint32_t i = particle_of<IND>(g, t);
-    const f3 xi = ld3(xs, ys, zs, i);
+    const bool live = t < n;
+    const uint32_t i = live ? particle_of<IND>(g, t) : 0u;
+    const f3 xi = live ? ld3(xs, ys, zs, i) : g.bmin;
     float acc = 0.f;
-    walk<IND>(g, xi, xs, ys, zs,
-        [&](uint32_t j, f3, float d2) { acc += xi.x * xs[j] * kW<K>(kp, d2); });
-    res[i] = acc;
+    if constexpr (IND) {
+        if (live)
+            walk<IND>(g, xi, xs, ys, zs, [&](uint32_t j, f3, float d2) { acc += xi.x * xs[j] * kW<K>(kp, d2); });
+    } else {
+        __shared__ float4 stage[kBlock / kWave][2 * kQueryStage];
+        walk_staged<kQueryStage, 2>(g, xi, xs, ys, zs, stage[threadIdx.x / kWave], live,
+            [&](uint32_t, f3, float d2, f3 xj) { acc += xi.x * xj.x * kW<K>(kp, d2); });
+    }
+    if (live)
+        res[i] = acc;
 }
 
 int ensure_capacity(reyn_grid_builder* b, uint32_t n)

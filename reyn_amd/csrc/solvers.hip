@@ -67,6 +67,11 @@ constexpr uint32_t kListCap = 64;
 #define REYN_LIST_RING 16
 #endif
 constexpr uint32_t kRing = REYN_LIST_RING, kFlush = 4;
+// records of the density pass's strip staging, per wavefront
+#ifndef REYN_STAGE
+#define REYN_STAGE 96
+#endif
+constexpr uint32_t kStage = REYN_STAGE;
 
 size_t carve_scratch(char* p, uint32_t n, Scratch* s, bool lists, uint32_t nblocks, uint32_t nscenes,
     uint32_t nbdy)
@@ -206,6 +211,8 @@ __global__ __launch_bounds__(kBlock) REYN_OCC(REYN_OCC_DENSITY) void k_density(S
     // take minima over the whole wavefront; they record nothing
     const uint32_t i = live ? particle_of<IND>(g, t) : 0u;
     const f3 xi = live ? ld3(f.x, f.y, f.z, i) : g.bmin;
+    const uint32_t wv = threadIdx.x / kWave;
+    __shared__ float4 stage[kBlock / kWave][kStage > 0 ? kStage : 1]; // strip staging, per wavefront
     ksum acc;
     bool summed = false;
     if (listing) {
@@ -221,7 +228,7 @@ __global__ __launch_bounds__(kBlock) REYN_OCC(REYN_OCC_DENSITY) void k_density(S
         // are written at the end. A lane whose ring is full stores its later
         // entries directly.
         __shared__ uint32_t ring[kBlock / kWave][kRing][kWave];
-        const uint32_t wv = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+        const uint32_t lane = threadIdx.x % kWave;
         uint32_t cnt = 0, flushed = 0, direct = L.cap; // `flushed` is uniform
         auto flush = [&](uint32_t end) { // rows [flushed, end) from the ring
             const uint32_t have = min(cnt, direct);
@@ -230,31 +237,33 @@ __global__ __launch_bounds__(kBlock) REYN_OCC(REYN_OCC_DENSITY) void k_density(S
                     L.idx[t + r * L.n] = ring[wv][r % kRing][lane];
             flushed = end;
         };
-        walk4<IND>(
-            g, xi, P4,
-            [&](uint32_t j, f3, float, float) {
-                if (!live || j == i)
-                    return;
-                if (cnt < L.cap) {
-                    if (cnt >= direct) {
-                        L.idx[t + cnt * L.n] = j;
-                    } else if (cnt < flushed + kRing) {
-                        ring[wv][cnt % kRing][lane] = j;
-                    } else {
-                        direct = cnt;
-                        L.idx[t + cnt * L.n] = j;
-                    }
+        auto record = [&](uint32_t j, f3, float, float) {
+            if (!live || j == i)
+                return;
+            if (cnt < L.cap) {
+                if (cnt >= direct) {
+                    L.idx[t + cnt * L.n] = j;
+                } else if (cnt < flushed + kRing) {
+                    ring[wv][cnt % kRing][lane] = j;
+                } else {
+                    direct = cnt;
+                    L.idx[t + cnt * L.n] = j;
                 }
-                ++cnt;
-            },
-            [&] {
-                // write rows while every lane has filled the next kFlush (a
-                // lane that stores directly, or one past the end, holds back
-                // no row)
-                const uint32_t full = (!live || cnt >= direct) ? L.cap : min(cnt, L.cap);
-                while (__ballot(full < flushed + kFlush) == 0 && flushed + kFlush <= L.cap)
-                    flush(flushed + kFlush);
-            });
+            }
+            ++cnt;
+        };
+        auto rows_out = [&] {
+            // write rows while every lane has filled the next kFlush (a lane
+            // that stores directly, or one past the end, holds back no row)
+            const uint32_t full = (!live || cnt >= direct) ? L.cap : min(cnt, L.cap);
+            while (__ballot(full < flushed + kFlush) == 0 && flushed + kFlush <= L.cap)
+                flush(flushed + kFlush);
+        };
+        if constexpr (IND || kStage == 0) {
+            walk4<IND>(g, xi, P4, record, rows_out);
+        } else {
+            walk4_staged<kStage, 1>(g, xi, P4, stage[wv], live, record, rows_out);
+        }
         flush(wave_max_u32(live ? min(cnt, direct) : 0u));
         if (!live)
             return;

@@ -329,6 +329,132 @@ __device__ __forceinline__ void walk4(const GridDev& g, f3 xi, const float4* __r
     walk4<IND>(g, xi, P, f, [] {});
 }
 
+// LDS stores of some lanes made visible to the other lanes of the wavefront.
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// Strip staging for walks over a cell-ordered state (no index indirection).
+// The lanes of a wavefront sit in neighbouring cells, so for one strip offset
+// their candidate ranges [s, e) fall into one short index window, or into two
+// when the wavefront spans two rows of cells. With WIN windows of up to CAP
+// records each that hold every lane's range, the wavefront loads the windows
+// once, with coalesced loads, into its own WIN * CAP slots of LDS `st`, and
+// each lane reads candidate q from st[q - base]; otherwise (`staged` false,
+// the same for the whole wavefront) every lane loads from memory as before.
+// One window costs fewer reductions and less LDS; two also cover the
+// wavefronts that cross a row.
+struct StageWin {
+    uint32_t base;
+    bool staged;
+};
+template <uint32_t CAP, int WIN, class Ld>
+__device__ __forceinline__ StageWin stage_strip(uint32_t s, uint32_t e, float4* __restrict__ st, Ld&& load)
+{
+    const uint32_t lane = threadIdx.x % kWave;
+    const bool ne = e > s;
+    const uint32_t lo1 = wave_min_u32(ne ? s : 0xffffffffu);
+    if constexpr (WIN == 1) {
+        const uint32_t hi = wave_max_u32(ne ? e : 0u);
+        if (hi <= lo1 || hi - lo1 > CAP) // uniform
+            return { 0u, false };
+        for (uint32_t q = lo1 + lane; q < hi; q += kWave)
+            st[q - lo1] = load(q);
+        wave_lds_sync();
+        return { lo1, true };
+    }
+    const bool in1 = ne && e - lo1 <= CAP;
+    const uint32_t hi1 = wave_max_u32(in1 ? e : 0u);
+    const uint32_t lo2 = wave_min_u32(ne && !in1 ? s : 0xffffffffu);
+    const bool in2 = ne && !in1 && e - lo2 <= CAP;
+    const uint32_t hi2 = wave_max_u32(in2 ? e : 0u);
+    const bool all = __ballot(ne && !in1 && !in2) == 0;
+    if (!all || hi1 <= lo1) // uniform
+        return { 0u, false };
+    for (uint32_t q = lo1 + lane; q < hi1; q += kWave)
+        st[q - lo1] = load(q);
+    for (uint32_t q = lo2 + lane; q < hi2; q += kWave)
+        st[CAP + (q - lo2)] = load(q);
+    wave_lds_sync();
+    // q - base in unsigned arithmetic: q - lo1, or CAP + q - lo2
+    return { in1 ? lo1 : lo2 - CAP, true };
+}
+
+// Calls f(q, x_ij, |x_ij|^2, p_q) for the candidates q in [s, e) within
+// reach, p_q = get(q) loaded kWalkBatch ahead (past the end the last one again).
+template <class G, class F>
+__device__ __forceinline__ void strip_loop(const GridDev& g, f3 xi, uint32_t s, uint32_t e, G&& get, F&& f)
+{
+    constexpr uint32_t B = kWalkBatch;
+    for (uint32_t q = s; q < e; q += B) {
+        float4 pj[B];
+#pragma unroll
+        for (uint32_t u = 0; u < B; ++u)
+            pj[u] = get(min(q + u, e - 1));
+#pragma unroll
+        for (uint32_t u = 0; u < B; ++u) {
+            if (q + u < e) {
+                const f3 xij = xi - f3 { pj[u].x, pj[u].y, pj[u].z };
+                const float d2 = dot(xij, xij);
+                if (d2 <= g.r2)
+                    f(q + u, xij, d2, pj[u]);
+            }
+        }
+    }
+}
+
+// walk4 over a cell-ordered state with every strip staged (stage_strip; `st`
+// holds WIN * CAP records of this wavefront). Every lane of the wavefront
+// must take part; lanes with `active` false visit nothing but reach every
+// strip_end with the others. The candidates, their order and the calls of f
+// are exactly walk4's.
+template <uint32_t CAP, int WIN, class F, class E>
+__device__ __forceinline__ void walk4_staged(const GridDev& g, f3 xi, const float4* __restrict__ P,
+    float4* __restrict__ st, bool active, F&& f, E&& strip_end)
+{
+    strips(g, xi, [&](uint32_t s, uint32_t e) {
+        if (!active)
+            e = s;
+        const StageWin w = stage_strip<CAP, WIN>(s, e, st, [&](uint32_t q) { return P[q]; });
+        auto body = [&](uint32_t q, f3 xij, float d2, float4 p) { f(q, xij, d2, p.w); };
+        if (w.staged) {
+            strip_loop(g, xi, s, e, [&](uint32_t q) { return st[q - w.base]; }, body);
+            wave_lds_sync(); // every lane has read before the next strip is staged
+        } else {
+            strip_loop(g, xi, s, e, [&](uint32_t q) { return gat(P, q); }, body);
+        }
+        strip_end();
+    });
+}
+
+// The same over SoA positions, staged as {x, y, z, 0}; f(j, x_ij, |x_ij|^2, x_j).
+template <uint32_t CAP, int WIN, class F>
+__device__ __forceinline__ void walk_staged(const GridDev& g, f3 xi, const float* __restrict__ xs,
+    const float* __restrict__ ys, const float* __restrict__ zs, float4* __restrict__ st, bool active, F&& f)
+{
+    strips(g, xi, [&](uint32_t s, uint32_t e) {
+        if (!active)
+            e = s;
+        const StageWin w
+            = stage_strip<CAP, WIN>(s, e, st, [&](uint32_t q) { return make_float4(xs[q], ys[q], zs[q], 0.f); });
+        auto body = [&](uint32_t q, f3 xij, float d2, float4 p) { f(q, xij, d2, f3 { p.x, p.y, p.z }); };
+        if (w.staged) {
+            strip_loop(g, xi, s, e, [&](uint32_t q) { return st[q - w.base]; }, body);
+            wave_lds_sync();
+        } else {
+            strip_loop(g, xi, s, e,
+                [&](uint32_t q) {
+                    const f3 p = gat3(xs, ys, zs, q);
+                    return make_float4(p.x, p.y, p.z, 0.f);
+                },
+                body);
+        }
+    });
+}
+
 // ---- per-step neighbour lists ------------------------------------------------
 // The density pass of a solver step walks the grid once and records, for the
 // particle in cell-order slot t, its fluid neighbours j != self in walk order:
