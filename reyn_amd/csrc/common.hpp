@@ -162,6 +162,28 @@ __device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v)
     return v;
 }
 
+// acc = op(acc, get(k)) for k = threadIdx.x, + blockDim.x, ... below n, in
+// that order, with U loads in flight at a time (a one-workgroup reduction of
+// thousands of partials otherwise waits for each load in turn).
+template <int U, class T, class G, class Op>
+__device__ __forceinline__ T strided_fold(uint32_t n, T acc, G&& get, Op&& op)
+{
+    const uint32_t stride = blockDim.x;
+    uint32_t k = threadIdx.x;
+    for (; k + (U - 1) * stride < n; k += U * stride) {
+        decltype(get(k)) v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            v[u] = get(k + u * stride);
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            acc = op(acc, v[u]);
+    }
+    for (; k < n; k += stride)
+        acc = op(acc, get(k));
+    return acc;
+}
+
 // Block-wide reduction in a fixed order (waves combined by wave 0), so that a
 // block's partial result does not depend on scheduling. op: 0 sum, 1 min, 2 max
 template <int OP>

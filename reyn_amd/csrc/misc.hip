@@ -71,20 +71,15 @@ __global__ __launch_bounds__(kBlock) void k_reduce_final(const double* __restric
 {
     __shared__ double lds[kBlock / kWave];
     const double neutral = OP == 0 ? 0.0 : (OP == 1 ? INFINITY : -INFINITY);
-    double v = neutral;
-    bool nan = false;
-    for (uint32_t k = threadIdx.x; k < nb; k += blockDim.x) {
-        const double p = part[k];
-        nan |= (p != p);
-        if (OP == 0)
-            v += p;
-        else if (OP == 1)
-            v = fmin(v, p);
-        else
-            v = fmax(v, p);
-    }
-    v = block_reduce<OP>(v, lds);
-    const double anynan = block_reduce<2>(nan ? 1.0 : 0.0, lds);
+    // x: the fold, y: 1 once a NaN was seen
+    const double2 r = strided_fold<8>(
+        nb, make_double2(neutral, 0.0), [&](uint32_t k) { return part[k]; },
+        [](double2 a, double p) {
+            const double f = OP == 0 ? a.x + p : (OP == 1 ? fmin(a.x, p) : fmax(a.x, p));
+            return make_double2(f, (p != p) ? 1.0 : a.y);
+        });
+    double v = block_reduce<OP>(r.x, lds);
+    const double anynan = block_reduce<2>(r.y, lds);
     if (threadIdx.x == 0)
         out[0] = anynan > 0.0 ? NAN : v;
 }

@@ -527,9 +527,8 @@ __global__ __launch_bounds__(kBlock) void k_ctl_pcisph(SceneArg sa, SolveCtl* ct
     SolveCtl* ctl = ctls + sc.idx;
     if (ctl->done)
         return;
-    double s = 0.0;
-    for (uint32_t k = threadIdx.x; k < sc.nblk; k += blockDim.x)
-        s += part[sc.blk_first + k];
+    double s = strided_fold<8>(
+        sc.nblk, 0.0, [&](uint32_t k) { return part[sc.blk_first + k]; }, [](double a, double v) { return a + v; });
     s = block_reduce<0>(s, lds);
     if (threadIdx.x == 0) {
         const uint32_t l = ctl->iter + 1;
@@ -740,11 +739,11 @@ __global__ __launch_bounds__(kBlock) void k_ctl_iisph(SceneArg sa, SolveCtl* ctl
         return;
     const uint32_t n = sc.n;
     const float rho0 = sc.c.rho0;
-    double s = 0.0, c = 0.0;
-    for (uint32_t k = threadIdx.x; k < sc.nblk; k += blockDim.x) {
-        s += part_err[sc.blk_first + k];
-        c += double(part_cnt[sc.blk_first + k]);
-    }
+    const double2 sc2 = strided_fold<8>(
+        sc.nblk, make_double2(0.0, 0.0),
+        [&](uint32_t k) { return make_double2(part_err[sc.blk_first + k], double(part_cnt[sc.blk_first + k])); },
+        [](double2 a, double2 v) { return make_double2(a.x + v.x, a.y + v.y); });
+    double s = sc2.x, c = sc2.y;
     s = block_reduce<0>(s, lds);
     c = block_reduce<0>(c, lds);
     if (threadIdx.x == 0) {
@@ -767,8 +766,9 @@ __global__ __launch_bounds__(kBlock) void k_ctl_reset(SceneArg sa, SolveCtl* ctl
     const SceneDev& sc = scene_of_ctl<BT>(sa);
     uint32_t mixed = mflag ? 0u : 1u;
     if (mflag)
-        for (uint32_t k = threadIdx.x; k < sc.nblk; k += blockDim.x)
-            mixed |= mflag[sc.blk_first + k];
+        mixed = strided_fold<8>(
+            sc.nblk, mixed, [&](uint32_t k) { return mflag[sc.blk_first + k]; },
+            [](uint32_t a, uint32_t v) { return a | v; });
     const int any = __syncthreads_or(mixed ? 1 : 0);
     if (threadIdx.x == 0) {
         SolveCtl& c = ctl[sc.idx];
