@@ -230,25 +230,28 @@ __global__ __launch_bounds__(kBlock) REYN_OCC(REYN_OCC_DENSITY) void k_density(S
         __shared__ uint32_t ring[kBlock / kWave][kRing][kWave];
         const uint32_t lane = threadIdx.x % kWave;
         uint32_t cnt = 0, flushed = 0, direct = L.cap; // `flushed` is uniform
+        // entries below `lim` = min(flushed + kRing, direct, cap) go to the
+        // ring: one test on the common path
+        uint32_t lim = min(kRing, L.cap);
         auto flush = [&](uint32_t end) { // rows [flushed, end) from the ring
             const uint32_t have = min(cnt, direct);
             for (uint32_t r = flushed; r < end; ++r)
                 if (r < have)
                     L.idx[t + r * L.n] = ring[wv][r % kRing][lane];
             flushed = end;
+            lim = min(min(flushed + kRing, direct), L.cap);
         };
         auto record = [&](uint32_t j, f3, float, float) {
             if (!live || j == i)
                 return;
-            if (cnt < L.cap) {
-                if (cnt >= direct) {
-                    L.idx[t + cnt * L.n] = j;
-                } else if (cnt < flushed + kRing) {
-                    ring[wv][cnt % kRing][lane] = j;
-                } else {
-                    direct = cnt;
-                    L.idx[t + cnt * L.n] = j;
-                }
+            if (cnt < lim) {
+                ring[wv][cnt % kRing][lane] = j;
+            } else if (cnt < L.cap) {
+                // the ring is full (or this lane already stores directly):
+                // from here on the entries go straight to the list
+                direct = min(direct, cnt);
+                lim = 0;
+                L.idx[t + cnt * L.n] = j;
             }
             ++cnt;
         };

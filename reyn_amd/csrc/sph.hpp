@@ -406,6 +406,38 @@ __device__ __forceinline__ void strip_loop(const GridDev& g, f3 xi, uint32_t s, 
     }
 }
 
+// strip_loop over staged records, p = &st[s - base]: whole batches of
+// kWalkBatch with no bounds tests and LDS addresses as immediate offsets from
+// one pointer, then the tail one by one (the same candidates in the same
+// order).
+template <class F>
+__device__ __forceinline__ void strip_loop_lds(const GridDev& g, f3 xi, uint32_t s, uint32_t e,
+    const float4* __restrict__ p, F&& f)
+{
+    constexpr uint32_t B = kWalkBatch;
+    uint32_t q = s;
+    for (; q + B <= e; q += B, p += B) {
+        float4 pj[B];
+#pragma unroll
+        for (uint32_t u = 0; u < B; ++u)
+            pj[u] = p[u];
+#pragma unroll
+        for (uint32_t u = 0; u < B; ++u) {
+            const f3 xij = xi - f3 { pj[u].x, pj[u].y, pj[u].z };
+            const float d2 = dot(xij, xij);
+            if (d2 <= g.r2)
+                f(q + u, xij, d2, pj[u]);
+        }
+    }
+    for (; q < e; ++q, ++p) {
+        const float4 pj = *p;
+        const f3 xij = xi - f3 { pj.x, pj.y, pj.z };
+        const float d2 = dot(xij, xij);
+        if (d2 <= g.r2)
+            f(q, xij, d2, pj);
+    }
+}
+
 // walk4 over a cell-ordered state with every strip staged (stage_strip; `st`
 // holds WIN * CAP records of this wavefront). Every lane of the wavefront
 // must take part; lanes with `active` false visit nothing but reach every
@@ -421,7 +453,7 @@ __device__ __forceinline__ void walk4_staged(const GridDev& g, f3 xi, const floa
         const StageWin w = stage_strip<CAP, WIN>(s, e, st, [&](uint32_t q) { return P[q]; });
         auto body = [&](uint32_t q, f3 xij, float d2, float4 p) { f(q, xij, d2, p.w); };
         if (w.staged) {
-            strip_loop(g, xi, s, e, [&](uint32_t q) { return st[q - w.base]; }, body);
+            strip_loop_lds(g, xi, s, e, st + (s - w.base), body);
             wave_lds_sync(); // every lane has read before the next strip is staged
         } else {
             strip_loop(g, xi, s, e, [&](uint32_t q) { return gat(P, q); }, body);
@@ -442,7 +474,7 @@ __device__ __forceinline__ void walk_staged(const GridDev& g, f3 xi, const float
             = stage_strip<CAP, WIN>(s, e, st, [&](uint32_t q) { return make_float4(xs[q], ys[q], zs[q], 0.f); });
         auto body = [&](uint32_t q, f3 xij, float d2, float4 p) { f(q, xij, d2, f3 { p.x, p.y, p.z }); };
         if (w.staged) {
-            strip_loop(g, xi, s, e, [&](uint32_t q) { return st[q - w.base]; }, body);
+            strip_loop_lds(g, xi, s, e, st + (s - w.base), body);
             wave_lds_sync();
         } else {
             strip_loop(g, xi, s, e,
