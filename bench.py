@@ -15,6 +15,9 @@ Prints one JSON line with particle-steps/s, the roofline of the dominant
 kernel (algorithmic bytes per launch / its average HIP-event duration, timed
 live on the launch stream over the timed steps) and the CPU baseline (the
 oracle's FP32 OpenMP port of the same step, on the same state, on this host).
+`config.phases` gives every kernel's HBM fraction and its VALU fraction:
+VALU instructions per launch (rocprofv3 SQ_INSTS_VALU, profiles/valu_insts.json)
+over its live launch time, against one wave64 instruction per CU per clock.
 Two more objects in the same line:
 
 - "batch": 64 PCISPH scenes of 10k-300k particles (seeded) advanced in one
@@ -41,6 +44,9 @@ from pathlib import Path
 import numpy as np
 
 HBM_PEAK_GBS = 8000.0
+# VALU issue peak: one wave64 FP32 instruction per CU per clock (256 CUs at
+# 2.4 GHz; packed FP32 is not used), i.e. 39.3 T lane-operations/s
+VALU_PEAK_INSTS = 256 * 2.4e9
 METRIC = "particle-steps/sec + % HBM roofline (IISPH 2M); serving p50/p99 vs plain"
 
 
@@ -87,6 +93,18 @@ def measured_traffic(kernel: str) -> float | None:
     path = Path(__file__).resolve().parent / "profiles" / "hbm_traffic.json"
     try:
         table = json.loads(path.read_text())["bytes_per_launch"]
+    except (OSError, ValueError, KeyError):
+        return None
+    v = table.get(kernel)
+    return float(v) if v is not None else None
+
+
+def measured_valu(kernel: str) -> float | None:
+    """VALU instructions per launch of `kernel` from the committed rocprofv3
+    SQ_INSTS_VALU summary, or None when it does not cover the kernel."""
+    path = Path(__file__).resolve().parent / "profiles" / "valu_insts.json"
+    try:
+        table = json.loads(path.read_text())["valu_insts_per_launch"]
     except (OSError, ValueError, KeyError):
         return None
     v = table.get(kernel)
@@ -478,21 +496,35 @@ def main() -> int:
     value = sc.n * args.steps / wall
     nfields = 8
     roofline = None
+    phases = []  # per kernel: HBM fraction (algorithmic bytes) and VALU fraction (counted instructions)
+    step_valu_frac = None
     if prof:
-        log(f"{'kernel':<18}{'launches':>9}{'total ms':>11}{'avg us':>10}{'GB/s':>9}{'HBM %':>7}")
+        log(f"{'kernel':<18}{'launches':>9}{'total ms':>11}{'avg us':>10}{'GB/s':>9}{'HBM %':>7}{'VALU %':>8}")
         best = None
+        valu_total = 0.0
         for name, (cnt, tot) in sorted(prof.items(), key=lambda kv: -kv[1][1]):
             by = algorithmic_bytes(name, sc.n, sc.n_bdy, ncells, nfields)
             avg_us = tot * 1e3 / cnt
             gbs = by / (avg_us * 1e-6) / 1e9 if by else float("nan")
-            log(f"{name:<18}{cnt:>9}{tot:>11.3f}{avg_us:>10.1f}{gbs:>9.0f}{100 * gbs / HBM_PEAK_GBS:>7.1f}")
+            vi = measured_valu(name)
+            vf = vi / (avg_us * 1e-6) / VALU_PEAK_INSTS if vi else float("nan")
+            log(f"{name:<18}{cnt:>9}{tot:>11.3f}{avg_us:>10.1f}{gbs:>9.0f}{100 * gbs / HBM_PEAK_GBS:>7.1f}"
+                f"{100 * vf:>8.1f}")
+            if vi:
+                valu_total += vi * cnt
+                phases.append({"kernel": name, "launches": cnt, "avg_launch_us": round(avg_us, 2),
+                               "hbm_frac": round(gbs / HBM_PEAK_GBS, 5) if by else None,
+                               "valu_frac": round(vf, 4)})
             if best is None and by:
                 best = (name, cnt, tot, by, avg_us, gbs)
         name, cnt, tot, by, avg_us, gbs = best
+        vi = measured_valu(name)
         roofline = {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(gbs / HBM_PEAK_GBS, 5), "traffic": measured_traffic(name), "kernel": name,
                     "bytes_per_launch": by, "avg_launch_us": round(avg_us, 2), "launches": cnt,
-                    "share_of_step_time": round(tot / gpu_ms, 4)}
+                    "share_of_step_time": round(tot / gpu_ms, 4),
+                    "valu_frac": round(vi / (avg_us * 1e-6) / VALU_PEAK_INSTS, 4) if vi else None}
+        step_valu_frac = valu_total / (gpu_ms * 1e-3) / VALU_PEAK_INSTS
     mean_iters = float(np.mean(iters))
     sbytes = step_bytes(sc.n, sc.n_bdy, ncells, mean_iters)
     step_frac = sbytes / (ms_per_step * 1e-3) / (HBM_PEAK_GBS * 1e9)
@@ -543,6 +575,8 @@ def main() -> int:
             "simulated_time_timed": float(sum(dts)),
             "density_mean_rel_dev": rho_err,
             "step_hbm_frac": round(step_frac, 5),
+            "step_valu_frac": round(step_valu_frac, 4) if step_valu_frac else None,
+            "phases": phases,
             "step_bytes": sbytes,
             "event_ms_timed": round(gpu_ms, 3),
         },
