@@ -250,6 +250,12 @@ public:
     Particles& operator=(const Particles&) = delete;
 
     size_t size() const { return xx.size(); }
+    // every field to n entries, contents kept up to min(old, new)
+    void resize(size_t n)
+    {
+        for (DeviceBuffer<float>* b : { &xx, &xy, &xz, &vx, &vy, &vz, &m })
+            b->resize(n);
+    }
     reyn_particles view() const
     {
         return reyn_particles { xx.ptr(), xy.ptr(), xz.ptr(), vx.ptr(), vy.ptr(), vz.ptr(), m.ptr(),
@@ -747,6 +753,164 @@ inline BoundarySamples sample_mesh(const Mesh& mesh, float h, float rho_0, float
         pts[i] = v3(x[i], y[i], z[i]);
     return BoundarySamples(pts, h, rho_0, 5, ctx);
 }
+
+// ---- Scene (src/scene/scene.cuh:48-156, scene.cu:239-491) --------------------------
+//
+// The reference's two scene kinds: a box of fluid in a boundary sampled from
+// an OBJ file (every shape but "fluid"), and the volume of an OBJ shape named
+// "fluid" filled with fluid inside the other shapes. Fluid points start on a
+// lattice of spacing h (chosen for about N_desired particles), points within
+// cull_bdy_radius h of a boundary sample are removed, and the rest are jittered
+// by a normal deviate of standard deviation jitter_stddev h. The boundary is
+// sampled with relaxation_iters rounds of relaxation (the reference's default
+// 50). The random numbers are a counter hash of `seed` in place of curand and
+// thrust's engine.
+class Scene {
+public:
+    float ρ₀;
+    float h;
+    uint32_t N = 0;
+    BoundarySamples bdy;
+    float3 bound_min, bound_max;
+
+    Scene(const std::filesystem::path& path, uint32_t N_desired, float3 min, float3 max, float rho_0,
+        Particles& state, DeviceBuffer<float>& tmp, float bdy_oversampling_factor = 2.f, float cull_bdy_radius = 1.f,
+        float jitter_stddev = 0.01f, uint32_t relaxation_iters = 50, uint32_t seed = 1614201312u,
+        Context& ctx = Context::current())
+        : ρ₀(rho_0)
+        , h(std::cbrt(((max.x - min.x) * (max.y - min.y)) * (max.z - min.z) / static_cast<float>(N_desired)))
+        , bdy(sample_mesh(Mesh::load_obj(path, { "fluid" }), h, rho_0, bdy_oversampling_factor, seed, ctx,
+              relaxation_iters))
+        , bound_min(bdy.bound_min)
+        , bound_max(bdy.bound_max)
+        , grid_builder_(bound_min, bound_max, 2.f * h, 0.5f, ctx)
+    {
+        const long nx = std::lround(std::floor((max.x - min.x) / h)), ny = std::lround(std::floor((max.y - min.y) / h)),
+                   nz = std::lround(std::floor((max.z - min.z) / h));
+        const size_t n = size_t(std::labs(nx)) * size_t(std::labs(ny)) * size_t(std::labs(nz));
+        if (n == 0)
+            throw std::domain_error("Initialization of box failed, zero particles placed");
+        state.resize(n);
+        ctx.check(reyn_lattice_box(ctx.get(), min, uint32_t(std::labs(nx)), uint32_t(std::labs(ny)),
+                      uint32_t(std::labs(nz)), h, state.xx.ptr(), state.xy.ptr(), state.xz.ptr()),
+            "Scene lattice");
+        finish(state, tmp, n, cull_bdy_radius, jitter_stddev, seed, ctx);
+    }
+
+    static Scene from_obj(const std::filesystem::path& path, uint32_t N_desired, float rho_0, Particles& state,
+        DeviceBuffer<float>& tmp, float bdy_oversampling_factor = 2.f, float cull_bdy_radius = 1.f,
+        float jitter_stddev = 0.01f, uint32_t relaxation_iters = 50, uint32_t seed = 1614201312u,
+        Context& ctx = Context::current())
+    {
+        const Mesh fluid = Mesh::load_obj(path, {}, "fluid");
+        const float h = static_cast<float>(std::cbrt(fluid.get_volume() / static_cast<double>(N_desired)));
+        return Scene(path, fluid, h, rho_0, state, tmp, bdy_oversampling_factor, cull_bdy_radius, jitter_stddev,
+            relaxation_iters, seed, ctx);
+    }
+
+    // Scene::hard_enforce_bounds (scene.cu:466-491)
+    void hard_enforce_bounds(Particles& state) const { reyn::hard_enforce_bounds(bound_min, bound_max, state); }
+
+    // Scene::get_grid (scene.cuh:136-156): the grid of radius search_radius h,
+    // the state and the further buffers reordered into cell order
+    template <IsFltDevBufPtr... MoreBufs>
+    UniformGrid<Resort::yes> get_grid(Particles& state, DeviceBuffer<float>& tmp, float search_radius = 2.f,
+        MoreBufs... reorder)
+    {
+        return grid_builder_.construct_and_reorder(search_radius * h, tmp, state, reorder...);
+    }
+
+private:
+    UniformGridBuilder grid_builder_;
+
+    // the fluid volume of `fluid`, the boundary from the other shapes of path
+    Scene(const std::filesystem::path& path, const Mesh& fluid, float h_, float rho_0, Particles& state,
+        DeviceBuffer<float>& tmp, float bdy_oversampling_factor, float cull_bdy_radius, float jitter_stddev,
+        uint32_t relaxation_iters, uint32_t seed, Context& ctx)
+        : ρ₀(rho_0)
+        , h(h_)
+        , bdy(sample_mesh(Mesh::load_obj(path, { "fluid" }), h_, rho_0, bdy_oversampling_factor, seed, ctx,
+              relaxation_iters))
+        , bound_min(bdy.bound_min)
+        , bound_max(bdy.bound_max)
+        , grid_builder_(grow_min(bdy.bound_min, fluid, h_), grow_max(bdy.bound_max, fluid, h_), 2.f * h_, 0.5f, ctx)
+    {
+        // the scene bounds also hold the fluid's box, with half a spacing to spare
+        bound_min = grow_min(bound_min, fluid, h);
+        bound_max = grow_max(bound_max, fluid, h);
+        const auto [lo, hi] = mesh_box(fluid);
+        const long nx = std::lround(std::floor((hi.x - lo.x) / h)), ny = std::lround(std::floor((hi.y - lo.y) / h)),
+                   nz = std::lround(std::floor((hi.z - lo.z) / h));
+        const size_t n = size_t(std::labs(nx)) * size_t(std::labs(ny)) * size_t(std::labs(nz));
+        if (n == 0)
+            throw std::domain_error("Initialization of box failed, zero particles placed");
+        // lattice candidates in the fluid's box, kept where inside the fluid
+        state.resize(n);
+        ctx.check(reyn_lattice_box(ctx.get(), lo, uint32_t(std::labs(nx)), uint32_t(std::labs(ny)),
+                      uint32_t(std::labs(nz)), h, state.xx.ptr(), state.xy.ptr(), state.xz.ptr()),
+            "Scene lattice");
+        const LBVH bvh(fluid, ctx);
+        DeviceBuffer<uint8_t> keep(n, ctx);
+        bvh.inside(state.xx, state.xy, state.xz, keep);
+        uint32_t kept = 0;
+        float* xyz[3] = { state.xx.ptr(), state.xy.ptr(), state.xz.ptr() };
+        ctx.check(reyn_compact_f32(ctx.get(), keep.ptr(), uint32_t(n), xyz, 3, &kept), "Scene fluid volume");
+        state.resize(kept);
+        finish(state, tmp, kept, cull_bdy_radius, jitter_stddev, seed, ctx);
+    }
+
+    static std::pair<float3, float3> mesh_box(const Mesh& m)
+    {
+        uint32_t nv = 0;
+        reyn_mesh_info(m.get(), &nv, nullptr, nullptr, nullptr);
+        std::vector<double> x(nv), y(nv), z(nv);
+        std::vector<uint32_t> f(3 * size_t(m.face_count()));
+        reyn_mesh_data(m.get(), x.data(), y.data(), z.data(), f.data());
+        double lo[3] = { 1e300, 1e300, 1e300 }, hi[3] = { -1e300, -1e300, -1e300 };
+        for (uint32_t v = 0; v < nv; ++v) {
+            const double p[3] = { x[v], y[v], z[v] };
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = std::min(lo[a], p[a]);
+                hi[a] = std::max(hi[a], p[a]);
+            }
+        }
+        return { v3(float(lo[0]), float(lo[1]), float(lo[2])), v3(float(hi[0]), float(hi[1]), float(hi[2])) };
+    }
+    static float3 grow_min(float3 b, const Mesh& fluid, float h)
+    {
+        const float3 lo = mesh_box(fluid).first;
+        return v3(std::min(b.x, lo.x - 0.5f * h), std::min(b.y, lo.y - 0.5f * h), std::min(b.z, lo.z - 0.5f * h));
+    }
+    static float3 grow_max(float3 b, const Mesh& fluid, float h)
+    {
+        const float3 hi = mesh_box(fluid).second;
+        return v3(std::max(b.x, hi.x + 0.5f * h), std::max(b.y, hi.y + 0.5f * h), std::max(b.z, hi.z + 0.5f * h));
+    }
+
+    // rest mass and zero velocity, the cull near the boundary, the jitter
+    void finish(Particles& state, DeviceBuffer<float>& tmp, size_t n, float cull_bdy_radius, float jitter_stddev,
+        uint32_t seed, Context& ctx)
+    {
+        state.m.fill(ρ₀ * h * h * h);
+        for (DeviceBuffer<float>* b : { &state.vx, &state.vy, &state.vz })
+            b->fill(0.f);
+        DeviceBuffer<uint8_t> keep(n, ctx);
+        const reyn_boundary b = bdy.get().b;
+        ctx.check(reyn_boundary_cull(ctx.get(), &b, cull_bdy_radius * h, state.xx.ptr(), state.xy.ptr(),
+                      state.xz.ptr(), uint32_t(n), keep.ptr()),
+            "Scene cull");
+        float* xyz[3] = { state.xx.ptr(), state.xy.ptr(), state.xz.ptr() };
+        uint32_t kept = 0;
+        ctx.check(reyn_compact_f32(ctx.get(), keep.ptr(), uint32_t(n), xyz, 3, &kept), "Scene cull");
+        state.resize(kept);
+        N = kept;
+        ctx.check(reyn_jitter_normal(ctx.get(), state.xx.ptr(), state.xy.ptr(), state.xz.ptr(), kept, jitter_stddev * h,
+                      seed ^ 0x5bd1e995u),
+            "Scene jitter");
+        tmp.resize(kept);
+        ctx.synchronize();
+    }
+};
 
 // ---- save_to_ply (src/scene/ply_io.cu:7-56) ----------------------------------------
 //

@@ -411,6 +411,29 @@ int reyn_dam_break_sizes(const reyn_dam_break* d, uint32_t* n_lattice, uint32_t*
 int reyn_dam_break_generate(reyn_ctx* ctx, const reyn_dam_break* d, float* x, float* y, float* z,
     uint32_t* n_fluid, float* bx, float* by, float* bz);
 
+/* ---- scene set-up --------------------------------------------------------------
+ * Device helpers from which the C++ `reyn::Scene` builds the reference's scenes
+ * (src/scene/scene.cu:239-464). All work on device arrays. */
+/* Point i of an nx x ny x nz lattice at origin + (ix h, iy h, iz h), x fastest
+ * (`_init_box_kernel`, scene.cu:14-37). */
+int reyn_lattice_box(reyn_ctx* ctx, reyn_float3 origin, uint32_t nx, uint32_t ny, uint32_t nz,
+    float h, float* xs, float* ys, float* zs);
+/* x += sigma N(0, 1) per axis (`jitter_positions`, scene.cu:225-237): normal
+ * deviates from a counter hash of (seed, point, draw) by the Box-Muller
+ * transform, in place of thrust's random engine. */
+int reyn_jitter_normal(reyn_ctx* ctx, float* xs, float* ys, float* zs, uint32_t n, float sigma,
+    uint32_t seed);
+/* keep[k] = 0 when a boundary sample lies within `radius` of point k
+ * (`CullPredicate`, scene.cu:55-83); radius at most the boundary grid's search
+ * radius. */
+int reyn_boundary_cull(reyn_ctx* ctx, const reyn_boundary* b, float radius, const float* xs,
+    const float* ys, const float* zs, uint32_t n, uint8_t* keep);
+/* Stable in-place compaction of nfields >= 1 float arrays of n entries to the
+ * entries with keep[k] != 0 (`cull_particles_by_predicate`, scene.cu:183-206,
+ * thrust::remove_if); `count` receives the number kept. Synchronises. */
+int reyn_compact_f32(reyn_ctx* ctx, const uint8_t* keep, uint32_t n, float* const* fields,
+    uint32_t nfields, uint32_t* count);
+
 /* ---- triangle meshes ---------------------------------------------------------
  * `Mesh`, `load_mesh_from_obj` (src/scene/loader.h, loader.cpp:13-107): vertices
  * in double, triangles as vertex indices. OBJ: `v` and `f` lines (corners "v",

@@ -235,6 +235,10 @@ _SIGS = {
     "reyn_mesh_inside": (i32, [vp, vp, vp, vp, vp, u32, vp]),
     "reyn_mesh_relax": (i32, [vp, vp, f32, f32, vp, vp, vp, u32, u32, f32, P(f32)]),
     "reyn_mesh_project": (i32, [vp, vp, vp, vp, vp, u32]),
+    "reyn_lattice_box": (i32, [vp, Float3, u32, u32, u32, f32, vp, vp, vp]),
+    "reyn_jitter_normal": (i32, [vp, vp, vp, vp, u32, f32, u32]),
+    "reyn_boundary_cull": (i32, [vp, vp, f32, vp, vp, vp, u32, vp]),
+    "reyn_compact_f32": (i32, [vp, vp, u32, vp, u32, P(u32)]),
     "reyn_lbvh_create": (i32, [vp, vp, P(vp)]),
     "reyn_lbvh_destroy": (i32, [vp]),
     "reyn_lbvh_info": (i32, [vp, P(u32), P(u32)]),

@@ -11,6 +11,7 @@
 #include <functional>
 #include <numeric>
 #include <random>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -45,16 +46,16 @@ static void run(const char* name, const std::function<void()>& f)
     std::fflush(stdout);
 }
 
-struct Scene {
+struct Lattice {
     float h;
     std::vector<float> x, y, z;
     std::vector<float3> wall;
     float3 box;
 };
 
-static Scene dam_break(int side, float h, float3 box)
+static Lattice dam_break(int side, float h, float3 box)
 {
-    Scene s { h, {}, {}, {}, {}, box };
+    Lattice s { h, {}, {}, {}, {}, box };
     std::mt19937 rng(7);
     std::uniform_real_distribution<float> jit(-0.01f * h, 0.01f * h);
     for (int k = 0; k < side; ++k)
@@ -76,7 +77,7 @@ static Scene dam_break(int side, float h, float3 box)
     return s;
 }
 
-static void upload(Particles& st, const Scene& s, float rho_0)
+static void upload(Particles& st, const Lattice& s, float rho_0)
 {
     st.xx.upload(s.x);
     st.xy.upload(s.y);
@@ -122,12 +123,12 @@ int main()
 
     run("construct_and_reorder leaves the state in cell order", [] {
         const float rho_0 = 1000.f;
-        Scene s = dam_break(12, 0.05f, v3(1.f, 1.f, 1.f));
+        Lattice s = dam_break(12, 0.05f, v3(1.f, 1.f, 1.f));
         // shuffle the input order
         std::vector<size_t> perm(s.x.size());
         std::iota(perm.begin(), perm.end(), 0);
         std::shuffle(perm.begin(), perm.end(), std::mt19937(3));
-        Scene t = s;
+        Lattice t = s;
         for (size_t i = 0; i < perm.size(); ++i) {
             t.x[i] = s.x[perm[i]];
             t.y[i] = s.y[perm[i]];
@@ -157,7 +158,7 @@ int main()
 
     run("IISPH dam break through the host API", [] {
         const float rho_0 = 1000.f, h = 1.f / 24;
-        Scene s = dam_break(16, h, v3(1.5f, 1.f, 1.f));
+        Lattice s = dam_break(16, h, v3(1.5f, 1.f, 1.f));
         const size_t N = s.x.size();
         Particles state(N, rho_0);
         upload(state, s, rho_0);
@@ -186,7 +187,7 @@ int main()
 
     run("SESPH and PCISPH steps through the host API", [] {
         const float rho_0 = 1000.f, h = 0.05f;
-        Scene s = dam_break(12, h, v3(1.f, 1.f, 1.f));
+        Lattice s = dam_break(12, h, v3(1.f, 1.f, 1.f));
         const size_t N = s.x.size();
         BoundarySamples bdy(s.wall, h, rho_0);
         UniformGridBuilder builder(bdy.bound_min, bdy.bound_max, 2.f * h, 1.0f);
@@ -220,8 +221,8 @@ int main()
     });
 
     run("SceneBatch: scenes alone and in a batch that they join and leave", [] {
-        Scene a = dam_break(10, 0.05f, v3(1.f, 1.f, 1.f));
-        Scene b = dam_break(8, 0.06f, v3(1.2f, 1.f, 0.9f));
+        Lattice a = dam_break(10, 0.05f, v3(1.f, 1.f, 1.f));
+        Lattice b = dam_break(8, 0.06f, v3(1.2f, 1.f, 0.9f));
         std::vector<float> abx, aby, abz, bbx, bby, bbz;
         for (const float3& w : a.wall) {
             abx.push_back(w.x);
@@ -233,7 +234,7 @@ int main()
             bby.push_back(w.y);
             bbz.push_back(w.z);
         }
-        auto desc = [](const Scene& s, const std::vector<float>& bx, const std::vector<float>& by,
+        auto desc = [](const Lattice& s, const std::vector<float>& bx, const std::vector<float>& by,
                         const std::vector<float>& bz) {
             reyn_batch_scene d {};
             d.n = uint32_t(s.x.size());
@@ -366,6 +367,75 @@ int main()
         CHECK(qx[0] == 1.f && qy[0] == 0.5f && qz[0] == -0.5f);
         CHECK(qx[1] == 1.f && qy[1] == 0.1f && qz[1] == 0.2f);
         CHECK(qx[2] == -0.5f && qy[2] == 0.f && qz[2] == 1.f);
+    });
+
+    run("Scene: a fluid box in a mesh boundary, and a fluid volume from an OBJ shape, stepped by the main loop", [] {
+        const char* root = std::getenv("REYN_TEST_GOLDEN");
+        const std::string dir = root ? root : "tests/golden";
+        // box scene: the cube's surface as the boundary, fluid in [-0.9, 0.1]^3
+        Particles state(1, 1000.f);
+        DeviceBuffer<float> tmp(1);
+        Scene box(dir + "/cube.obj", 8000, v3(-0.9f), v3(0.1f), 1000.f, state, tmp, 2.f, 1.f, 0.01f, 5);
+        CHECK(std::fabs(box.h - 0.05f) < 1e-6f);
+        CHECK(box.N == state.size() && (box.N == 8000u || box.N == 6859u));
+        CHECK(box.bound_min.x >= -1.f && box.bound_max.x <= 1.f);
+        const std::vector<float> x = state.xx.download(), m = state.m.download(), vx = state.vx.download();
+        CHECK(*std::min_element(x.begin(), x.end()) > -0.91f && *std::max_element(x.begin(), x.end()) < 0.11f);
+        CHECK(m[0] == 1000.f * box.h * box.h * box.h && vx[0] == 0.f);
+        // the reference's main loop on it (src/main.cu:59-78)
+        DeviceBuffer<float> rho(1), aii(1), si(1), ax(1), ay(1), az(1), prs(1);
+        DeviceBuffer<uint32_t> bits(1);
+        IISPH<B3, Resort::yes> solver(B3(2.f * box.h), box.N, 0.001f, box.h, box.ρ₀, rho, aii, si, ax, ay, az, prs,
+            bits);
+        float dt = 0.0005f;
+        uint32_t iters = 0;
+        for (int s = 0; s < 3; ++s) {
+            dt = simple_dt_controller(iters, dt, 1.f, box.h, state, v3(0.f, -9.81f, 0.f), 10, 0.001f);
+            const auto grid = box.get_grid(state, tmp, 2.f, &prs);
+            iters = solver.step(state, grid, box.bdy, dt);
+            box.hard_enforce_bounds(state);
+        }
+        CHECK(iters > 0 && std::isfinite(rho.avg()) && rho.avg() > 500.0);
+        // fluid volume: a cube named fluid, [-0.5, 0.5]^3, inside the cube [-1, 1]^3
+        const std::string obj = "/tmp/reyn_scene_test.obj";
+        {
+            std::ifstream in(dir + "/cube.obj");
+            std::vector<std::string> v, f;
+            for (std::string line; std::getline(in, line);) {
+                if (line.rfind("v ", 0) == 0)
+                    v.push_back(line);
+                if (line.rfind("f ", 0) == 0)
+                    f.push_back(line);
+            }
+            std::ofstream out(obj);
+            out << "o box\n";
+            for (const auto& l : v)
+                out << l << "\n";
+            for (const auto& l : f)
+                out << l << "\n";
+            out << "o fluid\n";
+            for (const auto& l : v) {
+                std::istringstream is(l.substr(2));
+                double a, b, c;
+                is >> a >> b >> c;
+                out << "v " << 0.5 * a << " " << 0.5 * b << " " << 0.5 * c << "\n";
+            }
+            for (const auto& l : f) {
+                std::istringstream is(l.substr(2));
+                out << "f";
+                for (int k; is >> k;)
+                    out << " " << k + 8;
+                out << "\n";
+            }
+        }
+        Particles st2(1, 1000.f);
+        Scene vol = Scene::from_obj(obj, 8000, 1000.f, st2, tmp, 2.f, 1.f, 0.f, 5);
+        CHECK(std::fabs(vol.h - 0.05f) < 1e-6f);
+        // lattice points on the fluid's faces count as outside: 19 per axis remain
+        CHECK(vol.N == 6859u && st2.size() == 6859u);
+        const std::vector<float> y = st2.xy.download();
+        CHECK(*std::min_element(y.begin(), y.end()) > -0.5f && *std::max_element(y.begin(), y.end()) < 0.5f);
+        std::remove(obj.c_str());
     });
 
     run("save_to_ply writes the reference's binary PLY layout", [] {
