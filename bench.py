@@ -413,8 +413,9 @@ def main() -> int:
         return 0
 
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); this build's path is "
-              "single-GPU, see the sharding plan in DESIGN.md", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): the path is built and measured "
+              "on a single GPU; scenes and batches would shard by scene, one large scene by z-slabs of the cell "
+              "order", file=sys.stderr)
         return 2
 
     from reyn_amd import capi, scenes

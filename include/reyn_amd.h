@@ -509,7 +509,7 @@ int reyn_lbvh_project(reyn_ctx* ctx, const reyn_lbvh* b, float* xs, float* ys, f
     uint32_t n);
 /* The reference's `InsidePredicate` (scene.cu:88-140) through the LBVH:
  * inside[k] = 1 when point k lies behind the closest feature of the surface,
- * judged by the angle-weighted pseudo-normal of the face, edge or vertex that
+ * decided by the angle-weighted pseudo-normal of the face, edge or vertex that
  * holds the closest point (the reference reads the closest face's normal);
  * points on the surface count as outside. For closed, consistently oriented
  * meshes with shared vertices it agrees with `reyn_mesh_inside`.
