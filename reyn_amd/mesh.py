@@ -223,4 +223,93 @@ def obstacle_scene(ctx: Context, mesh: Mesh, fluid_counts, spacing: float, box_h
                        "mesh_samples": int(surf.shape[0]), "culled": int((~keep).sum())})
 
 
-__all__ = ["LBVH", "Mesh", "obstacle_scene"]
+def _cull_and_jitter(ctx: Context, cols, n: int, xb: np.ndarray, h: float, rho_0: float, cull_bdy_radius: float,
+                     jitter_stddev: float, seed: int) -> np.ndarray:
+    """Removes the points within cull_bdy_radius h of a boundary sample (stable
+    order) and jitters the rest; returns them as a host (n, 3) array."""
+    from .sim import BoundarySamples
+
+    bdy = BoundarySamples(ctx, xb, h, rho_0)
+    keep = ctx.empty(max(n, 1), np.uint8)
+    view = bdy.view()
+    ctx.check(ctx._lib.reyn_boundary_cull(ctx.h, C.byref(view), float(cull_bdy_radius * h), *[c.ptr for c in cols],
+                                          n, keep.ptr), "scene cull")
+    arr = (C.c_void_p * 3)(*[c.ptr for c in cols])
+    kept = C.c_uint32()
+    ctx.check(ctx._lib.reyn_compact_f32(ctx.h, keep.ptr, n, arr, 3, C.byref(kept)), "scene cull")
+    ctx.check(ctx._lib.reyn_jitter_normal(ctx.h, *[c.ptr for c in cols], kept.value, float(jitter_stddev * h),
+                                          (int(seed) ^ 0x5BD1E995) & 0xFFFFFFFF), "scene jitter")
+    return np.stack([c.numpy()[:kept.value] for c in cols], axis=1)
+
+
+def _lattice(ctx: Context, lo, counts, h: float):
+    n = int(counts[0]) * int(counts[1]) * int(counts[2])
+    cols = [ctx.empty(max(n, 1)) for _ in range(3)]
+    ctx.check(ctx._lib.reyn_lattice_box(ctx.h, capi.f3(np.asarray(lo, np.float32)), int(counts[0]), int(counts[1]),
+                                        int(counts[2]), float(h), *[c.ptr for c in cols]), "scene lattice")
+    return cols, n
+
+
+def _boundary(ctx: Context, path, h: float, oversampling: float, relaxation_iters: int, seed: int) -> np.ndarray:
+    m = Mesh.load_obj(path, ignore=("fluid",))
+    try:
+        return m.sample(ctx, h, oversampling, seed, relaxation_iters=relaxation_iters)
+    finally:
+        m.close()
+
+
+def box_scene(ctx: Context, path, n_desired: int, lo, hi, rho_0: float = 1000.0, oversampling: float = 2.0,
+              cull_bdy_radius: float = 1.0, jitter_stddev: float = 0.01, relaxation_iters: int = 50,
+              seed: int = 1614201312) -> Scene:
+    """The reference's `Scene(path, N_desired, min, max, ...)` (scene.cu:239-310):
+    a lattice box of fluid of spacing h = cbrt(volume / n_desired) in the
+    boundary sampled (and relaxed) from every shape of the OBJ file but
+    "fluid"; the same set-up as the C++ `reyn::Scene`."""
+    lo, hi = np.asarray(lo, np.float32), np.asarray(hi, np.float32)
+    d = hi - lo
+    h = float(np.cbrt(np.float32((d[0] * d[1]) * d[2]) / np.float32(n_desired)).astype(np.float32))
+    xb = _boundary(ctx, path, h, oversampling, relaxation_iters, seed)
+    counts = [abs(int(np.floor(np.float32(v) / np.float32(h)))) for v in d]
+    if counts[0] * counts[1] * counts[2] == 0:
+        raise ValueError("Initialization of box failed, zero particles placed")
+    cols, n = _lattice(ctx, lo, counts, h)
+    x = _cull_and_jitter(ctx, cols, n, xb, h, rho_0, cull_bdy_radius, jitter_stddev, seed)
+    return Scene(x.astype(np.float32), xb, h, float(rho_0), xb.min(axis=0), xb.max(axis=0),
+                 meta={"kind": "box", "lattice": counts})
+
+
+def scene_from_obj(ctx: Context, path, n_desired: int, rho_0: float = 1000.0, oversampling: float = 2.0,
+                   cull_bdy_radius: float = 1.0, jitter_stddev: float = 0.01, relaxation_iters: int = 50,
+                   seed: int = 1614201312) -> Scene:
+    """The reference's `Scene::from_obj` (scene.cu:312-464): the volume of the
+    OBJ shape named "fluid", filled on a lattice of spacing
+    h = cbrt(volume / n_desired) and kept where the LBVH inside test says so,
+    in the boundary sampled from the other shapes; bounds grown to hold the
+    fluid's box with half a spacing to spare."""
+    fluid = Mesh.load_obj(path, only="fluid")
+    try:
+        h = float(np.float32(np.cbrt(fluid.info()["volume"] / float(n_desired))))
+        v, _ = fluid.arrays()
+        flo, fhi = v.min(axis=0).astype(np.float32), v.max(axis=0).astype(np.float32)
+        xb = _boundary(ctx, path, h, oversampling, relaxation_iters, seed)
+        counts = [abs(int(np.floor((np.float32(b) - np.float32(a)) / np.float32(h)))) for a, b in zip(flo, fhi)]
+        if counts[0] * counts[1] * counts[2] == 0:
+            raise ValueError("Initialization of box failed, zero particles placed")
+        cols, n = _lattice(ctx, flo, counts, h)
+        bvh = LBVH(ctx, fluid)
+        inside = ctx.empty(max(n, 1), np.uint8)
+        ctx.check(ctx._lib.reyn_lbvh_inside(ctx.h, bvh.h, *[c.ptr for c in cols], n, inside.ptr), "fluid volume")
+        arr = (C.c_void_p * 3)(*[c.ptr for c in cols])
+        kept = C.c_uint32()
+        ctx.check(ctx._lib.reyn_compact_f32(ctx.h, inside.ptr, n, arr, 3, C.byref(kept)), "fluid volume")
+        bvh.close()
+        x = _cull_and_jitter(ctx, cols, kept.value, xb, h, rho_0, cull_bdy_radius, jitter_stddev, seed)
+    finally:
+        fluid.close()
+    half = np.float32(0.5 * h)
+    lo = np.minimum(xb.min(axis=0), flo - half)
+    hi = np.maximum(xb.max(axis=0), fhi + half)
+    return Scene(x.astype(np.float32), xb, h, float(rho_0), lo, hi, meta={"kind": "from_obj", "lattice": counts})
+
+
+__all__ = ["LBVH", "Mesh", "box_scene", "obstacle_scene", "scene_from_obj"]

@@ -222,3 +222,41 @@ def test_lbvh_inside_matches_winding_number(ctx, name):
     if name == "torus":  # the inscribed polyhedron lies within the smooth torus
         r = np.hypot(np.hypot(p[:, 0], p[:, 1]) - 1.0, p[:, 2])
         assert np.all(a[r < 0.34]) and not np.any(a[r > 0.36])
+
+
+# ---- the reference's two scene kinds (Scene, Scene::from_obj) ---------------------
+
+
+def _cube_in_cube(path):
+    """An OBJ with a shape "box" (the [-1, 1]^3 cube) and a shape "fluid" (the
+    cube scaled to [-0.5, 0.5]^3), faces as in the reference's cube.obj."""
+    lines = (GOLD / "cube.obj").read_text().splitlines()
+    v = [ln for ln in lines if ln.startswith("v ")]
+    f = [ln for ln in lines if ln.startswith("f ")]
+    out = ["o box", *v, *f, "o fluid"]
+    out += ["v " + " ".join(str(0.5 * float(t)) for t in ln.split()[1:]) for ln in v]
+    out += ["f " + " ".join(str(int(t) + 8) for t in ln.split()[1:]) for ln in f]
+    path.write_text("\n".join(out) + "\n")
+    return path
+
+
+def test_scene_from_obj_and_box_scene_step(ctx, tmp_path):
+    from reyn_amd.mesh import box_scene, scene_from_obj
+
+    obj = _cube_in_cube(tmp_path / "cube_in_cube.obj")
+    sc = scene_from_obj(ctx, obj, 8000, jitter_stddev=0.0, relaxation_iters=5)
+    print(f"from_obj: {sc.n} fluid, {sc.n_bdy} boundary, h {sc.spacing}", flush=True)
+    assert abs(sc.spacing - 0.05) < 1e-6
+    assert sc.n == 19 ** 3  # lattice points on the fluid's faces count as outside
+    assert sc.x.min() > -0.5 and sc.x.max() < 0.5
+    assert np.abs(sc.xb).max() <= 1.0 + 1e-6 and np.allclose(np.abs(sc.xb).max(axis=1), 1.0, atol=1e-5)
+    bx = box_scene(ctx, GOLD / "cube.obj", 8000, (-0.9, -0.9, -0.9), (0.1, 0.1, 0.1), relaxation_iters=5)
+    print(f"box: {bx.n} fluid, {bx.n_bdy} boundary", flush=True)
+    assert bx.n in (19 ** 3, 20 ** 3)
+    # both step through a batch
+    b = Batch(ctx, [sc, bx], solver="iisph", kernel="B3")
+    b.step(3)
+    for sid in b.ids():
+        st = b.download(sid)
+        assert all(np.isfinite(st[k]).all() for k in ("xx", "xy", "xz", "vx", "vy", "vz", "rho"))
+    b.close()
