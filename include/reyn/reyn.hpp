@@ -23,6 +23,8 @@
 
 #include <algorithm>
 #include <bit>
+#include <cmath>
+#include <cstdlib>
 #include <concepts>
 #include <cstring>
 #include <cstdint>

@@ -7,4 +7,5 @@ root="$(cd "$here/../.." && pwd)"
 mkdir -p "$root/reyn_amd/bin"
 flags=(-std=c++20 -O2 -Wall -Wextra -I"$root/include" -L"$root/reyn_amd" -lreyn_amd "-Wl,-rpath,\$ORIGIN/..")
 g++ "$here/dambreak.cpp" -o "$root/reyn_amd/bin/dambreak" "${flags[@]}"
+g++ "$here/scene_main.cpp" -o "$root/reyn_amd/bin/scene_main" "${flags[@]}"
 g++ "$root/tests/cpp/test_host_api.cpp" -o "$root/reyn_amd/bin/test_host_api" "${flags[@]}"
