@@ -509,6 +509,7 @@ __global__ __launch_bounds__(kBlock) void k_pcisph_prs_acc(SceneArg sa, FluidDev
                 const f3 xij = xi - f3 { q[u].x, q[u].y, q[u].z };
                 term(xij, dot(xij, xij), (kk + u < c) ? m[u] : 0.f, q[u].w);
             }
+            asm volatile("" ::: "memory"); // rounds stay in source order, as in nbrs4
         }
     } else {
         walk<IND>(g, xi, f.x, f.y, f.z, [&](uint32_t j, f3 xij, float d2) { term(xij, d2, gat(f.m, j), gat(rho, j)); });
@@ -651,6 +652,7 @@ __global__ __launch_bounds__(kBlock) REYN_OCC(REYN_OCC_PACC) void k_iisph_pacc(S
                 const float mj = (k + u < c) ? m[u] : 0.f;
                 acc += (mj * (pi + q[u].w)) * kdW<K>(kp, xij, dot(xij, xij));
             }
+            asm volatile("" ::: "memory"); // rounds stay in source order, as in nbrs4
         }
     } else {
         walk<IND>(g, xi, f.x, f.y, f.z, [&](uint32_t j, f3 xij, float d2) {
