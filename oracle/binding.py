@@ -9,13 +9,16 @@ baseline; the product path never imports it.
 from __future__ import annotations
 
 import ctypes as C
+import os
 import subprocess
 from pathlib import Path
 
 import numpy as np
 
 HERE = Path(__file__).resolve().parent
-LIB_PATH = HERE / "liboracle_reyn.so"
+# REYN_ORACLE_LIB names another build of the same source (oracle/mutations.py
+# loads deliberately broken builds this way to check that the pins fail)
+LIB_PATH = Path(os.environ.get("REYN_ORACLE_LIB", HERE / "liboracle_reyn.so"))
 
 u32, i32, f32, f64, vp = C.c_uint32, C.c_int32, C.c_float, C.c_double, C.c_void_p
 
@@ -335,3 +338,59 @@ def gather(sorted_, src):
     s = _c(sorted_, np.uint32)
     lib().orc_gather_f32(_p(s), u32(s.size), _p(src), _p(dst))
     return dst
+
+
+def grid_build_par(geom: OrcGrid, x, y, z):
+    """The CPU baseline's parallel counting sort: (prefix, sorted)."""
+    x, y, z = (_c(a, np.float32) for a in (x, y, z))
+    prefix = np.empty(geom.ncells + 1, np.uint32)
+    sorted_ = np.empty(x.size, np.uint32)
+    lib().orc_grid_build_par(C.byref(geom), _p(x), _p(y), _p(z), u32(x.size), _p(prefix), _p(sorted_))
+    return prefix, sorted_
+
+
+class PortState:
+    """Float32 SoA state of the CPU baseline loops, updated in place."""
+
+    FIELDS = ("x", "y", "z", "vx", "vy", "vz", "m")
+
+    def __init__(self, x, v, m, p=None):
+        x = np.asarray(x, np.float32)
+        v = np.zeros_like(x) if v is None else np.asarray(v, np.float32)
+        self.x, self.y, self.z = (np.ascontiguousarray(x[:, a]) for a in range(3))
+        self.vx, self.vy, self.vz = (np.ascontiguousarray(v[:, a]) for a in range(3))
+        self.m = np.ascontiguousarray(np.broadcast_to(np.asarray(m, np.float32), (x.shape[0],)))
+        self.p = np.zeros(x.shape[0], np.float32) if p is None else _c(p, np.float32).copy()
+        self.n = x.shape[0]
+
+    def ptrs(self):
+        return [_p(getattr(self, f)) for f in self.FIELDS]
+
+
+def port_iisph_run(geom: OrcGrid, st: PortState, bdy: Bdy | None, prm: OrcSph, ip: OrcIisph, lo, hi, lam: float,
+                   iter_target: int, change_factor: float, dt: float, iters: int, steps: int):
+    """`steps` steps of the float IISPH main loop (controller, parallel grid and
+    reorder, step, bounds); returns (per-step seconds, per-step iterations,
+    dt, iterations) and updates `st`."""
+    lo, hi = _c(lo, np.float32), _c(hi, np.float32)
+    secs = np.zeros(steps)
+    its = np.zeros(steps, np.uint32)
+    dt_c, it_c = f32(dt), u32(iters)
+    b = bdy.args() if bdy else NO_BDY
+    lib().orc_port_iisph_run(C.byref(geom), *st.ptrs(), _p(st.p), u32(st.n), *b, C.byref(prm), C.byref(ip), _p(lo),
+                             _p(hi), f32(lam), u32(iter_target), f32(change_factor), C.byref(dt_c), C.byref(it_c),
+                             u32(steps), _p(secs), _p(its))
+    return secs, its, float(dt_c.value), int(it_c.value)
+
+
+def port_sesph_run(geom: OrcGrid, st: PortState, bdy: Bdy | None, prm: OrcSph, k: float, lo, hi, lam: float,
+                   steps: int):
+    """`steps` steps of the float SESPH main loop with dt = cfl_time_step(lam);
+    returns (per-step seconds, simulated time) and updates `st`."""
+    lo, hi = _c(lo, np.float32), _c(hi, np.float32)
+    secs = np.zeros(steps)
+    t = C.c_double()
+    b = bdy.args() if bdy else NO_BDY
+    lib().orc_port_sesph_run(C.byref(geom), *st.ptrs(), u32(st.n), *b, C.byref(prm), f32(k), _p(lo), _p(hi),
+                             f32(lam), u32(steps), _p(secs), C.byref(t))
+    return secs, float(t.value)

@@ -20,6 +20,7 @@
 // (closed forms, brute force, invariants) is listed in tests/test_oracle.py;
 // no function here is left without such a check.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstdlib>
@@ -31,6 +32,14 @@
 #ifdef _OPENMP
 #include <omp.h>
 #endif
+
+#include <thrust/execution_policy.h>
+#include <thrust/functional.h>
+#include <thrust/gather.h>
+#include <thrust/iterator/counting_iterator.h>
+#include <thrust/scan.h>
+#include <thrust/system/omp/execution_policy.h>
+#include <thrust/transform_reduce.h>
 
 #define ORC_API extern "C" __attribute__((visibility("default")))
 
@@ -971,4 +980,168 @@ ORC_API double orc_pcisph_delta_f64(int kind, float h_bar, float h)
             }
     const double V0 = double(h) * h * h;
     return 1.0 / (2.0 * V0 * V0 * (sx * sx + sy * sy + sz * sz + sum_sq));
+}
+
+// ============================ the CPU baseline ==============================
+// The float port of the reference main loop (src/main.cu:60-78) as the CPU
+// baseline runs it: the grid is a parallel counting sort (atomic per-cell
+// counts as uniformgrid.cu:20-34, rocThrust's OpenMP exclusive scan as
+// uniformgrid.cu:128-133, a scatter through atomic slots as :40-63, then each
+// cell's indices put in ascending order, so the result is the serial build's
+// bit for bit); the reorder is rocThrust's OpenMP gather (buffer.cuh:184-209);
+// every neighbour loop is an OpenMP loop over particles. Whole steps are timed
+// with std::chrono::steady_clock.
+
+namespace {
+
+struct PortGrid {
+    std::vector<uint32_t> counts, prefix, sorted, keys, fill;
+    std::vector<float> tmp;
+};
+
+void grid_build_par(const orc_grid* geom, const float* x, const float* y, const float* z, uint32_t n, PortGrid& w)
+{
+    const long nx = geom->n[0], ny = geom->n[1], nz = geom->n[2];
+    const size_t C = size_t(nx) * ny * nz;
+    w.counts.assign(C + 1, 0u);
+    w.prefix.resize(C + 1);
+    w.sorted.resize(n);
+    w.keys.resize(n);
+    uint32_t* counts = w.counts.data();
+    uint32_t* keys = w.keys.data();
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < long(n); ++i) {
+        const int cx = cell_axis(x[i], geom->bmin[0], geom->cell, int(nx));
+        const int cy = cell_axis(y[i], geom->bmin[1], geom->cell, int(ny));
+        const int cz = cell_axis(z[i], geom->bmin[2], geom->cell, int(nz));
+        const uint32_t k = static_cast<uint32_t>(cz * nx * ny + cy * nx + cx);
+        keys[i] = k;
+#pragma omp atomic
+        counts[k] += 1u;
+    }
+    thrust::exclusive_scan(thrust::omp::par, counts, counts + C + 1, w.prefix.data());
+    w.fill.assign(w.prefix.begin(), w.prefix.end() - 1);
+    uint32_t* fill = w.fill.data();
+    uint32_t* sorted = w.sorted.data();
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < long(n); ++i) {
+        uint32_t slot;
+#pragma omp atomic capture
+        slot = fill[keys[i]]++;
+        sorted[slot] = uint32_t(i);
+    }
+    const uint32_t* prefix = w.prefix.data();
+#pragma omp parallel for schedule(dynamic, 4096)
+    for (long c = 0; c < long(C); ++c)
+        if (prefix[c + 1] - prefix[c] > 1)
+            std::sort(sorted + prefix[c], sorted + prefix[c + 1]);
+}
+
+// fields[k] <- fields[k][sorted] through w.tmp
+void reorder_par(PortGrid& w, float* const* fields, uint32_t nfields, uint32_t n)
+{
+    w.tmp.resize(n);
+    for (uint32_t k = 0; k < nfields; ++k) {
+        thrust::gather(thrust::omp::par, w.sorted.begin(), w.sorted.end(), fields[k], w.tmp.begin());
+        std::copy(w.tmp.begin(), w.tmp.end(), fields[k]);
+    }
+}
+
+// cfl_time_step / simple_dt_controller (src/timestep/cfl.cu:5-52), max |v| in FP64
+float port_cfl(float lambda, float h, const float* vx, const float* vy, const float* vz, uint32_t n, const float* g)
+{
+    const double vmax = thrust::transform_reduce(
+        thrust::omp::par, thrust::counting_iterator<uint32_t>(0), thrust::counting_iterator<uint32_t>(n),
+        [=](uint32_t i) {
+            const double a = vx[i], b = vy[i], c = vz[i];
+            const double v = std::sqrt(a * a + b * b + c * c);
+            return v != v ? double(INFINITY) : v;
+        },
+        0.0, thrust::maximum<double>());
+    const float dt_vel = static_cast<float>(double(lambda * h) / vmax);
+    const double gn = std::sqrt(double(g[0]) * g[0] + double(g[1]) * g[1] + double(g[2]) * g[2]);
+    const float dt_g = std::sqrt(static_cast<float>(2.0 * double(lambda) * double(h) / gn));
+    return std::fmin(dt_vel, dt_g);
+}
+
+double seconds_since(std::chrono::steady_clock::time_point t0)
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+} // namespace
+
+ORC_API void orc_grid_build_par(const orc_grid* geom, const float* x, const float* y, const float* z, uint32_t n,
+    uint32_t* prefix, uint32_t* sorted)
+{
+    PortGrid w;
+    grid_build_par(geom, x, y, z, n, w);
+    std::copy(w.prefix.begin(), w.prefix.end(), prefix);
+    std::copy(w.sorted.begin(), w.sorted.end(), sorted);
+}
+
+// `steps` steps of the IISPH main loop (main.cu:60-78) in float: the time-step
+// controller (lambda, iter_target, change_factor) from *dt and *iters, the
+// grid with the reorder of x, v, m and the warm-start pressure, the IISPH step,
+// the bounds. The state arrays are updated in place; step_seconds[k] and
+// step_iters[k] receive each step's wall time and iteration count.
+ORC_API void orc_port_iisph_run(const orc_grid* geom, float* x, float* y, float* z, float* vx, float* vy, float* vz,
+    float* m, float* p, uint32_t n, const orc_grid* bg, const float* bx, const float* by, const float* bz,
+    const float* bm, uint32_t nb, const orc_sph* prm, const orc_iisph* ip, const float* lo, const float* hi,
+    float lambda, uint32_t iter_target, float change_factor, float* dt, uint32_t* iters, uint32_t steps,
+    double* step_seconds, uint32_t* step_iters)
+{
+    PortGrid w;
+    std::vector<float> rho(n), aii(n), si(n), ax(n), ay(n), az(n);
+    const SphPrm q = to_prm(prm);
+    const IisphPrm iq { ip->omega, ip->eta, ip->min_iter, ip->max_iter, ip->stop_rule, ip->warm_start };
+    Bdy<float> b { bx, by, bz, bm, nb, bg ? to_og(bg) : OGrid {} };
+    float* fields[8] = { x, y, z, vx, vy, vz, m, p };
+    for (uint32_t k = 0; k < steps; ++k) {
+        const auto t0 = std::chrono::steady_clock::now();
+        const float factor = *iters < iter_target ? 1.f + change_factor : 1.f - change_factor;
+        *dt = std::fmin(port_cfl(lambda, q.h, vx, vy, vz, n, q.g), *dt * factor);
+        grid_build_par(geom, x, y, z, n, w);
+        reorder_par(w, fields, 8, n);
+        orc_grid g = *geom;
+        g.prefix = w.prefix.data();
+        g.sorted = nullptr;
+        Fluid<float> s { x, y, z, vx, vy, vz, m, n };
+        *iters = iisph_step(to_og(&g), s, nb ? &b : nullptr, q, iq, 0, *dt, rho.data(), p, aii.data(), si.data(),
+            ax.data(), ay.data(), az.data(), nullptr, nullptr);
+        orc_hard_enforce_bounds(lo, hi, x, y, z, vx, vy, vz, n);
+        step_seconds[k] = seconds_since(t0);
+        step_iters[k] = *iters;
+    }
+}
+
+// `steps` steps of the SESPH main loop in float with dt = cfl_time_step(lambda)
+// every step: grid with the reorder of x, v, m, the SESPH step (stiffness k),
+// the bounds. *time receives the simulated time advanced.
+ORC_API void orc_port_sesph_run(const orc_grid* geom, float* x, float* y, float* z, float* vx, float* vy, float* vz,
+    float* m, uint32_t n, const orc_grid* bg, const float* bx, const float* by, const float* bz, const float* bm,
+    uint32_t nb, const orc_sph* prm, float k, const float* lo, const float* hi, float lambda, uint32_t steps,
+    double* step_seconds, double* time)
+{
+    PortGrid w;
+    std::vector<float> rho(n), ax(n), ay(n), az(n);
+    const SphPrm q = to_prm(prm);
+    Bdy<float> b { bx, by, bz, bm, nb, bg ? to_og(bg) : OGrid {} };
+    float* fields[7] = { x, y, z, vx, vy, vz, m };
+    double t = 0.0;
+    for (uint32_t s = 0; s < steps; ++s) {
+        const auto t0 = std::chrono::steady_clock::now();
+        const float dt = port_cfl(lambda, q.h, vx, vy, vz, n, q.g);
+        grid_build_par(geom, x, y, z, n, w);
+        reorder_par(w, fields, 7, n);
+        orc_grid g = *geom;
+        g.prefix = w.prefix.data();
+        g.sorted = nullptr;
+        Fluid<float> st { x, y, z, vx, vy, vz, m, n };
+        sesph_step(to_og(&g), st, nb ? &b : nullptr, q, k, dt, rho.data(), ax.data(), ay.data(), az.data());
+        orc_hard_enforce_bounds(lo, hi, x, y, z, vx, vy, vz, n);
+        step_seconds[s] = seconds_since(t0);
+        t += dt;
+    }
+    *time = t;
 }
