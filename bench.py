@@ -75,7 +75,8 @@ def algorithmic_bytes(name: str, n: int, nb: int, ncells: int, nfields: int) -> 
         "k_scatter": 12 * n + 4 * n,
         "k_stabilize": 16 * n + 4 * n,
         "k_gather": 4 * n + 8 * nfields * n,
-        "k_bounds": 24 * n + 24 * n,
+        # x -> x where clamped (the velocity of the few particles at a face)
+        "k_bounds": 12 * n,
     }
     return float(table[name]) if name in table else None
 
@@ -119,53 +120,109 @@ def parse_profile(text: str) -> dict:
     return out
 
 
-def cpu_baseline(sc, host_state, p, dt_steps, args, log) -> dict | None:
-    """The oracle's FP32 port of the same step on this host's cores, from the
-    GPU's state after the timed steps (bounded sample of `cpu_steps` steps)."""
+def cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_leg(ctx, sim, sc, args, log) -> tuple[dict, dict, dict | None]:
+    """The oracle's part of the bench, run after the timed steps:
+
+    - the FP64 check: one more GPU step from the GPU's state against the same
+      step in the FP64 oracle (per-field errors);
+    - the CPU baseline: the oracle's float port of the same main loop (time-step
+      controller, parallel counting sort with rocThrust's OpenMP scan, OpenMP
+      gathers, the IISPH step, bounds) from the GPU's state after that step, on
+      all host threads: 5 warm-up steps, then the median of 3 runs of
+      `cpu_steps` steps (std::chrono around whole steps);
+    - configuration 1 on the host: the SESPH dam break (32^3, CFL 0.4, C2, cell
+      = support radius) for its 1000 steps in the float port, particle-steps/s
+      over steps 5-999, beside the same 1000 steps on the GPU."""
     from oracle import binding as orc
-    from reyn_amd import capi
+    from oracle import parity
+    from reyn_amd import capi, scenes
+    from reyn_amd.sim import SESPH, BoundarySamples, Particles, Simulation
 
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
     orc.set_threads(threads)
-    h = sc.spacing
-    r = 2.0 * h
-    fields = {k: host_state[k].astype(np.float32) for k in ("xx", "xy", "xz", "vx", "vy", "vz", "m")}
-    pr = p.astype(np.float32)
-    xb = sc.xb_sorted
-    mb = sc.mb
-    bgeom = orc.geometry(sc.bdy_min, sc.bdy_max, r, 0.5)
-    bgrid = orc.grid_build(bgeom, xb[:, 0], xb[:, 1], xb[:, 2]).resorted()
-    bdy = orc.Bdy(bgrid, xb[:, 0], xb[:, 1], xb[:, 2], mb, dtype=np.float32)
-    geom = orc.geometry(sc.bound_min, sc.bound_max, r, 0.5)
+    model = cpu_model()
+    h, r = sc.spacing, 2.0 * sc.spacing
+    ob = parity.oracle_boundary(sim.bdy, r)
+    t = time.perf_counter()
+    errs, _, _ = parity.iisph_sim_step_parity(sim, ob)
+    log(f"fp64 check of step {sim.steps}: " + ", ".join(f"{k} {v:.2e}" for k, v in errs.items())
+        + f" ({time.perf_counter() - t:.1f} s)")
+    fp64 = {"step": sim.steps, **{k: (float(v) if k != "iterations" else int(v)) for k, v in errs.items()}}
+
+    hs = sim.state.host()
+    port = orc.PortState(np.stack([hs["xx"], hs["xy"], hs["xz"]], 1), np.stack([hs["vx"], hs["vy"], hs["vz"]], 1),
+                         hs["m"], sim.solver.p.numpy())
+    bdy = orc.Bdy(ob.grid, ob.x, ob.y, ob.z, ob.m, dtype=np.float32)
+    geom = orc.geometry(sc.bound_min, sc.bound_max, r, sim.cell_ratio)
     prm = orc.sph(capi.KERNEL_B3, r, h, 0.001, sc.rho_0)
-    ip = orc.OrcIisph(0.5, 0.001, 3, 100, 1, 1)
-    t0 = time.perf_counter()
-    iters = []
-    for _ in range(args.cpu_steps):
-        dt = dt_steps
-        g = orc.grid_build(geom, fields["xx"], fields["xy"], fields["xz"])
-        for k in fields:
-            fields[k] = orc.gather(g.sorted, fields[k])
-        pr = orc.gather(g.sorted, pr)
-        st = orc.State({"x": fields["xx"], "y": fields["xy"], "z": fields["xz"], "vx": fields["vx"],
-                        "vy": fields["vy"], "vz": fields["vz"], "m": fields["m"]}, dtype=np.float32)
-        res = orc.iisph_step(g.resorted(), st, bdy, prm, ip, pr, dt)
-        pr = res["p"]
-        iters.append(res["iterations"])
-        x, v = orc.hard_enforce_bounds(sc.bound_min, sc.bound_max, np.stack([st.x, st.y, st.z], 1),
-                                       np.stack([st.vx, st.vy, st.vz], 1))
-        for a, k in enumerate(("xx", "xy", "xz")):
-            fields[k] = np.ascontiguousarray(x[:, a])
-        for a, k in enumerate(("vx", "vy", "vz")):
-            fields[k] = np.ascontiguousarray(v[:, a])
-        fields["m"] = st.m
-    wall = time.perf_counter() - t0
-    value = sc.n * args.cpu_steps / wall
-    log(f"cpu baseline: {args.cpu_steps} step(s) in {wall:.2f} s on {threads} threads, iterations {iters}")
-    return {"value": value, "unit": "particle-steps/s", "cores": threads, "kind": "port",
-            "sample": f"{args.cpu_steps} full IISPH step(s) of the same {sc.n}-particle scene from the GPU's state "
-                      f"after the timed steps (Jacobi iterations {iters}), oracle FP32 OpenMP port, "
-                      f"{wall:.2f} s wall"}
+    ip = orc.OrcIisph(0.5, 0.001, 3, 100, capi.IISPH_STOP_MEAN, 1)
+    dt, it = sim.dt, sim.last_iters
+
+    def run(steps):
+        nonlocal dt, it
+        secs, its, dt, it = orc.port_iisph_run(geom, port, bdy, prm, ip, sc.bound_min, sc.bound_max,
+                                               sim.cfl_lambda, sim.iter_target, sim.change_factor, dt, it, steps)
+        return secs, its
+    warm, warm_its = run(5)
+    runs = [run(args.cpu_steps) for _ in range(3)]
+    per_run = sorted(float(secs.sum()) for secs, _ in runs)
+    median = per_run[1]
+    its_all = [int(v) for _, its in runs for v in its]
+    value = sc.n * args.cpu_steps / median
+    log(f"cpu baseline: 5 warm-up steps {warm.sum():.1f} s, runs of {args.cpu_steps} steps {per_run} s on "
+        f"{threads} threads ({model}), iterations {its_all}")
+    cpu = {"value": value, "unit": "particle-steps/s", "cores": threads, "kind": "port", "cpu_model": model,
+           "sample": f"the same {sc.n}-particle IISPH main loop from the GPU's state after step {sim.steps}: 5 "
+                     f"warm-up steps, then the median of 3 runs of {args.cpu_steps} steps (Jacobi iterations "
+                     f"{its_all}), oracle FP32 OpenMP port with a parallel counting sort and rocThrust OpenMP "
+                     f"scan and gathers, {median:.2f} s per run",
+           "runs_s": per_run, "warmup_s": float(warm.sum())}
+
+    c1 = None
+    if not args.no_config1:
+        s1 = scenes.config1_sesph()
+        h1, r1 = s1.spacing, 2.0 * s1.spacing
+        st = Particles(ctx, s1.x, None, s1.mass, s1.rho_0)
+        b1 = BoundarySamples(ctx, s1.xb, h1, s1.rho_0, 1.0)
+        solver = SESPH(ctx, "C2", s1.n, 0.001, h1, s1.rho_0, k=1000.0)
+        sim1 = Simulation(ctx, solver, st, b1, h1, s1.bound_min, s1.bound_max, cfl_lambda=0.4, cell_ratio=1.0)
+        ob1 = parity.oracle_boundary(b1, r1)
+        for _ in range(5):
+            sim1.step()
+        ctx.synchronize()
+        t = time.perf_counter()
+        for _ in range(995):
+            sim1.step()
+        ctx.synchronize()
+        gpu_s = time.perf_counter() - t
+        port1 = orc.PortState(s1.x, None, s1.mass)
+        secs, simt = orc.port_sesph_run(orc.geometry(s1.bound_min, s1.bound_max, r1, 1.0), port1,
+                                        orc.Bdy(ob1.grid, ob1.x, ob1.y, ob1.z, ob1.m, dtype=np.float32),
+                                        orc.sph(capi.KERNEL_C2, r1, h1, 0.001, s1.rho_0), 1000.0, s1.bound_min,
+                                        s1.bound_max, 0.4, 1000)
+        cpu_s = float(secs[5:].sum())
+        c1 = {"workload": f"SESPH dam break, {s1.n} fluid particles (32^3 at s = 0.02, jitter 1%, seed 1) + "
+                          f"{s1.n_bdy} boundary samples in [0,1]^3, C2, k = 1000, CFL lambda 0.4 every step, cell = "
+                          "support radius, 1000 steps; rates over steps 5-999",
+              "gpu_particle_steps_per_s": round(s1.n * 995 / gpu_s, 1),
+              "cpu_particle_steps_per_s": round(s1.n * 995 / cpu_s, 1), "cpu_threads": threads,
+              "cpu_model": model, "cpu_full_run_s": round(float(secs.sum()), 3),
+              "cpu_simulated_time": simt, "gpu_simulated_time": sim1.time}
+        log(f"configuration 1: GPU {c1['gpu_particle_steps_per_s'] / 1e6:.1f} M, CPU "
+            f"{c1['cpu_particle_steps_per_s'] / 1e6:.2f} M particle-steps/s over steps 5-999")
+        for o in (solver, b1, st, sim1.builder):
+            getattr(o, "close", lambda: None)()
+    return cpu, fp64, c1
 
 
 def grid_leg(ctx, args, log) -> dict:
@@ -244,10 +301,12 @@ def grid_leg(ctx, args, log) -> dict:
 
 
 def batch_leg(ctx, args, log) -> dict:
-    """64 PCISPH scenes in one batch (autotuned) against 64 sequential plain
-    solves (the reference's main loop per scene through the single-scene
-    entry points); both timed over `batch_steps` steps after 2 untimed ones,
-    set-up excluded."""
+    """64 PCISPH scenes in one batch (autotuned cell ratio and check interval)
+    with every step after the first replayed as captured HIP graphs, and the
+    same batch launched directly, against 64 sequential plain solves (the
+    reference's main loop per scene through the single-scene entry points) at
+    the same cell ratio and check interval; all timed over `batch_steps` steps
+    after 2 untimed ones, set-up excluded."""
     from reyn_amd import scenes
     from reyn_amd.sim import PCISPH, Batch, BoundarySamples, Particles, Simulation
     from reyn_amd.tune import autotune_batch
@@ -256,23 +315,31 @@ def batch_leg(ctx, args, log) -> dict:
     n = sum(sc.n for sc in scs)
     opts = dict(solver="pcisph", kernel="W6", eta_rho_max=0.001, min_iter=3, max_iter=100)
     best, trials = autotune_batch(ctx, scs, log=log, **opts)
-    b = Batch(ctx, scs, **opts, **best)
-    b.step(2)
-    ctx.synchronize()
-    t = time.perf_counter()
-    b.step(args.batch_steps)
-    ctx.synchronize()
-    t_batch = time.perf_counter() - t
-    it_b = [b.info(k).iterations for k in range(len(scs))]
-    b.close()
+
+    def run_batch(graph):
+        b = Batch(ctx, scs, graph=graph, **opts, **best)
+        b.step(2)
+        ctx.synchronize()
+        t = time.perf_counter()
+        b.step(args.batch_steps)
+        ctx.synchronize()
+        dt = time.perf_counter() - t
+        its = [b.info(k).iterations for k in range(len(scs))]
+        b.close()
+        return dt, its
+    t_graph, it_b = run_batch(True)
+    t_direct, it_d = run_batch(False)
+    assert it_b == it_d, "captured and direct batches took different iterations"
     t_seq = 0.0
     for sc in scs:
         h = sc.spacing
         state = Particles(ctx, sc.x, None, sc.mass, sc.rho_0)
         bdy = BoundarySamples(ctx, sc.xb, h, sc.rho_0)
-        solver = PCISPH(ctx, "W6", sc.n, 0.001, h, sc.rho_0, eta_rho_max=0.001, min_iter=3, max_iter=100)
+        solver = PCISPH(ctx, "W6", sc.n, 0.001, h, sc.rho_0, eta_rho_max=0.001, min_iter=3, max_iter=100,
+                        check_every=best["check_every"])
         sim = Simulation(ctx, solver, state, bdy, h, sc.bound_min, sc.bound_max, cfl_lambda=1.0,
-                         dt_policy="controller", dt0=0.0005, iter_target=10, change_factor=0.001)
+                         cell_ratio=best["cell_ratio"], dt_policy="controller", dt0=0.0005, iter_target=10,
+                         change_factor=0.001)
         sim.step()
         sim.step()
         ctx.synchronize()
@@ -283,18 +350,25 @@ def batch_leg(ctx, args, log) -> dict:
         t_seq += time.perf_counter() - t
         for o in (solver, bdy, state, sim.builder):
             getattr(o, "close", lambda: None)()
-    v_b = n * args.batch_steps / t_batch
+    v_b = n * args.batch_steps / t_graph
+    v_d = n * args.batch_steps / t_direct
     v_s = n * args.batch_steps / t_seq
-    log(f"batch: {v_b / 1e6:.1f} M particle-steps/s batched, {v_s / 1e6:.1f} sequential plain, "
-        f"x{v_b / v_s:.2f}; iterations {min(it_b)}..{max(it_b)}")
+    log(f"batch: {v_b / 1e6:.1f} M particle-steps/s batched with graphs, {v_d / 1e6:.1f} M launched directly, "
+        f"{v_s / 1e6:.1f} M sequential plain at the same settings, x{v_b / v_s:.2f}; iterations "
+        f"{min(it_b)}..{max(it_b)}")
     return {
         "workload": f"64 PCISPH dam-break scenes of 10k-300k particles (log-uniform, seed 4; {n} in total), "
                     "W6 kernel, eta 0.1%, min 3 iterations, time-step controller, grid + reorder every step",
         "particles": n, "steps": args.batch_steps,
         "batched_particle_steps_per_s": round(v_b, 1),
+        "batched_direct_particle_steps_per_s": round(v_d, 1),
         "sequential_plain_particle_steps_per_s": round(v_s, 1),
         "speedup": round(v_b / v_s, 3),
-        "batched_s": round(t_batch, 4), "sequential_s": round(t_seq, 4),
+        "speedup_direct": round(v_d / v_s, 3),
+        "settings": f"cell ratio {best['cell_ratio']} and check interval {best['check_every']} (autotuned) for the "
+                    "batch and the sequential solves alike; the batch replays each step after the first as "
+                    "captured HIP graphs (direct launches reported beside)",
+        "batched_s": round(t_graph, 4), "batched_direct_s": round(t_direct, 4), "sequential_s": round(t_seq, 4),
         "last_iterations_range": [min(it_b), max(it_b)],
         "autotune": {"chosen": best, "trials": [tr.as_dict() for tr in trials]},
     }
@@ -364,11 +438,16 @@ def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=480,
+                    help="untimed steps of the rollout before the timed window (480: the window is steps 480-499)")
+    ap.add_argument("--early-window", type=int, nargs=2, default=(5, 25), metavar=("FIRST", "END"),
+                    help="steps [FIRST, END) of the warm-up timed on their own (the early, near-static state)")
     ap.add_argument("--side", type=int, default=128, help="fluid lattice side (128 -> 2,097,152 particles)")
-    ap.add_argument("--cpu-steps", type=int, default=10,
-                    help="IISPH steps of the CPU baseline sample (about 1 s each on 16 cores)")
-    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=3,
+                    help="IISPH steps per run of the CPU baseline sample (about 1-2 s each on 16 cores)")
+    ap.add_argument("--no-cpu-baseline", action="store_true",
+                    help="skip the oracle leg (FP64 check of one step, CPU baseline, configuration 1)")
+    ap.add_argument("--no-config1", action="store_true", help="skip configuration 1 (GPU and CPU) in the oracle leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batch leg")
     ap.add_argument("--batch-steps", type=int, default=8)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
@@ -462,9 +541,24 @@ def main() -> int:
     ncells = sim.builder.geometry().ncells
     log(f"setup {time.perf_counter() - t_setup:.1f} s; fluid grid {ncells} cells")
 
+    # the rollout: `warmup` untimed steps (the early window of steps
+    # early[0] .. early[1] - 1 is timed by itself on the way), then the timed
+    # window of `steps` steps
+    early = tuple(args.early_window)
+    hist = {}
+    t_roll = time.perf_counter()
+    early_s = None
     for k in range(args.warmup):
+        if k == early[0]:
+            ctx.synchronize()
+            t_e = time.perf_counter()
         s = sim.step()
-        log(f"warmup {k}: dt {s.dt:.3e} iters {s.iterations} mean err {s.mean_rel_error:.2e}")
+        hist[s.iterations] = hist.get(s.iterations, 0) + 1
+        if k + 1 == early[1]:
+            ctx.synchronize()
+            early_s = time.perf_counter() - t_e
+        if k % 40 == 0 or k + 1 == args.warmup:
+            log(f"warmup {k}: dt {s.dt:.3e} iters {s.iterations} mean err {s.mean_rel_error:.2e}")
 
     ctx.synchronize()
     ctx._lib.reyn_profile_enable(ctx.h, 0 if args.no_profile else 1)
@@ -478,10 +572,13 @@ def main() -> int:
         iters.append(s.iterations)
         errs.append(s.mean_rel_error)
         dts.append(s.dt)
+        hist[s.iterations] = hist.get(s.iterations, 0) + 1
     gpu_ms = ctx.timer_stop()
     ctx.synchronize()
     wall = time.perf_counter() - t0
-    log(f"timed {args.steps} steps: {gpu_ms:.1f} ms (events), {wall * 1e3:.1f} ms (wall); iterations {iters}")
+    roll_wall = time.perf_counter() - t_roll
+    log(f"timed steps {args.warmup}..{args.warmup + args.steps - 1}: {gpu_ms:.1f} ms (events), {wall * 1e3:.1f} ms "
+        f"(wall); iterations {iters}; simulated time {sim.time:.4f} s")
 
     prof = {}
     if not args.no_profile:
@@ -527,18 +624,14 @@ def main() -> int:
                     "valu_frac": round(vi / (avg_us * 1e-6) / VALU_PEAK_INSTS, 4) if vi else None}
         step_valu_frac = valu_total / (gpu_ms * 1e-3) / VALU_PEAK_INSTS
     mean_iters = float(np.mean(iters))
+    sim_time_end = sim.time
     sbytes = step_bytes(sc.n, sc.n_bdy, ncells, mean_iters)
     step_frac = sbytes / (ms_per_step * 1e-3) / (HBM_PEAK_GBS * 1e9)
 
     rho_err = abs(reduce(ctx, solver.rho, "avg") / sc.rho_0 - 1.0)
-    cpu = None
+    cpu, fp64, config1 = None, None, None
     if not args.no_cpu_baseline:
-        host = state.host()
-        # boundary samples in their stored (cell) order, with the GPU's masses
-        sc.xb_sorted = bdy.positions()
-        sc.mb = bdy.m.numpy()
-        sc.bdy_min, sc.bdy_max = bdy.bound_min, bdy.bound_max
-        cpu = cpu_baseline(sc, host, solver.p.numpy(), sim.next_dt(), args, log)
+        cpu, fp64, config1 = cpu_leg(ctx, sim, sc, args, log)
     sim.close()
     batch = None if args.no_batch else batch_leg(ctx, args, log)
     serving = None if args.no_serve else serve_leg(ctx, args, log)
@@ -567,8 +660,18 @@ def main() -> int:
             "grid_cells": ncells,
             "cell_ratio": ratio,
             "cell_ratio_autotune": tune,
+            "timed_steps": [args.warmup, args.warmup + args.steps - 1],
             "jacobi_iterations_mean": round(mean_iters, 2),
             "jacobi_iterations": iters,
+            "rollout_jacobi_iterations_histogram": {str(k): hist[k] for k in sorted(hist)},
+            "rollout_steps": args.warmup + args.steps,
+            "rollout_particle_steps_per_s": round(sc.n * (args.warmup + args.steps) / roll_wall, 1),
+            "early_window": ({"steps": [early[0], early[1] - 1],
+                              "particle_steps_per_s": round(sc.n * (early[1] - early[0]) / early_s, 1),
+                              "ms_per_step": round(early_s * 1e3 / (early[1] - early[0]), 4)}
+                             if early_s else None),
+            "simulated_time_end": sim_time_end,
+            "fp64_step_errors": fp64,
             "mean_density_error_max": max(errs) if errs else None,
             "dt_policy": args.dt_policy,
             "cfl_lambda": lam,
@@ -583,6 +686,7 @@ def main() -> int:
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "config1": config1,
         "batch": batch,
         "serving": serving,
         "grid": grid,

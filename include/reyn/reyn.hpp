@@ -569,6 +569,7 @@ public:
         prm_.eta_rho_max = 0.001f;
         prm_.min_iter = 3;
         prm_.max_iter = 0;
+        prm_.check_every = 4;
         for (auto* b : { &rho_, &ax_, &ay_, &az_ })
             b->resize(N);
     }

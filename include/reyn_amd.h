@@ -29,7 +29,7 @@ extern "C" {
 #pragma GCC visibility push(default)
 #endif
 
-#define REYN_ABI_VERSION 2
+#define REYN_ABI_VERSION 3
 
 /* ---- status codes ------------------------------------------------------ */
 enum {
@@ -290,6 +290,7 @@ typedef struct reyn_pcisph_params {
     float eta_rho_max; /* 0.001 (PCISPH.cuh:36) */
     uint32_t min_iter; /* 3 (PCISPH.cuh:38) */
     uint32_t max_iter; /* 0 = no cap, as in the reference */
+    uint32_t check_every; /* iterations launched between host checks; 0 = 4 */
 } reyn_pcisph_params;
 
 typedef struct reyn_pcisph_work {

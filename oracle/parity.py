@@ -25,6 +25,71 @@ def rel_err(gpu, ref) -> float:
     return float(np.abs(gpu - ref).max() / scale) if scale > 0 else float(np.abs(gpu).max())
 
 
+def elem_rel_err(gpu, ref, floor: float = 1e-3) -> float:
+    """max_i |gpu_i - ref_i| / max(|ref_i|, floor max_j |ref_j|): the error of
+    each value relative to itself, with a floor for values near zero (rows of
+    vectors use their largest component)."""
+    gpu = np.asarray(gpu, np.float64)
+    ref = np.asarray(ref, np.float64)
+    if ref.size == 0:
+        return 0.0
+    mag = np.abs(ref) if ref.ndim == 1 else np.abs(ref).max(axis=1)
+    diff = np.abs(gpu - ref) if ref.ndim == 1 else np.abs(gpu - ref).max(axis=1)
+    scale = np.maximum(mag, floor * mag.max())
+    return float((diff / np.where(scale > 0, scale, 1.0)).max())
+
+
+def oracle_boundary(bdy, r: float):
+    """The oracle's view of a GPU boundary (its samples are stored in cell
+    order with the GPU's Akinci masses)."""
+    xb = bdy.positions()
+    bgeom = orc.geometry(bdy.bound_min, bdy.bound_max, r, bdy.cell_ratio)
+    bgrid = orc.grid_build(bgeom, xb[:, 0], xb[:, 1], xb[:, 2])
+    assert np.array_equal(bgrid.sorted, np.arange(xb.shape[0])), "boundary is not in cell order"
+    return orc.Bdy(bgrid.resorted(), xb[:, 0], xb[:, 1], xb[:, 2], bdy.m.numpy())
+
+
+def iisph_sim_step_parity(sim, ob=None):
+    """One step of an IISPH `Simulation` (time step, grid + reorder of the
+    state and the pressure, solver step, bounds) on the GPU, and the same step
+    in the FP64 oracle from the GPU's state before it, with the GPU's time step,
+    iteration count and solver parameters. Returns (errors, gpu, ref): per
+    field the error relative to the field's largest value and per element
+    (`elem_rel_err`)."""
+    ctx, solver = sim.ctx, sim.solver
+    h = sim.h
+    r = 2.0 * h
+    hs = sim.state.host()
+    p_in = solver.p.numpy()
+    ob = ob if ob is not None else oracle_boundary(sim.bdy, r)
+    s = sim.step()
+    out = sim.state.host()
+    gpu = {"rho": solver.rho.numpy(), "p": solver.p.numpy(), "a_ii": solver.a_ii.numpy(),
+           "s_i": solver.s_i.numpy(), "x": np.stack([out["xx"], out["xy"], out["xz"]], 1),
+           "v": np.stack([out["vx"], out["vy"], out["vz"]], 1)}
+    geom = orc.geometry(sim.bound_min, sim.bound_max, r, sim.cell_ratio)
+    g = orc.grid_build(geom, hs["xx"], hs["xy"], hs["xz"])
+    order = g.sorted
+    st = orc.State({"x": hs["xx"][order], "y": hs["xy"][order], "z": hs["xz"][order], "vx": hs["vx"][order],
+                    "vy": hs["vy"][order], "vz": hs["vz"][order], "m": hs["m"][order]})
+    x0 = np.stack([st.x, st.y, st.z], 1).astype(np.float32)
+    sp, q = solver.prm.sph, solver.prm
+    prm = orc.sph(sp.kernel, sp.h_bar, sp.h, sp.nu, sp.rho_0, g=(sp.g.x, sp.g.y, sp.g.z))
+    ip = orc.OrcIisph(q.omega, q.eta_rho_max, q.min_iter, q.max_iter, q.stop_rule, q.warm_start)
+    ref = orc.iisph_step(g.resorted(), st, ob, prm, ip, p_in[order], s.dt, fixed_iters=s.iterations)
+    x, v = orc.hard_enforce_bounds(sim.bound_min, sim.bound_max, np.stack([st.x, st.y, st.z], 1),
+                                   np.stack([st.vx, st.vy, st.vz], 1))
+    ref["x"], ref["v"] = x, v
+    errs = {}
+    for k in ("rho", "p", "a_ii", "s_i", "x", "v"):
+        errs[k] = rel_err(gpu[k], ref[k])
+        errs[k + "_elem"] = elem_rel_err(gpu[k], ref[k])
+    errs["dx"] = rel_err(gpu["x"] - x0, ref["x"] - x0)
+    errs["dt"] = s.dt
+    errs["iterations"] = s.iterations
+    return errs, gpu, ref
+
+
 def make_case(side=14, spacing=0.05, compress=0.97, vel=0.2, seed=7, box=(1.4, 1.2, 1.0), sphere=None):
     """A fluid block slightly compressed (lattice spacing compress*s) with
     random velocities, in a walled box; returns (scene, velocities)."""
@@ -158,3 +223,60 @@ def run_pcisph_case(ctx, kernel="W6", dt=1e-3, side=14, seed=13, min_iter=3, max
     errs["dx"] = rel_err(gpu["x"] - x0, ref["x"] - x0)
     errs["avg"] = abs(gpu["avg"] - avg) / avg
     return errs, gpu, ref
+
+
+def batch_scene_step_parity(batch, sid: int, bdy_iterations: int = 5):
+    """One step of a `Batch` (every resident scene advances) and the same step
+    of scene `sid` in the FP64 oracle from that scene's state before it: the
+    grid over the scene's domain at the batch's cell ratio, the reorder, the
+    batch's PCISPH or IISPH step with the scene's time step and iteration
+    count, the bounds. The oracle's boundary is the scene's samples with FP64
+    Akinci masses. Returns per-field errors relative to the largest value and
+    per element (`elem_rel_err`)."""
+    import ctypes as C
+
+    sc = batch.scenes[sid]
+    prm = batch.prm
+    before = batch.download(sid)
+    batch.step(1)
+    after = batch.download(sid)
+    inf = batch.info(sid)
+    h = float(sc.spacing)
+    r = 2.0 * h
+    geom = orc.geometry(sc.bound_min, sc.bound_max, r, prm.cell_ratio)
+    g = orc.grid_build(geom, before["xx"], before["xy"], before["xz"])
+    o = g.sorted
+    st = orc.State({"x": before["xx"][o], "y": before["xy"][o], "z": before["xz"][o], "vx": before["vx"][o],
+                    "vy": before["vy"][o], "vz": before["vz"][o], "m": np.full(sc.n, np.float32(sc.mass))})
+    xb = np.asarray(sc.xb, np.float32)
+    bg = orc.grid_build(orc.geometry(xb.min(0), xb.max(0), r, prm.cell_ratio), xb[:, 0], xb[:, 1], xb[:, 2])
+    xb = xb[bg.sorted]
+    bg = orc.grid_build(orc.geometry(xb.min(0), xb.max(0), r, prm.cell_ratio), xb[:, 0], xb[:, 1],
+                        xb[:, 2]).resorted()
+    mb = orc.boundary_masses(bg, xb[:, 0], xb[:, 1], xb[:, 2], h, prm.rho_0, bdy_iterations)
+    ob = orc.Bdy(bg, xb[:, 0], xb[:, 1], xb[:, 2], mb)
+    sph = orc.sph(prm.kernel, r, h, prm.nu, prm.rho_0, g=(prm.g.x, prm.g.y, prm.g.z))
+    if prm.solver == capi.SOLVER_PCISPH:
+        delta = C.c_double()
+        capi.lib().reyn_pcisph_delta(prm.kernel, np.float32(r), np.float32(h), C.byref(delta))
+        _, rho, _, _ = orc.pcisph_step(g.resorted(), st, ob, sph, delta.value, inf.dt, eta=prm.eta_rho_max,
+                                       min_iter=prm.min_iter, max_iter=prm.max_iter, fixed_iters=inf.iterations)
+        ref = {"rho": rho}
+    else:
+        ip = orc.OrcIisph(prm.omega, prm.eta_rho_max, prm.min_iter, prm.max_iter, prm.stop_rule, prm.warm_start)
+        res = orc.iisph_step(g.resorted(), st, ob, sph, ip, before["p"][o], inf.dt, fixed_iters=inf.iterations)
+        ref = {"rho": res["rho"], "p": res["p"]}
+    x, v = orc.hard_enforce_bounds(sc.bound_min, sc.bound_max, np.stack([st.x, st.y, st.z], 1),
+                                   np.stack([st.vx, st.vy, st.vz], 1))
+    ref["x"], ref["v"] = x, v
+    gpu = {"rho": after["rho"], "p": after["p"], "x": np.stack([after["xx"], after["xy"], after["xz"]], 1),
+           "v": np.stack([after["vx"], after["vy"], after["vz"]], 1)}
+    x0 = np.stack([st.x, st.y, st.z], 1).astype(np.float32)
+    errs = {}
+    for k in ref:
+        errs[k] = rel_err(gpu[k], ref[k])
+        errs[k + "_elem"] = elem_rel_err(gpu[k], ref[k])
+    errs["dx"] = rel_err(gpu["x"] - x0, ref["x"] - x0)
+    errs["dt"] = float(inf.dt)
+    errs["iterations"] = int(inf.iterations)
+    return errs

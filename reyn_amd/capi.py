@@ -12,7 +12,7 @@ from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "libreyn_amd.so"
 #: the C ABI version this binding is written for (REYN_ABI_VERSION of include/reyn_amd.h)
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 # status codes
 OK = 0
@@ -105,6 +105,7 @@ class PcisphParams(C.Structure):
         ("eta_rho_max", f32),
         ("min_iter", u32),
         ("max_iter", u32),
+        ("check_every", u32),
     ]
 
 

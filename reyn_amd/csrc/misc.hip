@@ -97,7 +97,10 @@ __global__ __launch_bounds__(kBlock) void k_bounds(f3 lo, f3 hi, FluidDev f)
         st3(f.vx, f.vy, f.vz, i, c * ld3(f.vx, f.vy, f.vz, i));
     }
     const f3 cl { fmaxf(fminf(x.x, hi.x), lo.x), fmaxf(fminf(x.y, hi.y), lo.y), fmaxf(fminf(x.z, hi.z), lo.z) };
-    st3(f.x, f.y, f.z, i, cl);
+    // only clamped positions are written back (the same values; inside the
+    // box nothing changes and nothing is stored)
+    if (!(cl.x == x.x && cl.y == x.y && cl.z == x.z))
+        st3(f.x, f.y, f.z, i, cl);
 }
 
 // one Jacobi refinement of Akinci masses: m_new = rho0 m_old / sum_j m_old_j W_bj

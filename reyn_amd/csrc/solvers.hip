@@ -1042,11 +1042,11 @@ int pcisph_end(reyn_ctx* ctx, const Common& c, const reyn_pcisph_work* work)
 
 template <bool BT>
 int pcisph_run(reyn_ctx* ctx, int kernel, const Common& c, const reyn_pcisph_work* work, const Scratch& s,
-    uint32_t nscenes, uint32_t min_iter, uint32_t max_iter, SolveCtl* res)
+    uint32_t nscenes, uint32_t min_iter, uint32_t max_iter, uint32_t check_every, SolveCtl* res)
 {
     REYN_TRY(pcisph_begin<BT>(ctx, kernel, c, work, s));
     REYN_TRY(run_loop(
-        ctx, c, s, nscenes, min_iter, max_iter, 4,
+        ctx, c, s, nscenes, min_iter, max_iter, check_every,
         [&]() -> int { return pcisph_iteration<BT>(ctx, kernel, c, work, s, nscenes, min_iter, max_iter); },
         res));
     return pcisph_end<BT>(ctx, c, work);
@@ -1442,7 +1442,8 @@ int reyn_pcisph_step(reyn_ctx* ctx, const reyn_pcisph_params* prm, const reyn_pa
     Scratch s;
     REYN_TRY(get_scratch(ctx, fluid->n, &s, true, 0, 1, c.a.one.b.present ? c.a.one.b.n : 0));
     SolveCtl res;
-    REYN_TRY(pcisph_run<false>(ctx, prm->sph.kernel, c, work, s, 1, prm->min_iter, prm->max_iter, &res));
+    REYN_TRY(pcisph_run<false>(ctx, prm->sph.kernel, c, work, s, 1, prm->min_iter, prm->max_iter, prm->check_every,
+        &res));
     if (info) {
         info->iterations = res.iter;
         info->avg_density = res.avg_density;

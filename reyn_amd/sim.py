@@ -166,6 +166,7 @@ class BoundarySamples:
         self.m = ctx.full(self.n, 1.0)
         self.bound_min = xb.min(axis=0).astype(np.float32)
         self.bound_max = xb.max(axis=0).astype(np.float32)
+        self.cell_ratio = float(cell_ratio)
         self.builder = UniformGridBuilder(ctx, self.bound_min, self.bound_max, 2.0 * h, cell_ratio)
         # the boundary grid is built once and the samples are stored in cell order
         self.grid = self.builder.construct_and_reorder(2.0 * h, [(self, "xs"), (self, "ys"), (self, "zs")])
@@ -221,12 +222,14 @@ class SESPH(_Solver):
 class PCISPH(_Solver):
     """src/solvers/PCISPH.cuh:8-111."""
 
-    def __init__(self, ctx, kernel, n, nu, h, rho_0, g=GRAVITY, eta_rho_max=0.001, min_iter=3, max_iter=0):
+    def __init__(self, ctx, kernel, n, nu, h, rho_0, g=GRAVITY, eta_rho_max=0.001, min_iter=3, max_iter=0,
+                 check_every=4):
         super().__init__(ctx, n)
         sph = sph_params(kernel, h, nu, rho_0, g)
         delta = C.c_double()
         capi.lib().reyn_pcisph_delta(sph.kernel, sph.h_bar, sph.h, C.byref(delta))
-        self.prm = capi.PcisphParams(sph, delta.value, float(eta_rho_max), int(min_iter), int(max_iter))
+        self.prm = capi.PcisphParams(sph, delta.value, float(eta_rho_max), int(min_iter), int(max_iter),
+                                     int(check_every))
         self.info = capi.SolveInfo()
 
     def step(self, state: Particles, grid: capi.Grid, bdy, dt: float) -> int:
@@ -327,6 +330,7 @@ class Simulation:
         self.bound_min = np.asarray(bound_min, np.float32)
         self.bound_max = np.asarray(bound_max, np.float32)
         self.cfl_lambda = cfl_lambda
+        self.cell_ratio = float(cell_ratio)
         self.builder = UniformGridBuilder(ctx, self.bound_min, self.bound_max, 2.0 * h, cell_ratio)
         self.reorder_every = max(1, int(reorder_every))
         self.extra_reorder = extra_reorder
