@@ -197,6 +197,14 @@ int reyn_grid_construct_and_reorder(reyn_ctx* ctx, reyn_grid_builder* b,
     float search_radius, float** fields, float** tmp, uint32_t nfields,
     uint32_t n, reyn_grid* out);
 
+/* The most points one cell of a grid whose queries search `range` cells on
+ * each side may hold: a build in which a cell holds more fails with
+ * REYN_ERR_CAPACITY (after finishing its tables), so that no neighbour walk
+ * over a built grid, (2 range + 1)^3 cells, exceeds its budget of 2^15
+ * candidates. Host only. No reference counterpart (the reference's walk has
+ * no bound, uniformgrid.cuh:113-178). */
+int reyn_grid_cell_capacity(int32_t range, uint32_t* cap);
+
 /* Device copies of the builder's last per-cell counts (ncells) and per-point
  * cell keys (n), for tests. */
 int reyn_grid_debug_tables(reyn_grid_builder* b, const uint32_t** counts,

@@ -75,6 +75,7 @@ def lib() -> C.CDLL:
         _lib.orc_pcisph_delta_f64.restype = f64
         _lib.orc_pcisph_delta_f64.argtypes = [C.c_int, f32, f32]
         _lib.orc_num_threads.restype = C.c_int
+        _lib.orc_grid_cell_capacity.restype = u32
     return _lib
 
 
@@ -134,9 +135,16 @@ def grid_build(geom: OrcGrid, x, y, z) -> Grid:
     keys = np.empty(n, np.uint32)
     outside = lib().orc_grid_build(C.byref(geom), _p(x), _p(y), _p(z), u32(n), _p(counts), _p(prefix),
                                    _p(sorted_), _p(keys))
+    if outside < 0:
+        raise ValueError(f"a grid cell holds more than {grid_cell_capacity(geom.range)} points")
     g = OrcGrid()
     C.pointer(g)[0] = geom
     return Grid(g, prefix, sorted_, counts, keys, int(outside))
+
+
+def grid_cell_capacity(range_: int) -> int:
+    """The most points per cell a grid build accepts at this search range."""
+    return int(lib().orc_grid_cell_capacity(int(range_)))
 
 
 def query_stats(grid: Grid, x, y, z):

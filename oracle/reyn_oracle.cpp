@@ -66,7 +66,16 @@ inline int cell_axis(float x, float lo, float cell, int n)
     return static_cast<int>(f);
 }
 
+// The GPU library's walk budget and the most points per cell a grid build
+// accepts, (2^15 candidates) / (2 range + 1)^3: a build over the limit fails
+// there with REYN_ERR_CAPACITY, and here with a negative return, so no walk
+// over a built grid has more candidates than the budget. The walk below is
+// never clipped.
 constexpr uint32_t kWalkBudget = 1u << 15;
+inline uint32_t walk_cell_cap(int range)
+{
+    return kWalkBudget / uint32_t((2 * range + 1) * (2 * range + 1) * (2 * range + 1));
+}
 
 // Strip walk of `ff_nbrs` (uniformgrid.cuh:113-178), clipped to the grid.
 // f(j) is called for every neighbour j, in walk order.
@@ -82,15 +91,11 @@ inline void walk(const OGrid& g, const T* xs, const T* ys, const T* zs, T xi, T 
     const int y0 = std::max(cy - R, 0), y1 = std::min(cy + R, g.n[1] - 1);
     const int z0 = std::max(cz - R, 0), z1 = std::min(cz + R, g.n[2] - 1);
     const long nx = g.n[0], nxny = long(g.n[0]) * g.n[1];
-    uint32_t budget = kWalkBudget;
     for (int z = z0; z <= z1; ++z)
         for (int y = y0; y <= y1; ++y) {
             const long base = z * nxny + y * nx;
             const uint32_t s = g.prefix[base + x0];
-            uint32_t e = g.prefix[base + x1 + 1];
-            if (e - s > budget)
-                e = s + budget;
-            budget -= e - s;
+            const uint32_t e = g.prefix[base + x1 + 1];
             for (uint32_t q = s; q < e; ++q) {
                 const uint32_t j = g.sorted ? g.sorted[q] : q;
                 const float dx = fx - static_cast<float>(xs[j]);
@@ -640,7 +645,8 @@ ORC_API int orc_grid_geometry(const float* bmin, const float* bmax, float r, flo
 }
 
 // Counting sort (uniformgrid.cu:20-63, 98-143) with ascending index inside a
-// cell. Returns the number of points outside the query-safe cells.
+// cell. Returns the number of points outside the query-safe cells, or -1 (with
+// the tables complete) when a cell holds more than walk_cell_cap(range) points.
 ORC_API long orc_grid_build(const orc_grid* geom, const float* x, const float* y, const float* z, uint32_t n,
     uint32_t* counts, uint32_t* prefix, uint32_t* sorted, uint32_t* keys)
 {
@@ -667,8 +673,14 @@ ORC_API long orc_grid_build(const orc_grid* geom, const float* x, const float* y
     std::vector<uint32_t> fill(prefix, prefix + C);
     for (uint32_t i = 0; i < n; ++i)
         sorted[fill[keys[i]]++] = i;
+    const uint32_t cap = walk_cell_cap(R);
+    for (size_t c = 0; c < C; ++c)
+        if (counts[c] > cap)
+            return -1;
     return outside;
 }
+
+ORC_API uint32_t orc_grid_cell_capacity(int range) { return walk_cell_cap(range); }
 
 // Reference grid test statistics (uniformgrid.cu:185-208) through the walk.
 ORC_API void orc_query_stats(const orc_grid* geom, const float* x, const float* y, const float* z, uint32_t n,

@@ -176,6 +176,7 @@ class BatchParams(C.Structure):
 P = C.POINTER
 _SIGS = {
     "reyn_abi_version": (i32, []),
+    "reyn_grid_cell_capacity": (i32, [i32, P(u32)]),
     "reyn_device_count": (i32, [P(C.c_int)]),
     "reyn_ctx_create": (i32, [C.c_int, P(vp)]),
     "reyn_ctx_destroy": (i32, [vp]),

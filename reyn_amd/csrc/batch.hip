@@ -463,6 +463,9 @@ int relayout(reyn_batch* b, const std::vector<SceneRes*>& next, const std::vecto
         REYN_HIP(ctx, hipHostMalloc(reinterpret_cast<void**>(&L.blk_list_host), sizeof(uint32_t) * L.nblocks,
             hipHostMallocDefault));
         REYN_TRY(builder_alloc(ctx, static_cast<size_t>(cells), 0.f, &L.fgrid));
+        // every scene's grid has the batch's cell ratio, so the same range
+        if (S > 0)
+            L.fgrid->cell_cap = walk_cell_cap(next[0]->g.range);
         // fluid state: resident scenes copied from the current state buffer,
         // new scenes uploaded
         std::vector<float> zeros;

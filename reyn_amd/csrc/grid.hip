@@ -468,6 +468,11 @@ int finish_build(reyn_grid_builder* b, const uint32_t* status_host)
         return set_error(ctx, REYN_ERR_CAPACITY,
             "construct: %u cells hold more than %u points (largest %u); the scene is degenerate",
             status_host[2], kMaxStableCell, status_host[1]);
+    if (b->cell_cap && status_host[1] > b->cell_cap)
+        return set_error(ctx, REYN_ERR_CAPACITY,
+            "construct: a cell holds %u points, more than the %u a neighbour walk over the cells in reach "
+            "may visit within its %u-candidate budget; the scene is degenerate",
+            status_host[1], b->cell_cap, kWalkBudget);
     if (status_host[0] > 0)
         return set_error(ctx, REYN_ERR_OUT_OF_BOUNDS,
             "construct: %u points lie outside the grid's query-safe cells and were clamped",
@@ -589,7 +594,16 @@ int reyn_grid_builder_create(reyn_ctx* ctx, reyn_float3 bmin, reyn_float3 bmax,
     reyn_grid_builder* b = nullptr;
     REYN_TRY(builder_alloc(ctx, ncells, search_radius, &b));
     b->geom = geom;
+    b->cell_cap = walk_cell_cap(geom.range);
     *out = b;
+    return REYN_OK;
+}
+
+int reyn_grid_cell_capacity(int32_t range, uint32_t* cap)
+{
+    if (!cap || range < 1 || range > 16)
+        return REYN_ERR_INVALID_ARGUMENT;
+    *cap = walk_cell_cap(range);
     return REYN_OK;
 }
 

@@ -109,6 +109,9 @@ int check_bdy(reyn_ctx* ctx, const reyn_boundary* b)
     if (b->grid.sorted)
         return set_error(ctx, REYN_ERR_INVALID_ARGUMENT,
             "boundary grid must be cell-ordered (Resort::yes)");
+    // packed 16-byte boundary records are addressed with 32-bit byte offsets
+    if (b->n >= (1u << 28))
+        return set_error(ctx, REYN_ERR_CAPACITY, "at most 2^28 - 1 boundary samples per call (got %u)", b->n);
     return check_grid(ctx, &b->grid, b->n, "boundary");
 }
 

@@ -188,7 +188,16 @@ __device__ __forceinline__ uint32_t cell_key(const GridDev& g, int3c c)
     return static_cast<uint32_t>(c.z) * static_cast<uint32_t>(g.nxny) + static_cast<uint32_t>(c.y * g.nx + c.x);
 }
 
+// Candidates one walk visits at most. A grid build fails with
+// REYN_ERR_CAPACITY when a cell holds more than walk_cell_cap(range) points,
+// so a walk over a grid that built without error (at most (2 range + 1)^3
+// cells of at most that many points) never reaches the budget; the clip in
+// `strips` only keeps walks over a hand-made grid bounded.
 constexpr uint32_t kWalkBudget = 1u << 15;
+__host__ __device__ constexpr uint32_t walk_cell_cap(int range)
+{
+    return kWalkBudget / uint32_t((2 * range + 1) * (2 * range + 1) * (2 * range + 1));
+}
 
 // Calls r(s, e) for the candidate range [s, e) (cell order) of each of the
 // (2R+1)^2 x-strips of 2R+1 cells around the cell of xi, clipped to the grid,
@@ -204,8 +213,8 @@ template <class Rg> __device__ __forceinline__ void strips(const GridDev& g, f3 
     const int3c c = cell_of(g, xi);
     const int R = g.range;
     const int x0 = max(c.x - R, 0), x1 = min(c.x + R, g.nx - 1);
-    // candidate budget: only degenerate inputs (thousands of points per cell,
-    // already reported by the build) reach it; it keeps every walk bounded
+    // candidate budget: unreachable over a grid that built without error
+    // (walk_cell_cap); it keeps a walk over any other grid bounded
     uint32_t budget = kWalkBudget;
     auto emit = [&](uint32_t s, uint32_t e) {
         if (e - s > budget)

@@ -76,3 +76,27 @@ def test_batch_rejects_empty_and_bad_scenes(ctx):
     b.step(2)
     assert b.info(0).steps == 2
     b.close()
+
+
+@pytest.mark.parametrize("ratio", [1.0, 0.5])
+def test_cell_above_the_walk_capacity_is_rejected(ctx, ratio):
+    """A cell holding more points than a neighbour walk may visit within its
+    budget fails the build with REYN_ERR_CAPACITY; at the limit it builds."""
+    import ctypes as C
+
+    r = 0.1
+    builder = UniformGridBuilder(ctx, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), r, ratio)
+    rng = int(builder.geometry().range)
+    cap = C.c_uint32()
+    assert capi.lib().reyn_grid_cell_capacity(rng, C.byref(cap)) == capi.OK
+    cap = int(cap.value)
+    # points packed into one small cube inside one cell
+    pts = np.random.default_rng(3).uniform(0.5, 0.5 + 0.01 * r, size=(cap + 1, 3)).astype(np.float32)
+    print(f"ratio {ratio}: range {rng}, cell capacity {cap}", flush=True)
+    at_cap = Particles(ctx, pts[:cap], None, 1.0, 1000.0)
+    builder.construct(r, at_cap.xx, at_cap.xy, at_cap.xz)
+    over = Particles(ctx, pts, None, 1.0, 1000.0)
+    with pytest.raises(capi.ReynError) as e:
+        builder.construct(r, over.xx, over.xy, over.xz)
+    assert e.value.code == capi.ERR_CAPACITY
+    builder.close()

@@ -414,3 +414,27 @@ def test_bounds_hand_cases():
                                                np.float32))
     c = np.float32(-0.1)
     np.testing.assert_array_equal(nv, [[1, 1, 1], [c, c, c], [c, c, c], [c, c, c]])
+
+
+def test_walk_capacity_rule_library_and_oracle():
+    """The most points per cell a grid build accepts: (2^15 candidates) /
+    (2 range + 1)^3, the same in the library (host-only call) and the oracle,
+    whose build rejects one point more."""
+    import ctypes as C
+
+    from reyn_amd import capi
+
+    for rng in (1, 2, 3, 4):
+        cap = C.c_uint32()
+        assert capi.lib().reyn_grid_cell_capacity(rng, C.byref(cap)) == capi.OK
+        w = (2 * rng + 1) ** 3
+        assert cap.value == orc.grid_cell_capacity(rng) == (1 << 15) // w
+        assert cap.value * w <= 1 << 15 < (cap.value + 1) * w
+    assert capi.lib().reyn_grid_cell_capacity(0, C.byref(cap)) == capi.ERR_INVALID_ARGUMENT
+    r = np.float32(0.1)
+    geom = orc.geometry((0, 0, 0), (1, 1, 1), r, 1.0)
+    cap = orc.grid_cell_capacity(geom.range)
+    pts = np.random.default_rng(2).uniform(0.5, 0.501, size=(cap + 1, 3)).astype(np.float32)
+    orc.grid_build(geom, pts[:cap, 0], pts[:cap, 1], pts[:cap, 2])
+    with pytest.raises(ValueError):
+        orc.grid_build(geom, pts[:, 0], pts[:, 1], pts[:, 2])
