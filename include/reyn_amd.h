@@ -285,6 +285,20 @@ int reyn_sesph_step(reyn_ctx* ctx, const reyn_sesph_params* prm,
     const reyn_particles* fluid, const reyn_grid* grid,
     const reyn_boundary* bdy, const reyn_sesph_work* work, float dt);
 
+/* reyn_sesph_step that allocates nothing. No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). The same
+ * arguments and step as reyn_sesph_step, with results equal to FP32 rounding
+ * (every pass walks the grid instead of reading a neighbour list). Its scratch
+ * is a static device array of 17 MiB that every GPU the library runs on holds
+ * until the process ends, for at most 2^20 fluid particles (more:
+ * REYN_ERR_CAPACITY). Calls that overlap on one GPU share that array and race;
+ * nothing in it needs to start at zero. Per-launch profiling
+ * (reyn_profile_enable) creates events, so it must be off. */
+int reyn_sesph_step_noalloc(reyn_ctx* ctx, const reyn_sesph_params* prm,
+    const reyn_particles* fluid, const reyn_grid* grid,
+    const reyn_boundary* bdy, const reyn_sesph_work* work, float dt);
+
 /* ---- PCISPH (src/solvers/PCISPH.cuh:8-111, PCISPH.cu:15-118) ------------ */
 
 /* `k_c_pcisph` (PCISPH.cuh:49-77): delta = 1 / (2 V0^2 (|sum grad W|^2 +

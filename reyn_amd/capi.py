@@ -208,6 +208,7 @@ _SIGS = {
     "reyn_integrate": (i32, [vp, P(Particles), vp, vp, vp, f32]),
     "reyn_integrate_a_v_only": (i32, [vp, P(Particles), vp, vp, vp, f32]),
     "reyn_sesph_step": (i32, [vp, P(SesphParams), P(Particles), P(Grid), P(Boundary), P(SesphWork), f32]),
+    "reyn_sesph_step_noalloc": (i32, [vp, P(SesphParams), P(Particles), P(Grid), P(Boundary), P(SesphWork), f32]),
     "reyn_pcisph_delta": (i32, [i32, f32, f32, P(f64)]),
     "reyn_pcisph_step": (i32, [vp, P(PcisphParams), P(Particles), P(Grid), P(Boundary), P(PcisphWork), f32, P(SolveInfo)]),
     "reyn_iisph_step": (i32, [vp, P(IisphParams), P(Particles), P(Grid), P(Boundary), P(IisphWork), f32, P(SolveInfo)]),

@@ -13,6 +13,7 @@
 // in chunks and synchronises once per chunk instead of once per iteration.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 
 #include <vector>
 
@@ -1258,6 +1259,39 @@ int batch_phase(reyn_ctx* ctx, const BatchTarget& bt, const BatchSolver& bs, int
     }
 }
 
+// ---- static scratch of the entry point that allocates nothing --------------
+// reyn_sesph_step_noalloc keeps its scratch (the boundary sums of the density
+// pass, 16 bytes per fluid particle, and the per-workgroup partials and loop
+// control) in this file-scope array instead of the context's grown scratch.
+// Every GPU the library runs on holds its own copy of the array until the
+// process ends. In host code `scratch` only names the symbol: its device
+// address on a device comes from hipGetSymbolAddress with that device
+// current, taken once per device and kept in `noalloc_ptr`.
+constexpr uint32_t kNoAllocMaxFluid = 1u << 20;
+constexpr size_t kNoAllocScratchBytes = 17u << 20; // 16 MiB of sums + partials and control
+constexpr int kNoAllocDevices = 64;
+static __device__ __align__(256) char scratch[kNoAllocScratchBytes];
+
+namespace {
+std::mutex noalloc_mu;
+char* noalloc_ptr[kNoAllocDevices] = {};
+} // namespace
+
+static int noalloc_scratch(reyn_ctx* ctx, char** out)
+{
+    if (ctx->device < 0 || ctx->device >= kNoAllocDevices)
+        return set_error(ctx, REYN_ERR_INVALID_ARGUMENT, "no-alloc scratch: device %d out of range", ctx->device);
+    std::lock_guard<std::mutex> lock(noalloc_mu);
+    if (!noalloc_ptr[ctx->device]) {
+        REYN_HIP(ctx, hipSetDevice(ctx->device));
+        void* p = nullptr;
+        REYN_HIP(ctx, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        noalloc_ptr[ctx->device] = static_cast<char*>(p);
+    }
+    *out = noalloc_ptr[ctx->device];
+    return REYN_OK;
+}
+
 } // namespace reyn
 
 using namespace reyn;
@@ -1359,6 +1393,33 @@ int reyn_sesph_step(reyn_ctx* ctx, const reyn_sesph_params* prm, const reyn_part
     c.a.one.dt = dt;
     Scratch s;
     REYN_TRY(get_scratch(ctx, fluid->n, &s, true, 0, 1, c.a.one.b.present ? c.a.one.b.n : 0));
+    return sesph_run<false>(ctx, prm->sph.kernel, c, prm->k, work, s);
+}
+
+int reyn_sesph_step_noalloc(reyn_ctx* ctx, const reyn_sesph_params* prm, const reyn_particles* fluid,
+    const reyn_grid* grid, const reyn_boundary* bdy, const reyn_sesph_work* work, float dt)
+{
+    if (!prm || !work)
+        return set_error(ctx, REYN_ERR_INVALID_ARGUMENT, "sesph (no alloc): NULL parameters or work buffers");
+    Common c;
+    REYN_TRY(prepare(ctx, &prm->sph, fluid, grid, bdy, &c));
+    if (fluid->n > kNoAllocMaxFluid)
+        return set_error(ctx, REYN_ERR_CAPACITY,
+            "sesph (no alloc): %u fluid particles exceed the %u its static scratch holds", fluid->n,
+            kNoAllocMaxFluid);
+    if (fluid->n == 0)
+        return REYN_OK;
+    if (!work->rho || !work->ax || !work->ay || !work->az)
+        return set_error(ctx, REYN_ERR_INVALID_ARGUMENT, "sesph (no alloc): NULL work buffer");
+    c.a.one.dt = dt;
+    char* p = nullptr;
+    REYN_TRY(noalloc_scratch(ctx, &p));
+    // the walk form: boundary sums and loop control only (no list, no packed
+    // records), within the static scratch
+    Scratch s;
+    if (carve_scratch(nullptr, fluid->n, &s, false, 0, 1, 0) > kNoAllocScratchBytes)
+        return set_error(ctx, REYN_ERR_CAPACITY, "sesph (no alloc): scratch layout exceeds the static scratch");
+    carve_scratch(p, fluid->n, &s, false, 0, 1, 0);
     return sesph_run<false>(ctx, prm->sph.kernel, c, prm->k, work, s);
 }
 
