@@ -4,7 +4,10 @@
  * Every entry point takes plain device pointers and sizes, enqueues its work
  * on the stream owned by a `reyn_ctx`, returns an `int` status (0 = ok,
  * negative = error) and never throws or exits. `reyn_last_error(ctx)` holds a
- * message for the last failing call on that context. Caller memory is never
+ * message for the last failing call on that context; after a successful
+ * solver step it may hold a note that the step walked the grid in every pass
+ * because the input exceeded the 32-bit offsets of the neighbour list or the
+ * packed records. Caller memory is never
  * freed by the library; buffers the library allocates itself (grid-builder
  * tables, solver scratch) belong to the handle that allocated them.
  *

@@ -432,8 +432,9 @@ int relayout(reyn_batch* b, const std::vector<SceneRes*>& next, const std::vecto
         nblk += blocks_for(r->n);
         cells += r->ncells;
     }
-    if (N >= (uint64_t(1) << 24))
-        return set_error(ctx, REYN_ERR_CAPACITY, "batch: at most 2^24 - 1 fluid particles in total (got %llu)",
+    // the batch's neighbour list addresses 16-bit entries with 32-bit byte offsets
+    if (N >= (uint64_t(1) << 25))
+        return set_error(ctx, REYN_ERR_CAPACITY, "batch: at most 2^25 - 1 fluid particles in total (got %llu)",
             static_cast<unsigned long long>(N));
     if (cells >= (uint64_t(1) << 32) - 1)
         return set_error(ctx, REYN_ERR_CAPACITY, "batch: the scenes' grids exceed 2^32 - 1 cells");

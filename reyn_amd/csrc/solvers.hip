@@ -83,9 +83,10 @@ size_t carve_scratch(char* p, uint32_t n, Scratch* s, bool lists, uint32_t nbloc
     const size_t head = 4 * f + al(nb * 8) + al(nb * 4) + al(sizeof(SolveCtl) * nscenes);
     // records and list entries are addressed with 32-bit byte offsets
     const bool records = lists && size_t(n) * 32 < (size_t(1) << 32);
-    const bool keep_list = records && size_t(n) * kListCap * 4 < (size_t(1) << 32);
+    // 16-bit entries: rows of word pairs addressed as 32-bit words
+    const bool keep_list = records && size_t(n) * kListCap * 2 < (size_t(1) << 32);
     const size_t rec_bytes = records ? al(size_t(n) * 16) + al(size_t(n) * 32) : 0;
-    const size_t list_bytes = keep_list ? f + al(size_t(n) * kListCap * 4) : 0;
+    const size_t list_bytes = keep_list ? f + al(size_t(n) * kListCap * 2) : 0;
     const size_t bdy_bytes = (records && nbdy) ? al(size_t(nbdy) * 16) : 0;
     const size_t total = head + rec_bytes + list_bytes + bdy_bytes;
     if (!p)
@@ -122,6 +123,14 @@ int get_scratch(reyn_ctx* ctx, uint32_t n, Scratch* s, bool lists = false, uint3
     char* p = nullptr;
     REYN_TRY(ctx_scratch(ctx, carve_scratch(nullptr, n, s, lists, nblocks, nscenes, nbdy), reinterpret_cast<void**>(&p)));
     carve_scratch(p, n, s, lists, nblocks, nscenes, nbdy);
+    // the call still succeeds, through the walk form in every pass; the note
+    // tells a caller why it is slower
+    if (lists && !s->P4)
+        set_error(ctx, REYN_OK, "note: %u particles exceed the packed records' 32-bit offsets (2^27 - 1 at most); "
+            "every pass walks the grid over the SoA arrays", n);
+    else if (lists && !s->L.cnt)
+        set_error(ctx, REYN_OK, "note: %u particles exceed the neighbour list's 32-bit offsets (2^25 - 1 at most); "
+            "every pass after the density pass walks the grid", n);
     return REYN_OK;
 }
 
@@ -205,7 +214,10 @@ __global__ __launch_bounds__(kBlock) REYN_OCC(REYN_OCC_DENSITY) void k_density(S
     const GridDev& g = sc.g;
     const BdyDev& b = sc.b;
     const KParams& kp = sc.c.kp;
-    const bool listing = L.cnt && P4;
+    // lists serve ranges up to kListMaxRange; beyond, every later loop walks
+    const bool listing = L.cnt && P4 && g.range <= kListMaxRange;
+    if (L.cnt && P4 && !listing && live)
+        L.cnt[t] = L.cap + 1;
     if (!live && !listing)
         return;
     // lanes past the end stay for the list staging below, whose row flushes
@@ -231,28 +243,36 @@ __global__ __launch_bounds__(kBlock) REYN_OCC(REYN_OCC_DENSITY) void k_density(S
         __shared__ uint32_t ring[kBlock / kWave][kRing][kWave];
         const uint32_t lane = threadIdx.x % kWave;
         uint32_t cnt = 0, flushed = 0, direct = L.cap; // `flushed` is uniform
+        bool fits = true; // every entry's offset fits its 13 bits
         // entries below `lim` = min(flushed + kRing, direct, cap) go to the
         // ring: one test on the common path
         uint32_t lim = min(kRing, L.cap);
-        auto flush = [&](uint32_t end) { // rows [flushed, end) from the ring
+        auto flush = [&](uint32_t end) { // rows [flushed, end) from the ring (`flushed` is even)
             const uint32_t have = min(cnt, direct);
-            for (uint32_t r = flushed; r < end; ++r)
-                if (r < have)
-                    L.idx[t + r * L.n] = ring[wv][r % kRing][lane];
+            for (uint32_t r = flushed; r < end; r += 2) {
+                // both entries of a word at once where this lane has both
+                if (r + 1 < have && r + 1 < end)
+                    put_pair(L, t, r, ring[wv][r % kRing][lane] | (ring[wv][(r + 1) % kRing][lane] << 16));
+                else if (r < have)
+                    put_entry(L, t, r, ring[wv][r % kRing][lane]);
+            }
             flushed = end;
             lim = min(min(flushed + kRing, direct), L.cap);
         };
-        auto record = [&](uint32_t j, f3, float, float) {
+        auto record = [&](uint32_t j, f3, float, float, const Cand& cd) {
             if (!live || j == i)
                 return;
+            bool ok;
+            const uint32_t e = encode_entry(cd, &ok);
+            fits = fits && ok;
             if (cnt < lim) {
-                ring[wv][cnt % kRing][lane] = j;
+                ring[wv][cnt % kRing][lane] = e;
             } else if (cnt < L.cap) {
                 // the ring is full (or this lane already stores directly):
                 // from here on the entries go straight to the list
                 direct = min(direct, cnt);
                 lim = 0;
-                L.idx[t + cnt * L.n] = j;
+                put_entry(L, t, cnt, e);
             }
             ++cnt;
         };
@@ -271,8 +291,10 @@ __global__ __launch_bounds__(kBlock) REYN_OCC(REYN_OCC_DENSITY) void k_density(S
         flush(wave_max_u32(live ? min(cnt, direct) : 0u));
         if (!live)
             return;
-        L.cnt[t] = cnt;
-        if (cnt <= L.cap) {
+        // a particle whose offsets do not all fit is not listed
+        const uint32_t stored = fits ? cnt : L.cap + 1;
+        L.cnt[t] = stored;
+        if (stored <= L.cap) {
             acc.add(gat(f.m, i) * kW<K>(kp, 0.f)); // the particle itself
             nbrs4<IND, kNbrBatch<K>>(g, L, t, xi, P4, f.x, f.y, f.z, f.m,
                 [&](uint32_t, f3, float d2, float mj) { acc.add(mj * kW<K>(kp, d2)); });
@@ -281,7 +303,7 @@ __global__ __launch_bounds__(kBlock) REYN_OCC(REYN_OCC_DENSITY) void k_density(S
     }
     if (!summed) {
         if (P4)
-            walk4<IND>(g, xi, P4, [&](uint32_t, f3, float d2, float mj) { acc.add(mj * kW<K>(kp, d2)); });
+            walk4<IND>(g, xi, P4, [&](uint32_t, f3, float d2, float mj, const Cand&) { acc.add(mj * kW<K>(kp, d2)); });
         else
             walk<IND>(g, xi, f.x, f.y, f.z, [&](uint32_t j, f3, float d2) { acc.add(gat(f.m, j) * kW<K>(kp, d2)); });
     }
@@ -297,7 +319,8 @@ __global__ __launch_bounds__(kBlock) REYN_OCC(REYN_OCC_DENSITY) void k_density(S
                 grad.add(mk * kdW<K>(kp, xik, d2));
         };
         if (b.p4)
-            walk4<false>(b.grid, xi, b.p4, [&](uint32_t, f3 xik, float d2, float mk) { visit(xik, d2, mk); });
+            walk4<false>(b.grid, xi, b.p4,
+                [&](uint32_t, f3 xik, float d2, float mk, const Cand&) { visit(xik, d2, mk); });
         else
             walk<false>(b.grid, xi, b.x, b.y, b.z, [&](uint32_t k, f3 xik, float d2) { visit(xik, d2, b.m[k]); });
     }
@@ -487,34 +510,33 @@ __global__ __launch_bounds__(kBlock) void k_pcisph_prs_acc(SceneArg sa, FluidDev
         acc += (mj * (pri + pj / (rhoj * rhoj))) * kdW<K>(kp, xij, d2);
     };
     const uint32_t c = (PK && L.cnt) ? L.cnt[t] : 0xffffffffu;
+    bool listed = false;
     if (PK && c <= L.cap) {
         const bool uniform = !ctl[sc.idx].mass_mixed; // the same for the whole scene
         const float mi = f.m[i];
         constexpr uint32_t B = kNbrBatch<K>;
-        uint32_t off = t;
-        for (uint32_t kk = 0; kk < c; kk += B, off += B * L.n) {
+        listed = with_list<B>(g, L, t, c, xi, [&](uint32_t kk, const uint32_t (&q)[B]) {
             uint32_t j[B];
-            float4 q[B];
+            float4 r[B];
             float m[B];
 #pragma unroll
-            for (uint32_t u = 0; u < B; ++u) // past the count: row kk again (valid)
-                j[u] = gat(L.idx, (kk + u < c) ? off + u * L.n : off);
+            for (uint32_t u = 0; u < B; ++u)
+                j[u] = IND ? gat(g.sorted, q[u]) : q[u];
 #pragma unroll
             for (uint32_t u = 0; u < B; ++u)
-                q[u] = gat(P4, j[u]);
+                r[u] = gat(P4, j[u]);
 #pragma unroll
             for (uint32_t u = 0; u < B; ++u)
                 m[u] = uniform ? mi : gat(f.m, j[u]);
 #pragma unroll
             for (uint32_t u = 0; u < B; ++u) {
-                const f3 xij = xi - f3 { q[u].x, q[u].y, q[u].z };
-                term(xij, dot(xij, xij), (kk + u < c) ? m[u] : 0.f, q[u].w);
+                const f3 xij = xi - f3 { r[u].x, r[u].y, r[u].z };
+                term(xij, dot(xij, xij), (kk + u < c) ? m[u] : 0.f, r[u].w);
             }
-            asm volatile("" ::: "memory"); // rounds stay in source order, as in nbrs4
-        }
-    } else {
-        walk<IND>(g, xi, f.x, f.y, f.z, [&](uint32_t j, f3 xij, float d2) { term(xij, d2, gat(f.m, j), gat(rho, j)); });
+        });
     }
+    if (!listed)
+        walk<IND>(g, xi, f.x, f.y, f.z, [&](uint32_t j, f3 xij, float d2) { term(xij, d2, gat(f.m, j), gat(rho, j)); });
     f3 a = -acc;
     if (bdy)
         a += -(0.5f * (pri + pr0)) * ld3(gbx, gby, gbz, i);
@@ -629,37 +651,36 @@ __global__ __launch_bounds__(kBlock) REYN_OCC(REYN_OCC_PACC) void k_iisph_pacc(S
     const float pi = p[i];
     f3 acc { 0.f, 0.f, 0.f };
     const uint32_t c = (PK && L.cnt) ? L.cnt[t] : 0xffffffffu;
+    bool listed = false;
     if (PK && c <= L.cap) {
         const bool uniform = !mctl[sc.idx].mass_mixed; // the same for the whole scene
         const float mi = f.m[i];
         constexpr uint32_t B = kNbrBatch<K>;
-        uint32_t off = t;
-        for (uint32_t k = 0; k < c; k += B, off += B * L.n) {
+        listed = with_list<B>(g, L, t, c, xi, [&](uint32_t k, const uint32_t (&q)[B]) {
             uint32_t j[B];
-            float4 q[B];
+            float4 r[B];
             float m[B];
 #pragma unroll
-            for (uint32_t u = 0; u < B; ++u) // past the count: row k again (valid)
-                j[u] = gat(L.idx, (k + u < c) ? off + u * L.n : off);
+            for (uint32_t u = 0; u < B; ++u)
+                j[u] = IND ? gat(g.sorted, q[u]) : q[u];
 #pragma unroll
             for (uint32_t u = 0; u < B; ++u)
-                q[u] = gat(P4, j[u]);
+                r[u] = gat(P4, j[u]);
 #pragma unroll
             for (uint32_t u = 0; u < B; ++u)
                 m[u] = uniform ? mi : gat(f.m, j[u]);
 #pragma unroll
             for (uint32_t u = 0; u < B; ++u) {
-                const f3 xij = xi - f3 { q[u].x, q[u].y, q[u].z };
+                const f3 xij = xi - f3 { r[u].x, r[u].y, r[u].z };
                 const float mj = (k + u < c) ? m[u] : 0.f;
-                acc += (mj * (pi + q[u].w)) * kdW<K>(kp, xij, dot(xij, xij));
+                acc += (mj * (pi + r[u].w)) * kdW<K>(kp, xij, dot(xij, xij));
             }
-            asm volatile("" ::: "memory"); // rounds stay in source order, as in nbrs4
-        }
-    } else {
+        });
+    }
+    if (!listed)
         walk<IND>(g, xi, f.x, f.y, f.z, [&](uint32_t j, f3 xij, float d2) {
             acc += (gat(f.m, j) * (pi + gat(p, j))) * kdW<K>(kp, xij, d2);
         });
-    }
     if (bdy)
         acc += pi * ld3(gbx, gby, gbz, i);
     const f3 a = -rho0_sq_inv * acc;

@@ -199,14 +199,17 @@ __host__ __device__ constexpr uint32_t walk_cell_cap(int range)
     return kWalkBudget / uint32_t((2 * range + 1) * (2 * range + 1) * (2 * range + 1));
 }
 
-// Calls r(s, e) for the candidate range [s, e) (cell order) of each of the
-// (2R+1)^2 x-strips of 2R+1 cells around the cell of xi, clipped to the grid,
-// in the order z, then y ascending. Every lane makes all (2R+1)^2 calls, with
-// an empty range for a strip outside the grid, so a call is a point the whole
-// wavefront reaches together. Clipping leaves every point whose cell lies R
-// or more cells inside the grid unchanged. For R <= 2 the bounds of
-// the strips of one z-plane are loaded together (from clamped, valid rows),
-// so a walk waits for 2R+1 round trips of cell-start loads instead of
+// Calls r(s, e, plane, base) for the candidate range [s, e) (cell order) of
+// each of the (2R+1)^2 x-strips of 2R+1 cells around the cell of xi, clipped to
+// the grid, in the order z, then y ascending; `plane` is the strip's z-plane
+// among the 2R+1 (0 for z - R) and, for R <= 2, `base` the plane's first
+// candidate position, prefix[clamped z, max(y - R, 0), max(x - R, 0)] (no
+// candidate of the plane lies below it; 0 for larger R). Every lane makes all
+// (2R+1)^2 calls, with an empty range for a strip outside the grid, so a call
+// is a point the whole wavefront reaches together. Clipping leaves every point
+// whose cell lies R or more cells inside the grid unchanged. For R <= 2 the
+// bounds of the strips of one z-plane are loaded together (from clamped, valid
+// rows), so a walk waits for 2R+1 round trips of cell-start loads instead of
 // (2R+1)^2.
 template <class Rg> __device__ __forceinline__ void strips(const GridDev& g, f3 xi, Rg&& r)
 {
@@ -216,11 +219,11 @@ template <class Rg> __device__ __forceinline__ void strips(const GridDev& g, f3 
     // candidate budget: unreachable over a grid that built without error
     // (walk_cell_cap); it keeps a walk over any other grid bounded
     uint32_t budget = kWalkBudget;
-    auto emit = [&](uint32_t s, uint32_t e) {
+    auto emit = [&](uint32_t s, uint32_t e, uint32_t plane, uint32_t base) {
         if (e - s > budget)
             e = s + budget;
         budget -= e - s;
-        r(s, e);
+        r(s, e, plane, base);
     };
     if (R <= 2) {
         for (int dz = -R; dz <= R; ++dz) {
@@ -241,7 +244,7 @@ template <class Rg> __device__ __forceinline__ void strips(const GridDev& g, f3 
                 if (u <= 2 * R) {
                     const int y = c.y - R + u;
                     const bool in = z >= 0 && z < g.nz && y >= 0 && y < g.ny;
-                    emit(in ? s[u] : 0u, in ? e[u] : 0u);
+                    emit(in ? s[u] : 0u, in ? e[u] : 0u, uint32_t(dz + R), s[0]);
                 }
             }
         }
@@ -256,7 +259,7 @@ template <class Rg> __device__ __forceinline__ void strips(const GridDev& g, f3 
                 s = gat(g.prefix, base + x0);
                 e = gat(g.prefix, base + x1 + 1);
             }
-            emit(s, e);
+            emit(s, e, uint32_t(dz + R), 0u);
         }
     }
 }
@@ -273,7 +276,7 @@ template <bool IND, class F>
 __device__ __forceinline__ void walk(const GridDev& g, f3 xi, const float* __restrict__ xs,
     const float* __restrict__ ys, const float* __restrict__ zs, F&& f)
 {
-    strips(g, xi, [&](uint32_t s, uint32_t e) {
+    strips(g, xi, [&](uint32_t s, uint32_t e, uint32_t, uint32_t) {
         constexpr uint32_t B = kWalkBatch;
         for (uint32_t q = s; q < e; q += B) {
             uint32_t j[B];
@@ -299,14 +302,20 @@ __device__ __forceinline__ void walk(const GridDev& g, f3 xi, const float* __res
     });
 }
 
-// The same walk over packed records P[j] = {x, y, z, m}; f(j, x_ij, d2, m_j).
-// `strip_end()` runs after each strip, at a point every lane of the
+// Where a candidate sits: its position q in cell order, the z-plane of its
+// strip and that plane's base (see `strips`).
+struct Cand {
+    uint32_t q, plane, base;
+};
+
+// The same walk over packed records P[j] = {x, y, z, m}; f(j, x_ij, d2, m_j,
+// cand). `strip_end()` runs after each strip, at a point every lane of the
 // wavefront that entered the walk reaches together.
 template <bool IND, class F, class E>
 __device__ __forceinline__ void walk4(const GridDev& g, f3 xi, const float4* __restrict__ P, F&& f,
     E&& strip_end)
 {
-    strips(g, xi, [&](uint32_t s, uint32_t e) {
+    strips(g, xi, [&](uint32_t s, uint32_t e, uint32_t plane, uint32_t base) {
         constexpr uint32_t B = kWalkBatch;
         for (uint32_t q = s; q < e; q += B) {
             uint32_t j[B];
@@ -325,7 +334,7 @@ __device__ __forceinline__ void walk4(const GridDev& g, f3 xi, const float4* __r
                     const f3 xij = xi - f3 { pj[u].x, pj[u].y, pj[u].z };
                     const float d2 = dot(xij, xij);
                     if (d2 <= g.r2)
-                        f(j[u], xij, d2, pj[u].w);
+                        f(j[u], xij, d2, pj[u].w, Cand { q + u, plane, base });
                 }
             }
         }
@@ -456,11 +465,11 @@ template <uint32_t CAP, int WIN, class F, class E>
 __device__ __forceinline__ void walk4_staged(const GridDev& g, f3 xi, const float4* __restrict__ P,
     float4* __restrict__ st, bool active, F&& f, E&& strip_end)
 {
-    strips(g, xi, [&](uint32_t s, uint32_t e) {
+    strips(g, xi, [&](uint32_t s, uint32_t e, uint32_t plane, uint32_t base) {
         if (!active)
             e = s;
         const StageWin w = stage_strip<CAP, WIN>(s, e, st, [&](uint32_t q) { return P[q]; });
-        auto body = [&](uint32_t q, f3 xij, float d2, float4 p) { f(q, xij, d2, p.w); };
+        auto body = [&](uint32_t q, f3 xij, float d2, float4 p) { f(q, xij, d2, p.w, Cand { q, plane, base }); };
         if (w.staged) {
             strip_loop_lds(g, xi, s, e, st + (s - w.base), body);
             wave_lds_sync(); // every lane has read before the next strip is staged
@@ -476,7 +485,7 @@ template <uint32_t CAP, int WIN, class F>
 __device__ __forceinline__ void walk_staged(const GridDev& g, f3 xi, const float* __restrict__ xs,
     const float* __restrict__ ys, const float* __restrict__ zs, float4* __restrict__ st, bool active, F&& f)
 {
-    strips(g, xi, [&](uint32_t s, uint32_t e) {
+    strips(g, xi, [&](uint32_t s, uint32_t e, uint32_t, uint32_t) {
         if (!active)
             e = s;
         const StageWin w
@@ -498,16 +507,124 @@ __device__ __forceinline__ void walk_staged(const GridDev& g, f3 xi, const float
 
 // ---- per-step neighbour lists ------------------------------------------------
 // The density pass of a solver step walks the grid once and records, for the
-// particle in cell-order slot t, its fluid neighbours j != self in walk order:
-// idx[k * n + t] for k < cnt[t] (column-major, so a wavefront reads each k as
-// one contiguous row). A particle with more than `cap` neighbours keeps the
-// true count and is walked again instead.
+// particle in cell-order slot t, its fluid neighbours j != self in walk order,
+// at most `cap` of them. An entry takes 16 bits: the z-plane u of the
+// neighbour's strip among the walk's 2R+1 planes (3 bits) and the neighbour's
+// cell-order position q minus the plane's base (13 bits; `strips`). A loop
+// that reads the list recomputes the bases from the particle's cell, so an
+// entry e decodes as q = e + B_u with B_u = base_u - (u << 13) (unsigned).
+// Entries k and k+1 (k even) share the 32-bit word [(k/2) n + t]: the rows
+// are column-major, so a wavefront reads one contiguous 256-byte row per two
+// entries. A particle with more than `cap` neighbours, or with a neighbour
+// 2^13 or more candidates above its plane's base, is stored with a count above
+// `cap` and walks the grid in every later loop; lists are kept for ranges up
+// to kListMaxRange (cell ratios from 0.5 up).
+
+constexpr uint32_t kPlaneShift = 13;
+constexpr uint32_t kPlaneSpan = 1u << kPlaneShift; // offsets from a plane's base below this
+constexpr int kListMaxRange = 2;
 
 struct NbrDev {
     uint32_t* cnt; // NULL: no list, every loop walks the grid
-    uint32_t* idx;
+    uint32_t* idx; // 16-bit entries in word pairs, see above
     uint32_t n, cap;
 };
+
+// Stores entry k of slot t.
+__device__ __forceinline__ void put_entry(const NbrDev& L, uint32_t t, uint32_t k, uint32_t e)
+{
+    reinterpret_cast<uint16_t*>(L.idx)[(size_t(k >> 1) * L.n + t) * 2 + (k & 1)] = static_cast<uint16_t>(e);
+}
+
+// Stores entries k and k+1 (k even) of slot t: e01 = e_k | e_{k+1} << 16.
+__device__ __forceinline__ void put_pair(const NbrDev& L, uint32_t t, uint32_t k, uint32_t e01)
+{
+    L.idx[size_t(k >> 1) * L.n + t] = e01;
+}
+
+// The entry of a candidate of the walk, and whether its offset fits.
+__device__ __forceinline__ uint32_t encode_entry(const Cand& c, bool* fits)
+{
+    const uint32_t off = c.q - c.base;
+    *fits = off < kPlaneSpan;
+    return off + (c.plane << kPlaneShift);
+}
+
+// The decode bases B_u = base_u - (u << kPlaneShift) of the particle at xi
+// for its NP = 2R+1 planes, from the same cell-start entries the walk loads.
+template <int NP> __device__ __forceinline__ void plane_bases(const GridDev& g, f3 xi, uint32_t (&B)[NP])
+{
+    constexpr int R = (NP - 1) / 2;
+    const int3c c = cell_of(g, xi);
+    const int x0 = max(c.x - R, 0);
+    const int yc = min(max(c.y - R, 0), g.ny - 1);
+#pragma unroll
+    for (int u = 0; u < NP; ++u) {
+        const int zc = min(max(c.z - R + u, 0), g.nz - 1);
+        const uint32_t row = static_cast<uint32_t>(zc * g.nxny + yc * g.nx);
+        B[u] = gat(g.prefix, row + x0) - (uint32_t(u) << kPlaneShift);
+    }
+}
+
+// q of entry e: e + B_u for the plane u in e's top bits.
+template <int NP> __device__ __forceinline__ uint32_t decode_entry(const uint32_t (&B)[NP], uint32_t e)
+{
+    uint32_t b = B[0];
+#pragma unroll
+    for (int u = 1; u < NP; ++u)
+        b = (e >= (uint32_t(u) << kPlaneShift)) ? B[u] : b;
+    return e + b;
+}
+
+// Reads the list of slot t (count c <= cap, particle at xi) in rounds of B
+// entries (B even) and calls body(k, q) with the round's first entry k and the
+// cell-order positions q[B]. Rounds run past the count with entry k again
+// (always written), so every q is valid; entry k + u is real when k + u < c.
+// The index words, the decodes and whatever the body gathers are all in
+// flight together within a round.
+template <int NP, uint32_t B, class Body>
+__device__ __forceinline__ void list_rounds(const GridDev& g, const NbrDev& L, uint32_t t, uint32_t c, f3 xi,
+    Body&& body)
+{
+    static_assert(B % 2 == 0, "list rounds read entries in word pairs");
+    uint32_t bases[NP];
+    plane_bases<NP>(g, xi, bases);
+    uint32_t off = t;
+    for (uint32_t k = 0; k < c; k += B, off += (B / 2) * L.n) {
+        uint32_t w[B / 2];
+#pragma unroll
+        for (uint32_t v = 0; v < B / 2; ++v) // past the count: the row of entry k (valid)
+            w[v] = gat(L.idx, (k + 2 * v < c) ? off + v * L.n : off);
+        uint32_t q[B];
+#pragma unroll
+        for (uint32_t u = 0; u < B; ++u) {
+            const uint32_t e = (k + u < c) ? ((u & 1) ? (w[u / 2] >> 16) : (w[u / 2] & 0xffffu)) : (w[0] & 0xffffu);
+            q[u] = decode_entry<NP>(bases, e);
+        }
+        body(k, q);
+        // Keeps the next round's index loads below this round's bodies. Left
+        // free, the compiler may issue them right after this round's gathers
+        // (a software-pipelined loop), which measured 10-15% slower for the
+        // two-record loops (a_ii/source and Jacobi passes) than issuing each
+        // gather as soon as its own index returns.
+        asm volatile("" ::: "memory");
+    }
+}
+
+// list_rounds for the grid's range (3 or 5 planes); false if the list cannot
+// serve this grid.
+template <uint32_t B, class Body>
+__device__ __forceinline__ bool with_list(const GridDev& g, const NbrDev& L, uint32_t t, uint32_t c, f3 xi,
+    Body&& body)
+{
+    if (g.range == 1)
+        list_rounds<3, B>(g, L, t, c, xi, body);
+    else if (g.range == 2)
+        list_rounds<5, B>(g, L, t, c, xi, body);
+    else
+        return false;
+    return true;
+}
 
 // Calls f(j, x_ij, d2, m_j) for the neighbours of the particle in slot t at
 // xi, reading the packed record {x, y, z, m} of j at P[S j]: one 16-byte load
@@ -517,12 +634,10 @@ struct NbrDev {
 // the SoA arrays, which hold the same values, and every list entry gives the
 // same x_ij and d2 bits as the walk.
 //
-// The list is read B entries per round with no branch inside a round: the
-// index loads, the record loads and whatever the bodies gather for j are all
-// in flight together. Rounds are padded past the count with the round's
-// first entry passed as m_j = 0, so every body must be linear in m_j (each
-// SPH pair term carries the factor m_j); a padding call then adds an exact
-// zero for finite particle states.
+// Rounds are padded past the count with the round's first entry passed as
+// m_j = 0, so every body must be linear in m_j (each SPH pair term carries
+// the factor m_j); a padding call then adds an exact zero for finite particle
+// states.
 #ifndef REYN_NBR_BATCH
 #define REYN_NBR_BATCH 8
 #endif
@@ -536,32 +651,23 @@ __device__ __forceinline__ bool nbrs4(const GridDev& g, const NbrDev& L, uint32_
 {
     if (L.cnt) {
         const uint32_t c = L.cnt[t];
-        if (c <= L.cap) {
-            uint32_t off = t;
-            for (uint32_t k = 0; k < c; k += B, off += B * L.n) {
-                uint32_t j[B];
-                float4 q[B];
+        if (c <= L.cap
+            && with_list<B>(g, L, t, c, xi, [&](uint32_t k, const uint32_t (&q)[B]) {
+                   uint32_t j[B];
+                   float4 r[B];
 #pragma unroll
-                for (uint32_t u = 0; u < B; ++u) // past the count: row k again (valid)
-                    j[u] = gat(L.idx, (k + u < c) ? off + u * L.n : off);
+                   for (uint32_t u = 0; u < B; ++u)
+                       j[u] = IND ? gat(g.sorted, q[u]) : q[u];
 #pragma unroll
-                for (uint32_t u = 0; u < B; ++u)
-                    q[u] = gat(P, S * j[u]);
+                   for (uint32_t u = 0; u < B; ++u)
+                       r[u] = gat(P, S * j[u]);
 #pragma unroll
-                for (uint32_t u = 0; u < B; ++u) {
-                    const f3 xij = xi - f3 { q[u].x, q[u].y, q[u].z };
-                    f(j[u], xij, dot(xij, xij), (k + u < c) ? q[u].w : 0.f);
-                }
-                // Keeps the next round's index loads below this round's
-                // bodies. Left free, the compiler may issue them right
-                // after this round's gathers (a software-pipelined loop),
-                // which measured 10-15% slower for the two-record loops
-                // (a_ii/source and Jacobi passes) than issuing each gather
-                // as soon as its own index returns.
-                asm volatile("" ::: "memory");
-            }
+                   for (uint32_t u = 0; u < B; ++u) {
+                       const f3 xij = xi - f3 { r[u].x, r[u].y, r[u].z };
+                       f(j[u], xij, dot(xij, xij), (k + u < c) ? r[u].w : 0.f);
+                   }
+               }))
             return true;
-        }
     }
     walk<IND>(g, xi, xs, ys, zs, [&](uint32_t j, f3 xij, float d2) { f(j, xij, d2, gat(ms, j)); });
     return false;
