@@ -339,7 +339,7 @@ def batch_leg(ctx, args, log) -> dict:
                         check_every=best["check_every"])
         sim = Simulation(ctx, solver, state, bdy, h, sc.bound_min, sc.bound_max, cfl_lambda=1.0,
                          cell_ratio=best["cell_ratio"], dt_policy="controller", dt0=0.0005, iter_target=10,
-                         change_factor=0.001)
+                         change_factor=0.001, cell_order=best["cell_order"])
         sim.step()
         sim.step()
         ctx.synchronize()
@@ -365,9 +365,9 @@ def batch_leg(ctx, args, log) -> dict:
         "sequential_plain_particle_steps_per_s": round(v_s, 1),
         "speedup": round(v_b / v_s, 3),
         "speedup_direct": round(v_d / v_s, 3),
-        "settings": f"cell ratio {best['cell_ratio']} and check interval {best['check_every']} (autotuned) for the "
-                    "batch and the sequential solves alike; the batch replays each step after the first as "
-                    "captured HIP graphs (direct launches reported beside)",
+        "settings": f"cell ratio {best['cell_ratio']}, {best['cell_order']} cell order and check interval "
+                    f"{best['check_every']} (autotuned) for the batch and the sequential solves alike; the batch "
+                    "replays each step after the first as captured HIP graphs (direct launches reported beside)",
         "batched_s": round(t_graph, 4), "batched_direct_s": round(t_direct, 4), "sequential_s": round(t_seq, 4),
         "last_iterations_range": [min(it_b), max(it_b)],
         "autotune": {"chosen": best, "trials": [tr.as_dict() for tr in trials]},
@@ -463,7 +463,11 @@ def main() -> int:
                          "(lambda 1, 10 target iterations, 0.1%% change per step); cfl: cfl_time_step(lambda)")
     ap.add_argument("--cfl-lambda", type=float, default=None)
     ap.add_argument("--cell-ratio", type=float, default=None,
-                    help="grid cell size / search radius (default: autotuned over 0.5 and 1.0)")
+                    help="grid cell size / search radius (default: autotuned over 0.5 and 1.0, with the order)")
+    ap.add_argument("--cell-order", choices=("xyz", "brick"), default=None,
+                    help="cell order of the fluid grid (default: autotuned with the ratio)")
+    ap.add_argument("--bdy-cell-ratio", type=float, default=1.0,
+                    help="cell size / search radius of the boundary grid (1.0: 9 strips per boundary walk)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run an identity gather of 2^26 floats (reads 512 MiB, writes 256 MiB), the "
                          "plain copy kernel that scales rocprofv3 FETCH_SIZE/WRITE_SIZE to bytes")
@@ -515,14 +519,14 @@ def main() -> int:
     ctx = Context(0)
     lam = args.cfl_lambda if args.cfl_lambda is not None else (1.0 if args.dt_policy == "controller" else 0.4)
 
-    def make(ratio):
+    def make(ratio, order="xyz"):
         state = Particles(ctx, sc.x, None, sc.mass, sc.rho_0)
-        bdy = BoundarySamples(ctx, sc.xb, sc.spacing, sc.rho_0)
+        bdy = BoundarySamples(ctx, sc.xb, sc.spacing, sc.rho_0, args.bdy_cell_ratio)
         solver = IISPH(ctx, "B3", sc.n, 0.001, sc.spacing, sc.rho_0, omega=0.5, eta_rho_max=0.001, min_iter=3,
                        max_iter=100, stop_rule=capi.IISPH_STOP_MEAN, warm_start=True, check_every=4)
         sim = Simulation(ctx, solver, state, bdy, sc.spacing, sc.bound_min, sc.bound_max, cfl_lambda=lam,
                          cell_ratio=ratio, extra_reorder=("p",), dt_policy=args.dt_policy, dt0=0.0005,
-                         iter_target=10, change_factor=0.001)
+                         iter_target=10, change_factor=0.001, cell_order=order)
 
         def close():
             for o in (solver, bdy, state, sim.builder):
@@ -531,12 +535,14 @@ def main() -> int:
         return sim
 
     tune = None
-    ratio = args.cell_ratio
+    ratio, order = args.cell_ratio, args.cell_order
     if ratio is None:
         from reyn_amd.tune import autotune_step
-        ratio, trials = autotune_step(ctx, make, log=log)
-        tune = {"chosen_cell_ratio": ratio, "trials_s": trials, "trial": "fresh scene, 3 steps after 2"}
-    sim = make(ratio)
+        (ratio, order), trials = autotune_step(ctx, make, orders=(order,) if order else ("xyz", "brick"), log=log)
+        tune = {"chosen_cell_ratio": ratio, "chosen_cell_order": order, "trials_s": trials,
+                "trial": "fresh scene, 3 steps after 2"}
+    order = order or "xyz"
+    sim = make(ratio, order)
     state, bdy, solver = sim.state, sim.bdy, sim.solver
     ncells = sim.builder.geometry().ncells
     log(f"setup {time.perf_counter() - t_setup:.1f} s; fluid grid {ncells} cells")
@@ -659,6 +665,8 @@ def main() -> int:
             "boundary_particles": sc.n_bdy,
             "grid_cells": ncells,
             "cell_ratio": ratio,
+            "cell_order": order,
+            "boundary_cell_ratio": args.bdy_cell_ratio,
             "cell_ratio_autotune": tune,
             "timed_steps": [args.warmup, args.warmup + args.steps - 1],
             "jacobi_iterations_mean": round(mean_iters, 2),

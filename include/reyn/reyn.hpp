@@ -344,12 +344,14 @@ concept IsFltDevBufPtr = std::is_same_v<T, DeviceBuffer<float>*>;
 
 class UniformGridBuilder {
 public:
-    // cell_ratio = cell size / search radius; the reference fixes 0.5
+    // cell_ratio = cell size / search radius; the reference fixes 0.5. `order`
+    // is REYN_CELL_ORDER_XYZ (the reference's keys) or REYN_CELL_ORDER_BRICK.
     UniformGridBuilder(float3 bound_min, float3 bound_max, float search_radius, float cell_ratio = 0.5f,
-        Context& ctx = Context::current())
+        Context& ctx = Context::current(), int32_t order = REYN_CELL_ORDER_XYZ)
         : ctx_(&ctx)
     {
-        ctx_->check(reyn_grid_builder_create(ctx.get(), bound_min, bound_max, search_radius, cell_ratio, &b_),
+        ctx_->check(reyn_grid_builder_create_ordered(ctx.get(), bound_min, bound_max, search_radius, cell_ratio,
+                        order, &b_),
             "UniformGridBuilder");
     }
     ~UniformGridBuilder() { reyn_grid_builder_destroy(b_); }

@@ -63,6 +63,17 @@ typedef struct reyn_float3 {
     float x, y, z;
 } reyn_float3;
 
+/* The order of the cell keys, and so of a reordered state.
+ * REYN_CELL_ORDER_XYZ is the reference's key z nx ny + y nx + x
+ * (src/datastructure/uniformgrid.cuh:33-40). REYN_CELL_ORDER_BRICK groups the
+ * cells in 2 x 2 x 2 bricks: key = 8 brick + 4 (z & 1) + 2 (y & 1) + (x & 1)
+ * with brick = (z/2) (nx/2) (ny/2) + (y/2) (nx/2) + x/2, the counts nx, ny, nz
+ * rounded up to even (a one-level Z-order: the 64 points of a wavefront sit
+ * in about one brick instead of one row of cells). No counterpart in the
+ * reference, whose only Morton codes order mesh faces
+ * (src/datastructure/lbvh.cuh:17-45). */
+enum { REYN_CELL_ORDER_XYZ = 0, REYN_CELL_ORDER_BRICK = 1 };
+
 /* Query-side view of a uniform grid; replaces the POD `UniformGrid<R>` and
  * `MaybeSorted<R>` (src/datastructure/uniformgrid.cuh:70-106).
  * `sorted == NULL` is `Resort::yes` (the indexed arrays are stored in cell
@@ -78,6 +89,7 @@ typedef struct reyn_grid {
     const uint32_t* prefix;
     const uint32_t* sorted;
     uint32_t n; /* number of indexed points */
+    int32_t order; /* REYN_CELL_ORDER_* */
 } reyn_grid;
 
 /* Fluid particle state in SoA layout; replaces `Particles`
@@ -172,6 +184,11 @@ typedef struct reyn_grid_builder reyn_grid_builder;
  * (:93-95). */
 int reyn_grid_builder_create(reyn_ctx* ctx, reyn_float3 bound_min,
     reyn_float3 bound_max, float search_radius, float cell_ratio,
+    reyn_grid_builder** out);
+/* The same with the cell order `order` (REYN_CELL_ORDER_*); the builder above
+ * is REYN_CELL_ORDER_XYZ. */
+int reyn_grid_builder_create_ordered(reyn_ctx* ctx, reyn_float3 bound_min,
+    reyn_float3 bound_max, float search_radius, float cell_ratio, int32_t order,
     reyn_grid_builder** out);
 int reyn_grid_builder_destroy(reyn_grid_builder* b);
 /* Geometry of the builder (prefix/sorted are NULL until a construct call). */
@@ -605,6 +622,9 @@ typedef struct reyn_batch_params {
     /* loop iterations launched between host checks after the first min_iter
      * (0 = 4); later chunks launch only the scenes whose loop is not done */
     uint32_t check_every;
+    /* REYN_CELL_ORDER_* of the fluid grids and so of each scene's state (the
+     * boundary grids are XYZ) */
+    int32_t cell_order;
 } reyn_batch_params;
 
 /* Uploads the scenes, builds their boundary grids and Akinci masses. The

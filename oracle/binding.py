@@ -32,6 +32,7 @@ class OrcGrid(C.Structure):
         ("range", i32),
         ("prefix", vp),
         ("sorted", vp),
+        ("order", i32),
     ]
 
     @property
@@ -115,11 +116,13 @@ class Grid:
         return Grid(g, self.prefix, None, self.counts, self.keys, self.outside)
 
 
-def geometry(bmin, bmax, r: float, ratio: float = 0.5) -> OrcGrid:
+def geometry(bmin, bmax, r: float, ratio: float = 0.5, order: int = 0) -> OrcGrid:
+    """Grid geometry; order 0 is the reference's XYZ cell order, 1 the
+    2 x 2 x 2 brick order."""
     g = OrcGrid()
     lo = _c(bmin, np.float32)
     hi = _c(bmax, np.float32)
-    rc = lib().orc_grid_geometry(_p(lo), _p(hi), f32(r), f32(ratio), C.byref(g))
+    rc = lib().orc_grid_geometry_ordered(_p(lo), _p(hi), f32(r), f32(ratio), C.c_int(int(order)), C.byref(g))
     if rc != 0:
         raise ValueError("invalid grid geometry")
     return g

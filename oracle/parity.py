@@ -67,7 +67,7 @@ def iisph_sim_step_parity(sim, ob=None):
     gpu = {"rho": solver.rho.numpy(), "p": solver.p.numpy(), "a_ii": solver.a_ii.numpy(),
            "s_i": solver.s_i.numpy(), "x": np.stack([out["xx"], out["xy"], out["xz"]], 1),
            "v": np.stack([out["vx"], out["vy"], out["vz"]], 1)}
-    geom = orc.geometry(sim.bound_min, sim.bound_max, r, sim.cell_ratio)
+    geom = orc.geometry(sim.bound_min, sim.bound_max, r, sim.cell_ratio, sim.cell_order)
     g = orc.grid_build(geom, hs["xx"], hs["xy"], hs["xz"])
     order = g.sorted
     st = orc.State({"x": hs["xx"][order], "y": hs["xy"][order], "z": hs["xz"][order], "vx": hs["vx"][order],
@@ -104,7 +104,7 @@ class GpuCase:
     """Uploads a scene, builds the boundary (GPU Akinci masses) and the
     fluid grid with reordering, and keeps host copies of every input."""
 
-    def __init__(self, ctx, sc, v, cell_ratio=0.5, with_bdy=True, masses=None):
+    def __init__(self, ctx, sc, v, cell_ratio=0.5, with_bdy=True, masses=None, order=0):
         self.ctx = ctx
         self.sc = sc
         h = sc.spacing
@@ -112,8 +112,9 @@ class GpuCase:
         self.r = 2.0 * h
         self.state = Particles(ctx, sc.x, v, sc.mass if masses is None else masses, sc.rho_0)
         self.bdy = BoundarySamples(ctx, sc.xb, h, sc.rho_0, cell_ratio) if with_bdy else None
-        self.builder = UniformGridBuilder(ctx, sc.bound_min, sc.bound_max, self.r, cell_ratio)
+        self.builder = UniformGridBuilder(ctx, sc.bound_min, sc.bound_max, self.r, cell_ratio, order)
         self.cell_ratio = cell_ratio
+        self.order = order
         self.extra = []
 
     def grid(self, extra_owner=None, extra=()):
@@ -127,7 +128,7 @@ class GpuCase:
         """Oracle grid/boundary/state built from the GPU's reordered inputs."""
         sc = self.sc
         hs = self.host
-        geom = orc.geometry(sc.bound_min, sc.bound_max, self.r, self.cell_ratio)
+        geom = orc.geometry(sc.bound_min, sc.bound_max, self.r, self.cell_ratio, self.order)
         og = orc.grid_build(geom, hs["xx"], hs["xy"], hs["xz"])
         # the reordered state is in cell order: its build is the identity
         assert np.array_equal(og.sorted, np.arange(og.sorted.size)), "GPU reorder is not in cell order"
@@ -151,7 +152,8 @@ def sph_oracle(kernel: int, h: float, rho0: float):
 
 
 def run_iisph_case(ctx, kernel="B3", dt=2e-3, side=14, warm=True, stop_rule=capi.IISPH_STOP_ANY, eta=0.005,
-                   min_iter=3, max_iter=0, seed=7, p0_scale=200.0, cell_ratio=0.5, compress=0.97, mass_spread=0.0):
+                   min_iter=3, max_iter=0, seed=7, p0_scale=200.0, cell_ratio=0.5, compress=0.97, mass_spread=0.0,
+                   order=0):
     """One IISPH step on the GPU against the FP64 oracle from the same inputs;
     `mass_spread` > 0 gives every particle its own mass, m (1 + U(-spread,
     spread))."""
@@ -160,7 +162,7 @@ def run_iisph_case(ctx, kernel="B3", dt=2e-3, side=14, warm=True, stop_rule=capi
     if mass_spread:
         u = np.random.default_rng(seed + 5).uniform(-mass_spread, mass_spread, sc.n)
         masses = (sc.mass * (1.0 + u)).astype(np.float32)
-    case = GpuCase(ctx, sc, v, cell_ratio, masses=masses)
+    case = GpuCase(ctx, sc, v, cell_ratio, masses=masses, order=order)
     solver = IISPH(ctx, kernel, sc.n, NU, sc.spacing, sc.rho_0, eta_rho_max=eta, min_iter=min_iter,
                    max_iter=max_iter, stop_rule=stop_rule, warm_start=warm)
     p0 = np.abs(np.random.default_rng(seed + 2).normal(0.0, p0_scale, sc.n)).astype(np.float32)
@@ -243,7 +245,7 @@ def batch_scene_step_parity(batch, sid: int, bdy_iterations: int = 5):
     inf = batch.info(sid)
     h = float(sc.spacing)
     r = 2.0 * h
-    geom = orc.geometry(sc.bound_min, sc.bound_max, r, prm.cell_ratio)
+    geom = orc.geometry(sc.bound_min, sc.bound_max, r, prm.cell_ratio, prm.cell_order)
     g = orc.grid_build(geom, before["xx"], before["xy"], before["xz"])
     o = g.sorted
     st = orc.State({"x": before["xx"][o], "y": before["xy"][o], "z": before["xz"][o], "vx": before["vx"][o],

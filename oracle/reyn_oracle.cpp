@@ -56,7 +56,21 @@ struct OGrid {
     int range;
     const uint32_t* prefix;
     const uint32_t* sorted; // nullptr: arrays are in cell order
+    int order; // 0: the reference's XYZ keys; 1: 2 x 2 x 2 bricks
 };
+
+// The key of cell (x, y, z): z nx ny + y nx + x (uniformgrid.cuh:33-40), or in
+// brick order 8 brick + 4 (z & 1) + 2 (y & 1) + (x & 1) with the bricks in
+// XYZ order over (nx/2, ny/2, nz/2).
+inline uint32_t okey(int order, const int n[3], int x, int y, int z)
+{
+    if (order == 1) {
+        const uint32_t nbx = uint32_t(n[0] / 2), nby = uint32_t(n[1] / 2);
+        const uint32_t brick = (uint32_t(z >> 1) * nby + uint32_t(y >> 1)) * nbx + uint32_t(x >> 1);
+        return brick * 8u + uint32_t(((z & 1) << 2) | ((y & 1) << 1) | (x & 1));
+    }
+    return uint32_t(z) * uint32_t(n[0]) * uint32_t(n[1]) + uint32_t(y) * uint32_t(n[0]) + uint32_t(x);
+}
 
 inline int cell_axis(float x, float lo, float cell, int n)
 {
@@ -90,22 +104,26 @@ inline void walk(const OGrid& g, const T* xs, const T* ys, const T* zs, T xi, T 
     const int x0 = std::max(cx - R, 0), x1 = std::min(cx + R, g.n[0] - 1);
     const int y0 = std::max(cy - R, 0), y1 = std::min(cy + R, g.n[1] - 1);
     const int z0 = std::max(cz - R, 0), z1 = std::min(cz + R, g.n[2] - 1);
-    const long nx = g.n[0], nxny = long(g.n[0]) * g.n[1];
+    // a strip of cells x0..x1 at (y, z): in XYZ order their keys are
+    // consecutive, one range; in brick order each cell is its own range,
+    // visited in ascending x (the same candidates in the same order)
+    const bool brick = g.order == 1;
     for (int z = z0; z <= z1; ++z)
-        for (int y = y0; y <= y1; ++y) {
-            const long base = z * nxny + y * nx;
-            const uint32_t s = g.prefix[base + x0];
-            const uint32_t e = g.prefix[base + x1 + 1];
-            for (uint32_t q = s; q < e; ++q) {
-                const uint32_t j = g.sorted ? g.sorted[q] : q;
-                const float dx = fx - static_cast<float>(xs[j]);
-                const float dy = fy - static_cast<float>(ys[j]);
-                const float dz = fz - static_cast<float>(zs[j]);
-                const float d2 = std::fma(dx, dx, std::fma(dy, dy, dz * dz));
-                if (d2 <= g.r2)
-                    f(j);
+        for (int y = y0; y <= y1; ++y)
+            for (int xa = x0; xa <= x1; xa = brick ? xa + 1 : x1 + 1) {
+                const int xb = brick ? xa : x1;
+                const uint32_t s = g.prefix[okey(g.order, g.n, xa, y, z)];
+                const uint32_t e = g.prefix[okey(g.order, g.n, xb, y, z) + 1];
+                for (uint32_t q = s; q < e; ++q) {
+                    const uint32_t j = g.sorted ? g.sorted[q] : q;
+                    const float dx = fx - static_cast<float>(xs[j]);
+                    const float dy = fy - static_cast<float>(ys[j]);
+                    const float dz = fz - static_cast<float>(zs[j]);
+                    const float d2 = std::fma(dx, dx, std::fma(dy, dy, dz * dz));
+                    if (d2 <= g.r2)
+                        f(j);
+                }
             }
-        }
 }
 
 // ---- smoothing kernels (src/kernels.cuh:19-174) ---------------------------
@@ -586,6 +604,7 @@ struct orc_grid {
     int32_t range;
     const uint32_t* prefix;
     const uint32_t* sorted;
+    int32_t order;
 };
 
 static OGrid to_og(const orc_grid* g)
@@ -600,6 +619,7 @@ static OGrid to_og(const orc_grid* g)
     o.range = g->range;
     o.prefix = g->prefix;
     o.sorted = g->sorted;
+    o.order = g->order;
     return o;
 }
 
@@ -622,8 +642,10 @@ ORC_API void orc_set_threads(int n)
 #endif
 }
 
-// UniformGridBuilder geometry (uniformgrid.cu:65-96) with cell = ratio * r.
-ORC_API int orc_grid_geometry(const float* bmin, const float* bmax, float r, float ratio, orc_grid* out)
+// UniformGridBuilder geometry (uniformgrid.cu:65-96) with cell = ratio * r;
+// in brick order the cell counts are rounded up to even.
+ORC_API int orc_grid_geometry_ordered(const float* bmin, const float* bmax, float r, float ratio, int order,
+    orc_grid* out)
 {
     const float cell = r * ratio;
     const float pad = static_cast<float>(r * 2.0);
@@ -635,13 +657,21 @@ ORC_API int orc_grid_geometry(const float* bmin, const float* bmax, float r, flo
             return -1;
         out->bmin[a] = lo;
         out->n[a] = static_cast<int32_t>(e);
+        if (order == 1)
+            out->n[a] += out->n[a] & 1;
     }
     out->cell = cell;
     out->r2 = r * r;
     out->range = static_cast<int32_t>(std::ceil(1.0 / double(ratio) - 1e-6));
     out->prefix = nullptr;
     out->sorted = nullptr;
+    out->order = order;
     return 0;
+}
+
+ORC_API int orc_grid_geometry(const float* bmin, const float* bmax, float r, float ratio, orc_grid* out)
+{
+    return orc_grid_geometry_ordered(bmin, bmax, r, ratio, 0, out);
 }
 
 // Counting sort (uniformgrid.cu:20-63, 98-143) with ascending index inside a
@@ -659,7 +689,7 @@ ORC_API long orc_grid_build(const orc_grid* geom, const float* x, const float* y
         const int cx = cell_axis(x[i], geom->bmin[0], geom->cell, int(nx));
         const int cy = cell_axis(y[i], geom->bmin[1], geom->cell, int(ny));
         const int cz = cell_axis(z[i], geom->bmin[2], geom->cell, int(nz));
-        keys[i] = static_cast<uint32_t>(cz * nx * ny + cy * nx + cx);
+        keys[i] = okey(geom->order, geom->n, cx, cy, cz);
         counts[keys[i]]++;
         if (cx < R || cy < R || cz < R || cx > nx - 1 - R || cy > ny - 1 - R || cz > nz - 1 - R)
             ++outside;
@@ -1026,7 +1056,7 @@ void grid_build_par(const orc_grid* geom, const float* x, const float* y, const 
         const int cx = cell_axis(x[i], geom->bmin[0], geom->cell, int(nx));
         const int cy = cell_axis(y[i], geom->bmin[1], geom->cell, int(ny));
         const int cz = cell_axis(z[i], geom->bmin[2], geom->cell, int(nz));
-        const uint32_t k = static_cast<uint32_t>(cz * nx * ny + cy * nx + cx);
+        const uint32_t k = okey(geom->order, geom->n, cx, cy, cz);
         keys[i] = k;
 #pragma omp atomic
         counts[k] += 1u;

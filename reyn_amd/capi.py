@@ -27,6 +27,8 @@ ERR_CAPACITY = -7
 KERNEL_C2, KERNEL_B3, KERNEL_W6, KERNEL_COS = 0, 1, 2, 3
 KERNELS = {"C2": KERNEL_C2, "B3": KERNEL_B3, "W6": KERNEL_W6, "COS": KERNEL_COS}
 IISPH_STOP_ANY, IISPH_STOP_MEAN = 0, 1
+CELL_ORDER_XYZ, CELL_ORDER_BRICK = 0, 1
+CELL_ORDERS = {"xyz": CELL_ORDER_XYZ, "brick": CELL_ORDER_BRICK}
 
 u32 = C.c_uint32
 u64 = C.c_uint64
@@ -54,6 +56,7 @@ class Grid(C.Structure):
         ("prefix", vp),
         ("sorted", vp),
         ("n", u32),
+        ("order", i32),
     ]
 
     @property
@@ -168,7 +171,7 @@ class BatchParams(C.Structure):
         ("solver", i32), ("kernel", i32), ("nu", f32), ("rho_0", f32), ("g", Float3), ("cell_ratio", f32),
         ("dt0", f32), ("cfl_lambda", f32), ("change_factor", f32), ("iter_target", u32),
         ("eta_rho_max", f32), ("omega", f32), ("min_iter", u32), ("max_iter", u32), ("stop_rule", i32),
-        ("warm_start", i32), ("graph", i32), ("check_every", u32),
+        ("warm_start", i32), ("graph", i32), ("check_every", u32), ("cell_order", i32),
     ]
 
 
@@ -193,6 +196,7 @@ _SIGS = {
     "reyn_profile_enable": (i32, [vp, C.c_int]),
     "reyn_profile_summary": (i32, [vp, C.c_char_p, C.c_size_t, P(C.c_size_t)]),
     "reyn_grid_builder_create": (i32, [vp, Float3, Float3, f32, f32, P(vp)]),
+    "reyn_grid_builder_create_ordered": (i32, [vp, Float3, Float3, f32, f32, i32, P(vp)]),
     "reyn_grid_builder_destroy": (i32, [vp]),
     "reyn_grid_builder_geometry": (i32, [vp, P(Grid)]),
     "reyn_grid_construct": (i32, [vp, vp, f32, vp, vp, vp, u32, P(Grid)]),

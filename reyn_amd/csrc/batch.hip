@@ -371,7 +371,7 @@ int make_scene(reyn_batch* b, const reyn_batch_scene& sc, SceneRes** out)
                 break;
         }
         reyn_grid geom {};
-        rc = grid_geometry(ctx, sc.bound_min, sc.bound_max, rad, prm.cell_ratio, &geom, &r->ncells);
+        rc = grid_geometry(ctx, sc.bound_min, sc.bound_max, rad, prm.cell_ratio, &geom, &r->ncells, prm.cell_order);
         if (rc != REYN_OK)
             break;
         r->g = to_dev(geom);
@@ -1002,7 +1002,8 @@ int reyn_batch_scene_bytes(const reyn_batch_params* prm, const reyn_batch_scene*
     const float rad = 2.f * sc->h;
     reyn_grid geom {};
     size_t cells = 0;
-    REYN_TRY(grid_geometry(nullptr, sc->bound_min, sc->bound_max, rad, prm->cell_ratio, &geom, &cells));
+    REYN_TRY(grid_geometry(nullptr, sc->bound_min, sc->bound_max, rad, prm->cell_ratio, &geom, &cells,
+        prm->cell_order));
     uint64_t total = 0;
     // fluid: both state buffers, the work arrays, the solver scratch and the
     // grid's per-point arrays; its cells: counts and prefix

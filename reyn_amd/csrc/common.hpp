@@ -96,7 +96,12 @@ struct ksum3 {
 
 // ---- launch geometry --------------------------------------------------------
 
-constexpr int kBlock = 256; // 4 wave64 per workgroup
+// lanes per workgroup: 4 wave64 (builds for measurement may set REYN_BLOCK
+// to another multiple of 64)
+#ifndef REYN_BLOCK
+#define REYN_BLOCK 256
+#endif
+constexpr int kBlock = REYN_BLOCK;
 constexpr int kWave = 64;
 constexpr uint32_t kXcds = 8; // MI355X: 8 XCDs, workgroups dispatched round-robin
 

@@ -86,7 +86,7 @@ __global__ __launch_bounds__(kBlock) void k_count(SceneArg sa, const float* __re
     if (valid) {
         const f3 p = ld3(xs, ys, zs, i);
         const int3c c = cell_of(g, p);
-        key = static_cast<uint32_t>(c.z * g.nxny + c.y * g.nx + c.x) + (BT ? cell_off[sc.idx] : 0u);
+        key = cell_key(g, c) + (BT ? cell_off[sc.idx] : 0u);
         const int R = g.range;
         outside = c.x < R || c.y < R || c.z < R || c.x > g.nx - 1 - R || c.y > g.ny - 1 - R
             || c.z > g.nz - 1 - R;
@@ -268,10 +268,27 @@ __global__ __launch_bounds__(kBlock) void k_occupancy(GridDev g, uint64_t ncells
     const uint64_t c = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
     if (c >= ncells)
         return;
+    const int R = g.range;
+    if (g.order == REYN_CELL_ORDER_BRICK) {
+        // key -> cell, then every cell in reach tested by its own count
+        const uint32_t br = static_cast<uint32_t>(c >> 3), loc = static_cast<uint32_t>(c & 7);
+        const int bx = static_cast<int>(br % uint32_t(g.nbx));
+        const int by = static_cast<int>((br / uint32_t(g.nbx)) % uint32_t(g.ny / 2));
+        const int bz = static_cast<int>(br / uint32_t(g.nbxy));
+        const int x = 2 * bx + int(loc & 1), y = 2 * by + int((loc >> 1) & 1), z = 2 * bz + int(loc >> 2);
+        uint8_t any = 0;
+        for (int zz = max(z - R, 0); zz <= min(z + R, g.nz - 1) && !any; ++zz)
+            for (int yy = max(y - R, 0); yy <= min(y + R, g.ny - 1) && !any; ++yy)
+                for (int xx = max(x - R, 0); xx <= min(x + R, g.nx - 1) && !any; ++xx) {
+                    const uint32_t k = cell_key(g, int3c { xx, yy, zz });
+                    any = g.prefix[k + 1] > g.prefix[k] ? 1 : 0;
+                }
+        near[c] = any;
+        return;
+    }
     const int x = static_cast<int>(c % uint64_t(g.nx));
     const int y = static_cast<int>((c / uint64_t(g.nx)) % uint64_t(g.ny));
     const int z = static_cast<int>(c / uint64_t(g.nxny));
-    const int R = g.range;
     const int x0 = max(x - R, 0), x1 = min(x + R, g.nx - 1);
     uint8_t any = 0;
     for (int zz = max(z - R, 0); zz <= min(z + R, g.nz - 1) && !any; ++zz)
@@ -508,8 +525,10 @@ int launch_gather(reyn_ctx* ctx, const uint32_t* sorted, uint32_t n, const float
 namespace reyn {
 
 int grid_geometry(reyn_ctx* ctx, reyn_float3 bmin, reyn_float3 bmax, float search_radius, float cell_ratio,
-    reyn_grid* out, size_t* ncells)
+    reyn_grid* out, size_t* ncells, int order)
 {
+    if (order != REYN_CELL_ORDER_XYZ && order != REYN_CELL_ORDER_BRICK)
+        return set_error(ctx, REYN_ERR_INVALID_ARGUMENT, "grid: unknown cell order %d", order);
     if (!(search_radius > 0.f) || !std::isfinite(search_radius))
         return set_error(ctx, REYN_ERR_INVALID_ARGUMENT, "grid: search radius must be positive");
     if (!(cell_ratio >= 0.25f && cell_ratio <= 4.f))
@@ -526,6 +545,10 @@ int grid_geometry(reyn_ctx* ctx, reyn_float3 bmin, reyn_float3 bmax, float searc
             return set_error(ctx, REYN_ERR_INVALID_ARGUMENT,
                 "Negative number of grid cells encountered in construction of uniform grid.");
         n[a] = static_cast<int>(e);
+        // bricks of 2 x 2 x 2 cells tile the grid: the counts are even (the
+        // extra layer of cells lies beyond the padded bound and stays empty)
+        if (order == REYN_CELL_ORDER_BRICK)
+            n[a] += n[a] & 1;
     }
     const double cells = double(n[0]) * double(n[1]) * double(n[2]);
     if (cells >= 1073741824.0)
@@ -539,6 +562,7 @@ int grid_geometry(reyn_ctx* ctx, reyn_float3 bmin, reyn_float3 bmax, float searc
     out->nz = n[2];
     // cells searched on each side: ceil(radius / cell)
     out->range = static_cast<int>(std::ceil(1.0 / double(cell_ratio) - 1e-6));
+    out->order = order;
     *ncells = size_t(n[0]) * size_t(n[1]) * size_t(n[2]);
     return REYN_OK;
 }
@@ -588,9 +612,18 @@ int reyn_grid_builder_create(reyn_ctx* ctx, reyn_float3 bmin, reyn_float3 bmax,
     if (!ctx || !out)
         return REYN_ERR_INVALID_ARGUMENT;
     *out = nullptr;
+    return reyn_grid_builder_create_ordered(ctx, bmin, bmax, search_radius, cell_ratio, REYN_CELL_ORDER_XYZ, out);
+}
+
+int reyn_grid_builder_create_ordered(reyn_ctx* ctx, reyn_float3 bmin, reyn_float3 bmax, float search_radius,
+    float cell_ratio, int32_t order, reyn_grid_builder** out)
+{
+    if (!ctx || !out)
+        return REYN_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
     reyn_grid geom {};
     size_t ncells = 0;
-    REYN_TRY(grid_geometry(ctx, bmin, bmax, search_radius, cell_ratio, &geom, &ncells));
+    REYN_TRY(grid_geometry(ctx, bmin, bmax, search_radius, cell_ratio, &geom, &ncells, order));
     reyn_grid_builder* b = nullptr;
     REYN_TRY(builder_alloc(ctx, ncells, search_radius, &b));
     b->geom = geom;

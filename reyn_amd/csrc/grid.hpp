@@ -27,7 +27,7 @@ namespace reyn {
 // The reference's grid geometry (uniformgrid.cu:65-96) with the cell size
 // scaled by `cell_ratio`; no memory.
 int grid_geometry(reyn_ctx* ctx, reyn_float3 bmin, reyn_float3 bmax, float search_radius, float cell_ratio,
-    reyn_grid* out, size_t* ncells);
+    reyn_grid* out, size_t* ncells, int order = REYN_CELL_ORDER_XYZ);
 
 // A builder whose cell tables cover `ncells` cells (a batch concatenates the
 // cells of all its scenes); its geometry fields are unused.

@@ -215,7 +215,7 @@ __global__ __launch_bounds__(kBlock) REYN_OCC(REYN_OCC_DENSITY) void k_density(S
     const BdyDev& b = sc.b;
     const KParams& kp = sc.c.kp;
     // lists serve ranges up to kListMaxRange; beyond, every later loop walks
-    const bool listing = L.cnt && P4 && g.range <= kListMaxRange;
+    const bool listing = L.cnt && P4 && lists_serve(g);
     if (L.cnt && P4 && !listing && live)
         L.cnt[t] = L.cap + 1;
     if (!live && !listing)

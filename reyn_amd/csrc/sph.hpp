@@ -143,6 +143,8 @@ struct GridDev {
     float r2;
     int nx, ny, nz, nxny;
     int range;
+    int order; // REYN_CELL_ORDER_*
+    int nbx, nbxy; // bricks per row and per plane (REYN_CELL_ORDER_BRICK)
     const uint32_t* __restrict__ prefix;
     const uint32_t* __restrict__ sorted;
 };
@@ -158,6 +160,9 @@ inline GridDev to_dev(const reyn_grid& g)
     d.nz = g.nz;
     d.nxny = g.nx * g.ny;
     d.range = g.range;
+    d.order = g.order;
+    d.nbx = g.nx / 2;
+    d.nbxy = (g.nx / 2) * (g.ny / 2);
     d.prefix = g.prefix;
     d.sorted = g.sorted;
     return d;
@@ -183,8 +188,15 @@ __device__ __forceinline__ int3c cell_of(const GridDev& g, f3 x)
         cell_axis(x.z, g.bmin.z, g.cell, g.nz) };
 }
 
+// The first key of brick (bx, by, bz) (REYN_CELL_ORDER_BRICK).
+__device__ __forceinline__ uint32_t brick_base(const GridDev& g, int bx, int by, int bz)
+{
+    return (static_cast<uint32_t>(bz) * static_cast<uint32_t>(g.nbxy) + static_cast<uint32_t>(by * g.nbx + bx)) << 3;
+}
 __device__ __forceinline__ uint32_t cell_key(const GridDev& g, int3c c)
 {
+    if (g.order == REYN_CELL_ORDER_BRICK)
+        return brick_base(g, c.x >> 1, c.y >> 1, c.z >> 1) + static_cast<uint32_t>(((c.z & 1) << 2) | ((c.y & 1) << 1) | (c.x & 1));
     return static_cast<uint32_t>(c.z) * static_cast<uint32_t>(g.nxny) + static_cast<uint32_t>(c.y * g.nx + c.x);
 }
 
@@ -225,6 +237,42 @@ template <class Rg> __device__ __forceinline__ void strips(const GridDev& g, f3 
         budget -= e - s;
         r(s, e, plane, base);
     };
+    if (g.order == REYN_CELL_ORDER_BRICK) {
+        // A strip's 2R+1 cells lie in R+1 bricks; the cells of one strip in
+        // one brick have consecutive keys, so each brick is one range (empty
+        // where clipped): (2R+1)^2 (R+1) calls. For R = 1 the strips fall in
+        // 4 segments, (brick plane, brick row) with z, y each the lower or
+        // upper of the two bricks the 3 cells span; `plane` is the segment and
+        // `base` the first candidate of its lower brick in x, below every
+        // candidate of the segment.
+        const int bz0 = max(c.z - R, 0) >> 1, by0 = max(c.y - R, 0) >> 1, bx0 = x0 >> 1;
+        for (int dz = -R; dz <= R; ++dz) {
+            const int z = c.z + dz;
+            const int zc = min(max(z, 0), g.nz - 1);
+            for (int dy = -R; dy <= R; ++dy) {
+                const int y = c.y + dy;
+                const int yc = min(max(y, 0), g.ny - 1);
+                const bool in = z >= 0 && z < g.nz && y >= 0 && y < g.ny;
+                uint32_t seg = 0, sbase = 0;
+                if (R == 1) {
+                    seg = static_cast<uint32_t>((((zc >> 1) - bz0) << 1) | ((yc >> 1) - by0));
+                    sbase = gat(g.prefix, brick_base(g, bx0, yc >> 1, zc >> 1));
+                }
+                int xs = x0;
+                for (int k = 0; k <= R; ++k) {
+                    const int xe = min(xs | 1, x1);
+                    uint32_t s0 = 0, e0 = 0;
+                    if (in && xs <= x1) {
+                        s0 = gat(g.prefix, cell_key(g, int3c { xs, yc, zc }));
+                        e0 = gat(g.prefix, cell_key(g, int3c { xe, yc, zc }) + 1);
+                    }
+                    emit(s0, e0, seg, sbase);
+                    xs = (xs | 1) + 1;
+                }
+            }
+        }
+        return;
+    }
     if (R <= 2) {
         for (int dz = -R; dz <= R; ++dz) {
             const int z = c.z + dz;
@@ -551,9 +599,21 @@ __device__ __forceinline__ uint32_t encode_entry(const Cand& c, bool* fits)
 }
 
 // The decode bases B_u = base_u - (u << kPlaneShift) of the particle at xi
-// for its NP = 2R+1 planes, from the same cell-start entries the walk loads.
+// for its NP = 2R+1 planes (XYZ order) or its 4 segments (brick order, range
+// 1), from the same cell-start entries the walk loads.
 template <int NP> __device__ __forceinline__ void plane_bases(const GridDev& g, f3 xi, uint32_t (&B)[NP])
 {
+    if constexpr (NP == 4) {
+        const int3c c = cell_of(g, xi);
+        const int bz0 = max(c.z - 1, 0) >> 1, by0 = max(c.y - 1, 0) >> 1, bx0 = max(c.x - 1, 0) >> 1;
+        const int bz1 = min(c.z + 1, g.nz - 1) >> 1, by1 = min(c.y + 1, g.ny - 1) >> 1;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int bz = (u >> 1) ? bz1 : bz0, by = (u & 1) ? by1 : by0;
+            B[u] = gat(g.prefix, brick_base(g, bx0, by, bz)) - (uint32_t(u) << kPlaneShift);
+        }
+        return;
+    }
     constexpr int R = (NP - 1) / 2;
     const int3c c = cell_of(g, xi);
     const int x0 = max(c.x - R, 0);
@@ -611,19 +671,31 @@ __device__ __forceinline__ void list_rounds(const GridDev& g, const NbrDev& L, u
     }
 }
 
-// list_rounds for the grid's range (3 or 5 planes); false if the list cannot
-// serve this grid.
+// list_rounds for the grid's range and order (3 or 5 planes in XYZ order, 4
+// segments in brick order at range 1); false if the list cannot serve this
+// grid.
 template <uint32_t B, class Body>
 __device__ __forceinline__ bool with_list(const GridDev& g, const NbrDev& L, uint32_t t, uint32_t c, f3 xi,
     Body&& body)
 {
-    if (g.range == 1)
+    if (g.order == REYN_CELL_ORDER_BRICK) {
+        if (g.range != 1)
+            return false;
+        list_rounds<4, B>(g, L, t, c, xi, body);
+    } else if (g.range == 1) {
         list_rounds<3, B>(g, L, t, c, xi, body);
-    else if (g.range == 2)
+    } else if (g.range == 2) {
         list_rounds<5, B>(g, L, t, c, xi, body);
-    else
+    } else {
         return false;
+    }
     return true;
+}
+
+// Whether the density pass keeps a list for this grid (as with_list serves).
+__device__ __forceinline__ bool lists_serve(const GridDev& g)
+{
+    return g.order == REYN_CELL_ORDER_BRICK ? g.range == 1 : g.range <= kListMaxRange;
 }
 
 // Calls f(j, x_ij, d2, m_j) for the neighbours of the particle in slot t at

@@ -61,12 +61,14 @@ class UniformGridBuilder:
     """Owns counts/prefix/sorted; ``construct`` and ``construct_and_reorder``
     return a ``capi.Grid`` view (uniformgrid.cu:65-181)."""
 
-    def __init__(self, ctx: Context, bound_min, bound_max, search_radius: float, cell_ratio: float = 0.5):
+    def __init__(self, ctx: Context, bound_min, bound_max, search_radius: float, cell_ratio: float = 0.5,
+                 order: str | int = "xyz"):
         self.ctx = ctx
         h = C.c_void_p()
         ctx.check(
-            ctx._lib.reyn_grid_builder_create(
-                ctx.h, capi.f3(bound_min), capi.f3(bound_max), float(search_radius), float(cell_ratio), C.byref(h)
+            ctx._lib.reyn_grid_builder_create_ordered(
+                ctx.h, capi.f3(bound_min), capi.f3(bound_max), float(search_radius), float(cell_ratio),
+                cell_order_id(order), C.byref(h)
             ),
             "UniformGridBuilder",
         )
@@ -319,7 +321,7 @@ class Simulation:
     def __init__(self, ctx: Context, solver, state: Particles, bdy: BoundarySamples | None, h: float,
                  bound_min, bound_max, cfl_lambda: float = 0.4, cell_ratio: float = 0.5, reorder_every: int = 1,
                  extra_reorder: tuple[str, ...] = (), dt_policy: str = "cfl", dt0: float = 0.0005,
-                 iter_target: int = 10, change_factor: float = 0.001, trace=None):
+                 iter_target: int = 10, change_factor: float = 0.001, trace=None, cell_order: str | int = "xyz"):
         if dt_policy not in ("cfl", "controller"):
             raise ValueError(f"unknown dt policy {dt_policy!r}")
         self.ctx = ctx
@@ -331,7 +333,8 @@ class Simulation:
         self.bound_max = np.asarray(bound_max, np.float32)
         self.cfl_lambda = cfl_lambda
         self.cell_ratio = float(cell_ratio)
-        self.builder = UniformGridBuilder(ctx, self.bound_min, self.bound_max, 2.0 * h, cell_ratio)
+        self.cell_order = cell_order_id(cell_order)
+        self.builder = UniformGridBuilder(ctx, self.bound_min, self.bound_max, 2.0 * h, cell_ratio, cell_order)
         self.reorder_every = max(1, int(reorder_every))
         self.extra_reorder = extra_reorder
         self.dt_policy = dt_policy
@@ -378,12 +381,17 @@ class Simulation:
         return stats
 
 
+def cell_order_id(order: str | int) -> int:
+    """REYN_CELL_ORDER_* from "xyz" / "brick" or the number itself."""
+    return capi.CELL_ORDERS[order] if isinstance(order, str) else int(order)
+
+
 def batch_params(scenes=(), solver: str = "pcisph", kernel: str | int = "W6", nu: float = 0.001,
                  cell_ratio: float = 0.5, dt0: float = 0.0005, cfl_lambda: float = 1.0,
                  change_factor: float = 0.001, iter_target: int = 10, eta_rho_max: float | None = None,
                  omega: float = 0.5, min_iter: int = 3, max_iter: int = 100,
                  stop_rule: int = capi.IISPH_STOP_MEAN, warm_start: bool = True, g=GRAVITY, graph: bool = False,
-                 check_every: int = 4, rho_0: float | None = None) -> capi.BatchParams:
+                 check_every: int = 4, rho_0: float | None = None, cell_order: str | int = "xyz") -> capi.BatchParams:
     """`reyn_batch_params` from keyword options (rho_0 defaults to the first
     scene's)."""
     solver_id = {"pcisph": capi.SOLVER_PCISPH, "iisph": capi.SOLVER_IISPH}[solver]
@@ -396,7 +404,7 @@ def batch_params(scenes=(), solver: str = "pcisph", kernel: str | int = "W6", nu
                             float(cell_ratio), float(dt0), float(cfl_lambda), float(change_factor),
                             int(iter_target), float(eta_rho_max), float(omega), int(min_iter),
                             int(max_iter), int(stop_rule), int(bool(warm_start)), int(bool(graph)),
-                            int(check_every))
+                            int(check_every), cell_order_id(cell_order))
 
 
 def batch_scenes(scenes, keep: list):

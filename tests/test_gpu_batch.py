@@ -23,14 +23,16 @@ def assert_same(a, b, what):
         assert np.array_equal(a[k], b[k]), f"{what}: field {k} differs (max |d| {np.abs(a[k] - b[k]).max()})"
 
 
-@pytest.mark.parametrize("solver,kernel", [("pcisph", "W6"), ("iisph", "B3")])
-def test_scene_alone_equals_scene_in_batch(ctx, solver, kernel):
+@pytest.mark.parametrize("solver,kernel,order", [("pcisph", "W6", "xyz"), ("iisph", "B3", "xyz"),
+                                                 ("iisph", "B3", "brick")])
+def test_scene_alone_equals_scene_in_batch(ctx, solver, kernel, order):
     scs = small_scenes()
-    print(f"{solver}: scenes of {[sc.n for sc in scs]} particles", flush=True)
-    mix = Batch(ctx, scs, solver=solver, kernel=kernel)
+    print(f"{solver}, {order} order: scenes of {[sc.n for sc in scs]} particles", flush=True)
+    opts = dict(solver=solver, kernel=kernel, cell_order=order, cell_ratio=1.0 if order == "brick" else 0.5)
+    mix = Batch(ctx, scs, **opts)
     mix.step(12)
     for s, sc in enumerate(scs):
-        alone = Batch(ctx, [sc], solver=solver, kernel=kernel)
+        alone = Batch(ctx, [sc], **opts)
         alone.step(12)
         a, b = alone.download(0), mix.download(s)
         assert_same(a, b, f"scene {s}")

@@ -23,8 +23,8 @@ def _upload_xyz(ctx, x):
     return [ctx.array(np.ascontiguousarray(x[:, a])) for a in range(3)]
 
 
-def _gpu_construct(ctx, x, lo, hi, r, ratio):
-    b = UniformGridBuilder(ctx, lo, hi, r, ratio)
+def _gpu_construct(ctx, x, lo, hi, r, ratio, order=0):
+    b = UniformGridBuilder(ctx, lo, hi, r, ratio, order)
     xx, xy, xz = _upload_xyz(ctx, x)
     g = b.construct(r, xx, xy, xz)
     prefix, sorted_ = grid_tables(ctx, g)
@@ -223,3 +223,54 @@ def test_boundary_occupancy(ctx, ratio):
                   "density")
         out.append(rho.numpy())
     assert np.array_equal(out[0], out[1])
+
+
+@pytest.mark.parametrize("ratio", [0.5, 1.0, 2.0])
+def test_brick_order_tables_bitexact_and_counts_equal_brute_force(ctx, ratio):
+    """The brick cell order on the reference test's points: keys, prefix and
+    sorted indices bit-exact with the oracle's brick mode, and every neighbour
+    count equal to brute force (the XYZ counts)."""
+    x = orc.reference_grid_points(50, 1.1)
+    r = np.float32(2.0 * np.float32(1.1))
+    lo, hi = np.zeros(3, np.float32), np.full(3, np.float32(1.1) * 50, np.float32)
+    b, g, (xx, xy, xz), prefix, sorted_, counts, keys = _gpu_construct(ctx, x, lo, hi, r, ratio, "brick")
+    geom = orc.geometry(lo, hi, r, ratio, order=1)
+    ref = orc.grid_build(geom, x[:, 0], x[:, 1], x[:, 2])
+    assert g.order == capi.CELL_ORDER_BRICK and (g.nx, g.ny, g.nz) == tuple(geom.n)
+    assert g.nx % 2 == 0 and g.ny % 2 == 0 and g.nz % 2 == 0
+    np.testing.assert_array_equal(keys, ref.keys)
+    np.testing.assert_array_equal(prefix, ref.prefix)
+    np.testing.assert_array_equal(sorted_, ref.sorted)
+    n = x.shape[0]
+    cnt, l2, v = (ctx.empty(n, np.uint32), ctx.empty(n), [ctx.empty(n) for _ in range(3)])
+    ctx.check(ctx._lib.reyn_grid_query_stats(ctx.h, C.byref(g), xx._ptr, xy._ptr, xz._ptr, n, cnt._ptr, l2._ptr,
+                                             v[0]._ptr, v[1]._ptr, v[2]._ptr))
+    idx = np.random.default_rng(4).choice(n, 20000, replace=False).astype(np.uint32)
+    bc, _, _ = orc.brute_stats_subset(r * r, x[:, 0], x[:, 1], x[:, 2], idx)
+    np.testing.assert_array_equal(cnt.numpy()[idx], bc)
+    # the reordered query over the brick order equals the oracle's brick walk
+    o_cnt, _, _ = orc.query_stats(ref, x[:, 0], x[:, 1], x[:, 2])
+    np.testing.assert_array_equal(cnt.numpy(), o_cnt)
+
+
+def test_brick_order_occupancy(ctx):
+    """Occupancy flags over a brick-ordered grid: near[key] is 1 exactly when a
+    point lies in the (2R+1)^3 cells around that key's cell."""
+    rng = np.random.default_rng(8)
+    x = rng.uniform(0.2, 0.5, size=(3000, 3)).astype(np.float32)
+    r = np.float32(0.05)
+    b, g, _, prefix, _, _, _ = _gpu_construct(ctx, x, (0, 0, 0), (1, 1, 1), r, 0.5, "brick")
+    nx, ny, nz, R = g.nx, g.ny, g.nz, g.range
+    near = ctx.empty(nx * ny * nz, np.uint8)
+    ctx.check(ctx._lib.reyn_grid_occupancy(ctx.h, C.byref(g), near._ptr), "occupancy")
+    cz, cy, cx = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    key = (((cz >> 1) * (ny // 2) + (cy >> 1)) * (nx // 2) + (cx >> 1)) * 8 + (cz & 1) * 4 + (cy & 1) * 2 + (cx & 1)
+    occ = (np.diff(prefix.astype(np.int64))[key] > 0)
+    pad = np.pad(occ, R)
+    want = np.zeros_like(occ)
+    for dz in range(2 * R + 1):
+        for dy in range(2 * R + 1):
+            for dx in range(2 * R + 1):
+                want |= pad[dz:dz + nz, dy:dy + ny, dx:dx + nx]
+    got = near.numpy()[key].astype(bool)
+    assert np.array_equal(got, want) and want.any() and (~want).any()

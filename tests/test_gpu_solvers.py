@@ -103,6 +103,19 @@ def test_iisph_step_parity_cell_ratios(ctx, ratio):
     assert errs["a_ii"] <= TOL_DIFF and errs["s_i"] <= TOL_DIFF and errs["v"] <= TOL_DIFF, errs
 
 
+@pytest.mark.parametrize("ratio,mass_spread", [(1.0, 0.0), (0.5, 0.0), (1.0, 0.1)])
+def test_iisph_step_parity_brick_order(ctx, ratio, mass_spread):
+    # the brick cell order: at ratio 1 the lists hold the 4 brick segments,
+    # at ratio 0.5 every loop walks the brick-ordered grid
+    errs, gpu, ref = parity.run_iisph_case(ctx, cell_ratio=ratio, order=capi.CELL_ORDER_BRICK,
+                                           mass_spread=mass_spread)
+    print(f"brick order, ratio {ratio}, mass spread {mass_spread}: {errs}", flush=True)
+    assert gpu["iterations"] >= 3
+    assert errs["rho"] <= TOL and errs["p"] <= TOL and errs["x"] <= TOL, errs
+    assert errs["a_ii"] <= TOL_DIFF and errs["s_i"] <= TOL_DIFF and errs["v"] <= TOL_DIFF, errs
+    assert errs["dx"] <= TOL_DIFF, errs
+
+
 @pytest.mark.parametrize("ratio", [0.5, 1.0])
 def test_iisph_step_parity_mixed_masses(ctx, ratio):
     # particle masses spread by 10%: the pressure accelerations gather m_j per

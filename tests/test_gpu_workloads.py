@@ -91,3 +91,15 @@ def test_config5_served_jobs(ctx):
     b.step(2)
     check_step(parity.batch_scene_step_parity(b, 0), f"job {big} ({spec[big][0].n} particles), step 3 vs FP64")
     b.close()
+
+
+def test_brick_order_batch_scene_against_fp64(ctx):
+    """A batch in the brick cell order: one step of a scene from its batch
+    state against the FP64 oracle's step over the oracle's brick-ordered grid."""
+    scs = scenes.config4_batch(count=4, seed=9, n_lo=5000, n_hi=20000)
+    opts = dict(solver="iisph", kernel="B3", eta_rho_max=0.001, min_iter=3, max_iter=100, cell_ratio=1.0,
+                check_every=2, cell_order="brick")
+    b = Batch(ctx, scs, **opts)
+    b.step(3)
+    check_step(parity.batch_scene_step_parity(b, 2), f"brick-order batch scene 2 ({scs[2].n} particles) vs FP64")
+    b.close()

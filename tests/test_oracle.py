@@ -438,3 +438,67 @@ def test_walk_capacity_rule_library_and_oracle():
     orc.grid_build(geom, pts[:cap, 0], pts[:cap, 1], pts[:cap, 2])
     with pytest.raises(ValueError):
         orc.grid_build(geom, pts[:, 0], pts[:, 1], pts[:, 2])
+
+
+def numpy_brick_grid(x, lo, hi, r, ratio):
+    """The brick-ordered grid in numpy: the XYZ geometry with every cell count
+    rounded up to even, key 8 brick + 4 (z & 1) + 2 (y & 1) + (x & 1) with the
+    bricks in XYZ order, np.bincount counts and a stable argsort."""
+    f32 = np.float32
+    cell = f32(r) * f32(ratio)
+    pad = f32(2.0 * float(r))
+    blo = np.asarray(lo, f32) - pad
+    bhi = np.asarray(hi, f32) + pad
+    n = np.ceil((bhi - blo) / cell).astype(np.int64)
+    n += n & 1
+    with np.errstate(invalid="ignore"):
+        q = np.floor((x.astype(f32) - blo) / cell)
+    q = np.where(np.isnan(q), 0, q)
+    q = np.clip(q, 0, n - 1).astype(np.int64)
+    b = q >> 1
+    nb = n // 2
+    brick = (b[:, 2] * nb[1] + b[:, 1]) * nb[0] + b[:, 0]
+    keys = brick * 8 + (q[:, 2] & 1) * 4 + (q[:, 1] & 1) * 2 + (q[:, 0] & 1)
+    counts = np.bincount(keys, minlength=int(np.prod(n)))
+    prefix = np.concatenate([[0], np.cumsum(counts)])
+    return n, keys, counts, prefix, np.argsort(keys, kind="stable")
+
+
+@pytest.mark.parametrize("ratio", [0.5, 1.0, 2.0])
+def test_brick_grid_build_matches_numpy_restatement(ratio):
+    rng = np.random.default_rng(15)
+    x = rng.uniform(-0.05, 1.05, size=(20000, 3)).astype(np.float32)
+    x[:50] = x[:50].round(1)
+    lo, hi, r = (0, 0, 0), (1, 1, 1), np.float32(0.05)
+    n, keys, counts, prefix, order = numpy_brick_grid(x, lo, hi, r, ratio)
+    geom = orc.geometry(lo, hi, r, ratio, order=1)
+    assert tuple(geom.n) == tuple(n) and geom.order == 1
+    g = orc.grid_build(geom, x[:, 0], x[:, 1], x[:, 2])
+    np.testing.assert_array_equal(g.keys, keys)
+    np.testing.assert_array_equal(g.counts, counts)
+    np.testing.assert_array_equal(g.prefix, prefix)
+    np.testing.assert_array_equal(g.sorted, order)
+    # the keys are a permutation of the cells: every cell has exactly one key
+    cells = np.stack(np.meshgrid(*[np.arange(int(v)) for v in n], indexing="ij"), -1).reshape(-1, 3)
+    b = cells >> 1
+    nb = n // 2
+    all_keys = ((b[:, 2] * nb[1] + b[:, 1]) * nb[0] + b[:, 0]) * 8 + (cells[:, 2] & 1) * 4 + (cells[:, 1] & 1) * 2 \
+        + (cells[:, 0] & 1)
+    np.testing.assert_array_equal(np.sort(all_keys), np.arange(int(np.prod(n))))
+
+
+@pytest.mark.parametrize("ratio", [0.5, 1.0])
+def test_brick_order_neighbour_sets_equal_xyz_order(ratio):
+    """The walk over the brick-ordered grid finds, for every point, exactly the
+    neighbours the XYZ walk finds (the sets; the visiting order differs)."""
+    rng = np.random.default_rng(19)
+    x = rng.uniform(0, 1, size=(4000, 3)).astype(np.float32)
+    r = np.float32(0.08)
+    out = {}
+    for order in (0, 1):
+        g = orc.grid_build(orc.geometry((0, 0, 0), (1, 1, 1), r, ratio, order=order), x[:, 0], x[:, 1], x[:, 2])
+        off, lst = orc.neighbour_lists(g, x[:, 0], x[:, 1], x[:, 2])
+        out[order] = [set(lst[off[i]:off[i + 1]].tolist()) for i in range(x.shape[0])]
+    assert out[0] == out[1]
+    cnt, _, _ = orc.brute_stats(r * r, x[:, 0], x[:, 1], x[:, 2])
+    assert [len(s) - 1 for s in out[1]] == cnt.tolist()
