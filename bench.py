@@ -88,7 +88,7 @@ def step_bytes(n: int, nb: int, ncells: int, iters: float) -> float:
 
 
 def measured_traffic(kernel: str) -> float | None:
-    """HBM bytes per launch of `kernel` from the committed rocprofv3
+    """HBM bytes per working launch of `kernel` from the committed rocprofv3
     FETCH_SIZE/WRITE_SIZE summary (scaled by the copy-kernel calibration it
     records), or None when the summary does not cover the kernel."""
     path = Path(__file__).resolve().parent / "profiles" / "hbm_traffic.json"
@@ -101,8 +101,9 @@ def measured_traffic(kernel: str) -> float | None:
 
 
 def measured_valu(kernel: str) -> float | None:
-    """VALU instructions per launch of `kernel` from the committed rocprofv3
-    SQ_INSTS_VALU summary, or None when it does not cover the kernel."""
+    """VALU instructions per working launch of `kernel` from the committed
+    rocprofv3 SQ_INSTS_VALU summary, or None when it does not cover the
+    kernel."""
     path = Path(__file__).resolve().parent / "profiles" / "valu_insts.json"
     try:
         table = json.loads(path.read_text())["valu_insts_per_launch"]
@@ -113,11 +114,142 @@ def measured_valu(kernel: str) -> float | None:
 
 
 def parse_profile(text: str) -> dict:
+    """Kernel name -> (launches, total ms, working launches, working ms) from
+    reyn_profile_summary (working launches: those that took at least a fifth
+    of the kernel's longest launch; a solver-loop launch that finds its loop
+    converged returns at once and is not one)."""
     out = {}
     for line in text.strip().splitlines():
-        name, cnt, ms = line.split()
-        out[name] = (int(cnt), float(ms))
+        name, cnt, ms, wcnt, wms = line.split()
+        out[name] = (int(cnt), float(ms), int(wcnt), float(wms))
     return out
+
+
+def profile_summary(ctx) -> dict:
+    import ctypes as C
+    need = C.c_size_t()
+    ctx.check(ctx._lib.reyn_profile_summary(ctx.h, None, 0, C.byref(need)))
+    buf = C.create_string_buffer(need.value + 16)
+    ctx.check(ctx._lib.reyn_profile_summary(ctx.h, buf, len(buf), None))
+    return parse_profile(buf.value.decode())
+
+
+# ---- rocprofv3 summaries ------------------------------------------------------
+# `bench.py --rocprof-summary KIND DIR...` reads the CSV output of rocprofv3
+# runs of this script (`--output-format csv`) and prints one JSON object. The
+# window is the timed steps of the profiled run: from its (warmup)-th k_vmax
+# dispatch (0-based; the first kernel of every step) to its last k_bounds
+# dispatch (the last kernel of every step); run the profiled command with
+# --no-cpu-baseline so nothing follows the timed steps but a final reduction.
+# A launch is a working launch when it took (trace) or counted (counters) at
+# least a fifth of the kernel's largest figure in the window: a solver-loop
+# launch that finds its loop converged returns at once and is not one.
+
+def _kernel_base(name: str) -> str:
+    n = name.replace("(anonymous namespace)", "").split("(")[0].split("<")[0]
+    return n.split("::")[-1].replace("void ", "").strip()
+
+
+def _csv_rows(d: str, suffix: str) -> list[dict]:
+    import csv
+    files = sorted(Path(d).rglob(f"*{suffix}"))
+    if not files:
+        raise SystemExit(f"no *{suffix} under {d}")
+    rows = []
+    for f in files:
+        with open(f, newline="") as fh:
+            rows += list(csv.DictReader(fh))
+    return rows
+
+
+def _window(names: list[str], warmup: int) -> tuple[int, int]:
+    vm = [k for k, n in enumerate(names) if _kernel_base(n) == "k_vmax"]
+    bd = [k for k, n in enumerate(names) if _kernel_base(n) == "k_bounds"]
+    if len(vm) <= warmup or not bd:
+        raise SystemExit(f"{len(vm)} k_vmax dispatches for a warm-up of {warmup} steps")
+    return vm[warmup], bd[-1]
+
+
+def _working(vals: list[float]) -> list[float]:
+    top = max(vals)
+    return [v for v in vals if v >= 0.2 * top]
+
+
+def rocprof_summary(kind: str, dirs: list[str], warmup: int) -> dict:
+    from collections import defaultdict
+    if kind == "trace":
+        rows = sorted(_csv_rows(dirs[0], "kernel_trace.csv"), key=lambda r: int(r["Dispatch_Id"]))
+        a, b = _window([r["Kernel_Name"] for r in rows], warmup)
+        per = defaultdict(list)
+        for r in rows[a:b + 1]:
+            per[_kernel_base(r["Kernel_Name"])].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
+        span = (int(rows[b]["End_Timestamp"]) - int(rows[a]["Start_Timestamp"])) / 1e6
+        out = {"window_dispatches": b - a + 1, "window_ms": round(span, 3),
+               "kernels_ms": round(sum(sum(v) for v in per.values()) / 1e3, 3), "kernels": {}}
+        for k, v in sorted(per.items(), key=lambda kv: -sum(kv[1])):
+            w = _working(v)
+            out["kernels"][k] = {"launches": len(v), "total_ms": round(sum(v) / 1e3, 4),
+                                 "working_launches": len(w), "working_avg_us": round(sum(w) / len(w), 2)}
+        return out
+
+    def counters(d):
+        rows = _csv_rows(d, "counter_collection.csv")
+        vals, names, meta = defaultdict(lambda: defaultdict(float)), {}, {}
+        for r in rows:
+            i = int(r["Dispatch_Id"])
+            vals[i][r["Counter_Name"]] += float(r["Counter_Value"])
+            names[i] = r["Kernel_Name"]
+            meta[i] = {k: int(r[k]) for k in ("VGPR_Count", "SGPR_Count", "LDS_Block_Size") if r.get(k)}
+        order = sorted(names)
+        return order, vals, names, meta
+
+    def per_kernel(d, counter):
+        order, vals, names, meta = counters(d)
+        a, b = _window([names[i] for i in order], warmup)
+        per, m = defaultdict(list), {}
+        for i in order[a:b + 1]:
+            k = _kernel_base(names[i])
+            per[k].append(vals[i])
+            m[k] = meta[i]
+        out = {}
+        for k, v in per.items():
+            key = [x[counter] for x in v]
+            top = max(key)
+            w = [x for x in v if x[counter] >= 0.2 * top] if top > 0 else v
+            out[k] = {"launches": len(v), "working_launches": len(w), **m[k],
+                      **{c: sum(x[c] for x in w) / len(w) for c in v[0]}}
+        return out
+
+    def calib(d, counter):
+        order, vals, names, _ = counters(d)
+        got = [vals[i][counter] for i in order if _kernel_base(names[i]) == "k_gather"]
+        if not got:
+            raise SystemExit(f"no k_gather dispatch under {d}")
+        return sum(got) / len(got)
+
+    if kind == "valu":
+        per = per_kernel(dirs[0], "SQ_INSTS_VALU")
+        return {"peak_wave_insts_per_s": VALU_PEAK_INSTS,
+                "valu_insts_per_launch": {k: round(v["SQ_INSTS_VALU"]) for k, v in per.items()
+                                          if v["SQ_INSTS_VALU"] > 0},
+                "kernels": per}
+    if kind == "traffic":
+        n = 1 << 26  # the copy calibration: reads 8n bytes, writes 4n bytes
+        cf, cw = calib(dirs[2], "FETCH_SIZE"), calib(dirs[3], "WRITE_SIZE")
+        fs, ws = 8 * n / (cf * 1024), 4 * n / (cw * 1024)
+        fe, wr = per_kernel(dirs[0], "FETCH_SIZE"), per_kernel(dirs[1], "WRITE_SIZE")
+        out = {"fetch_scale": fs, "write_scale": ws,
+               "calibration_kib": {"FETCH_SIZE": cf, "WRITE_SIZE": cw, "read_bytes": 8 * n, "write_bytes": 4 * n},
+               "bytes_per_launch": {}, "kernels": {}}
+        for k in sorted(fe, key=lambda k: -fe[k]["FETCH_SIZE"]):
+            f = fe[k]["FETCH_SIZE"]
+            w = wr.get(k, {}).get("WRITE_SIZE", 0.0)
+            out["bytes_per_launch"][k] = round(1024 * (f * fs + w * ws))
+            out["kernels"][k] = {"working_launches": fe[k]["working_launches"], "fetch_kib_raw": round(f, 1),
+                                 "write_kib_raw": round(w, 1), "fetch_bytes": round(1024 * f * fs),
+                                 "write_bytes": round(1024 * w * ws)}
+        return out
+    raise SystemExit(f"unknown summary kind {kind}")
 
 
 def cpu_model() -> str:
@@ -471,7 +603,14 @@ def main() -> int:
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run an identity gather of 2^26 floats (reads 512 MiB, writes 256 MiB), the "
                          "plain copy kernel that scales rocprofv3 FETCH_SIZE/WRITE_SIZE to bytes")
+    ap.add_argument("--rocprof-summary", nargs="+", metavar=("KIND", "DIR"), default=None,
+                    help="summarise rocprofv3 CSV output of a run of this script over its timed window and print "
+                         "JSON: trace DIR | valu DIR | traffic FETCH_DIR WRITE_DIR CALIB_FETCH_DIR CALIB_WRITE_DIR "
+                         "(the calibration dirs from --copy-calibration runs); uses --warmup")
     args = ap.parse_args()
+    if args.rocprof_summary:
+        print(json.dumps(rocprof_summary(args.rocprof_summary[0], args.rocprof_summary[1:], args.warmup), indent=1))
+        return 0
 
     def log(msg):
         print(msg, file=sys.stderr, flush=True)
@@ -551,15 +690,28 @@ def main() -> int:
     # early[0] .. early[1] - 1 is timed by itself on the way), then the timed
     # window of `steps` steps
     early = tuple(args.early_window)
+    # the last `steps` warm-up steps are the phase window: every launch timed
+    # with HIP events, for the per-kernel table and to find the dominant
+    # kernel; the timed window after it records events around that kernel's
+    # launches only (events around every launch add about 6% to a step)
+    phase_first = max(args.warmup - args.steps, 0) if not args.no_profile else args.warmup
     hist = {}
     t_roll = time.perf_counter()
     early_s = None
+    phase_iters = []
     for k in range(args.warmup):
         if k == early[0]:
             ctx.synchronize()
             t_e = time.perf_counter()
+        if k == phase_first:
+            ctx.synchronize()
+            ctx.check(ctx._lib.reyn_profile_select(ctx.h, None))
+            ctx.check(ctx._lib.reyn_profile_enable(ctx.h, 1))
+            t_phase = time.perf_counter()
         s = sim.step()
         hist[s.iterations] = hist.get(s.iterations, 0) + 1
+        if k >= phase_first:
+            phase_iters.append(s.iterations)
         if k + 1 == early[1]:
             ctx.synchronize()
             early_s = time.perf_counter() - t_e
@@ -567,7 +719,16 @@ def main() -> int:
             log(f"warmup {k}: dt {s.dt:.3e} iters {s.iterations} mean err {s.mean_rel_error:.2e}")
 
     ctx.synchronize()
-    ctx._lib.reyn_profile_enable(ctx.h, 0 if args.no_profile else 1)
+    phase_ms = (time.perf_counter() - t_phase) * 1e3 if phase_iters else 0.0
+    phase_prof = {}
+    dominant = None
+    if phase_iters:
+        phase_prof = profile_summary(ctx)
+        ranked = sorted((v[1], name) for name, v in phase_prof.items()
+                        if algorithmic_bytes(name, sc.n, sc.n_bdy, 1, 8) is not None)
+        dominant = ranked[-1][1] if ranked else None
+    ctx.check(ctx._lib.reyn_profile_enable(ctx.h, 1 if dominant else 0))
+    ctx.check(ctx._lib.reyn_profile_select(ctx.h, dominant.encode() if dominant else None))
     iters = []
     errs = []
     dts = []
@@ -587,14 +748,10 @@ def main() -> int:
         f"(wall); iterations {iters}; simulated time {sim.time:.4f} s")
 
     prof = {}
-    if not args.no_profile:
-        import ctypes as C
-        need = C.c_size_t()
-        ctx._lib.reyn_profile_summary(ctx.h, None, 0, C.byref(need))
-        buf = C.create_string_buffer(need.value + 16)
-        ctx._lib.reyn_profile_summary(ctx.h, buf, len(buf), None)
-        prof = parse_profile(buf.value.decode())
-        ctx._lib.reyn_profile_enable(ctx.h, 0)
+    if dominant:
+        prof = profile_summary(ctx)
+        ctx.check(ctx._lib.reyn_profile_enable(ctx.h, 0))
+        ctx.check(ctx._lib.reyn_profile_select(ctx.h, None))
 
     ms_per_step = wall * 1e3 / args.steps
     value = sc.n * args.steps / wall
@@ -602,33 +759,46 @@ def main() -> int:
     roofline = None
     phases = []  # per kernel: HBM fraction (algorithmic bytes) and VALU fraction (counted instructions)
     step_valu_frac = None
-    if prof:
-        log(f"{'kernel':<18}{'launches':>9}{'total ms':>11}{'avg us':>10}{'GB/s':>9}{'HBM %':>7}{'VALU %':>8}")
-        best = None
+    if phase_prof:
+        # per-kernel table of the phase window (every launch timed): working
+        # launches only, so a loop launch that returned at once (its loop had
+        # converged) does not dilute a kernel's mean
+        log(f"phase window, steps {phase_first}..{args.warmup - 1} ({phase_ms:.1f} ms, iterations {phase_iters}):")
+        log(f"{'kernel':<18}{'launches':>9}{'working':>9}{'total ms':>11}{'work us':>10}{'GB/s':>9}{'HBM %':>7}"
+            f"{'VALU %':>8}")
         valu_total = 0.0
-        for name, (cnt, tot) in sorted(prof.items(), key=lambda kv: -kv[1][1]):
+        for name, (cnt, tot, wcnt, wtot) in sorted(phase_prof.items(), key=lambda kv: -kv[1][1]):
             by = algorithmic_bytes(name, sc.n, sc.n_bdy, ncells, nfields)
-            avg_us = tot * 1e3 / cnt
+            avg_us = wtot * 1e3 / wcnt
             gbs = by / (avg_us * 1e-6) / 1e9 if by else float("nan")
             vi = measured_valu(name)
             vf = vi / (avg_us * 1e-6) / VALU_PEAK_INSTS if vi else float("nan")
-            log(f"{name:<18}{cnt:>9}{tot:>11.3f}{avg_us:>10.1f}{gbs:>9.0f}{100 * gbs / HBM_PEAK_GBS:>7.1f}"
+            log(f"{name:<18}{cnt:>9}{wcnt:>9}{tot:>11.3f}{avg_us:>10.1f}{gbs:>9.0f}{100 * gbs / HBM_PEAK_GBS:>7.1f}"
                 f"{100 * vf:>8.1f}")
             if vi:
-                valu_total += vi * cnt
-                phases.append({"kernel": name, "launches": cnt, "avg_launch_us": round(avg_us, 2),
-                               "hbm_frac": round(gbs / HBM_PEAK_GBS, 5) if by else None,
-                               "valu_frac": round(vf, 4)})
-            if best is None and by:
-                best = (name, cnt, tot, by, avg_us, gbs)
-        name, cnt, tot, by, avg_us, gbs = best
-        vi = measured_valu(name)
+                valu_total += vi * wcnt
+            phases.append({"kernel": name, "launches": cnt, "working_launches": wcnt,
+                           "working_avg_us": round(avg_us, 2), "total_ms": round(tot, 3),
+                           "hbm_frac": round(gbs / HBM_PEAK_GBS, 5) if by else None,
+                           "valu_frac": round(vf, 4) if vi else None})
+        step_valu_frac = valu_total / (phase_ms * 1e-3) / VALU_PEAK_INSTS
+    if prof and dominant in prof:
+        # the dominant kernel over the timed window (events around its launches only)
+        cnt, tot, wcnt, wtot = prof[dominant]
+        by = algorithmic_bytes(dominant, sc.n, sc.n_bdy, ncells, nfields)
+        avg_us = wtot * 1e3 / wcnt
+        gbs = by / (avg_us * 1e-6) / 1e9
+        vi = measured_valu(dominant)
+        tr = measured_traffic(dominant)
         roofline = {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(gbs / HBM_PEAK_GBS, 5), "traffic": measured_traffic(name), "kernel": name,
+                    "frac": round(gbs / HBM_PEAK_GBS, 5), "traffic": tr, "kernel": dominant,
                     "bytes_per_launch": by, "avg_launch_us": round(avg_us, 2), "launches": cnt,
+                    "working_launches": wcnt, "noop_launches_ms": round(tot - wtot, 3),
                     "share_of_step_time": round(tot / gpu_ms, 4),
+                    "traffic_over_algorithmic": round(tr / by, 3) if tr else None,
                     "valu_frac": round(vi / (avg_us * 1e-6) / VALU_PEAK_INSTS, 4) if vi else None}
-        step_valu_frac = valu_total / (gpu_ms * 1e-3) / VALU_PEAK_INSTS
+        log(f"timed window: {dominant} {wcnt} working launches of {cnt} (sum of iterations {sum(iters)}), "
+            f"{avg_us:.1f} us each, {gbs:.0f} GB/s algorithmic ({100 * gbs / HBM_PEAK_GBS:.2f}% of HBM peak)")
     mean_iters = float(np.mean(iters))
     sim_time_end = sim.time
     sbytes = step_bytes(sc.n, sc.n_bdy, ncells, mean_iters)
@@ -688,6 +858,8 @@ def main() -> int:
             "density_mean_rel_dev": rho_err,
             "step_hbm_frac": round(step_frac, 5),
             "step_valu_frac": round(step_valu_frac, 4) if step_valu_frac else None,
+            "phases_window": ([phase_first, args.warmup - 1] if phase_iters else None),
+            "phases_window_jacobi_iterations": phase_iters,
             "phases": phases,
             "step_bytes": sbytes,
             "event_ms_timed": round(gpu_ms, 3),

@@ -164,10 +164,17 @@ int reyn_timer_stop(reyn_ctx* ctx, float* ms);
  * reference counterpart; the reference has no profiler hooks, see its
  * nanobench micro-benchmarks, src/datastructure/uniformgrid.cu:385-423).
  * `reyn_profile_enable` clears the records and switches recording on or off;
+ * `reyn_profile_select` limits recording to the launches of one kernel
+ * (`name`, e.g. "k_iisph_jacobi"; NULL or "" records every kernel), so the
+ * events of the other launches do not weigh on a timed run;
  * `reyn_profile_summary` waits for the stream and writes one line per kernel,
- * "name launches total_ms", into buf (at most len bytes, NUL-terminated);
- * *needed receives the full size. */
+ * "name launches total_ms working_launches working_total_ms", into buf (at
+ * most len bytes, NUL-terminated); *needed receives the full size. Working
+ * launches are those that took at least a fifth of the kernel's longest
+ * launch: a solver-loop launch that finds its loop already converged returns
+ * at once and is not one of them. */
 int reyn_profile_enable(reyn_ctx* ctx, int on);
+int reyn_profile_select(reyn_ctx* ctx, const char* name);
 int reyn_profile_summary(reyn_ctx* ctx, char* buf, size_t len, size_t* needed);
 
 /* ---- uniform grid (src/datastructure/uniformgrid.cu:65-181) ------------- */

@@ -194,6 +194,7 @@ _SIGS = {
     "reyn_timer_start": (i32, [vp]),
     "reyn_timer_stop": (i32, [vp, P(f32)]),
     "reyn_profile_enable": (i32, [vp, C.c_int]),
+    "reyn_profile_select": (i32, [vp, C.c_char_p]),
     "reyn_profile_summary": (i32, [vp, C.c_char_p, C.c_size_t, P(C.c_size_t)]),
     "reyn_grid_builder_create": (i32, [vp, Float3, Float3, f32, f32, P(vp)]),
     "reyn_grid_builder_create_ordered": (i32, [vp, Float3, Float3, f32, f32, i32, P(vp)]),

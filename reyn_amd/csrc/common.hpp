@@ -53,6 +53,13 @@ template <class T> __device__ __forceinline__ T gat(const T* __restrict__ base, 
 {
     return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + static_cast<uint32_t>(i * sizeof(T)));
 }
+// gat with the non-temporal hint: data streamed once per pass, so it does not
+// push the gathered records out of the caches.
+template <class T> __device__ __forceinline__ T gat_nt(const T* __restrict__ base, uint32_t i)
+{
+    return __builtin_nontemporal_load(
+        reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + static_cast<uint32_t>(i * sizeof(T))));
+}
 // The same through an explicit byte offset (for records of mixed layout).
 template <class T> __device__ __forceinline__ T gatb(const void* __restrict__ base, uint32_t byte_off)
 {
@@ -234,6 +241,7 @@ struct reyn_ctx {
     std::string last_error;
     // per-launch event timing, enabled by reyn_profile_enable
     bool prof = false;
+    std::string prof_only; // record only this kernel's launches (empty: all)
     std::vector<hipEvent_t> prof_pool;
     uint32_t prof_used = 0;
     std::vector<reyn_prof_rec> prof_recs;
@@ -281,12 +289,14 @@ struct ProfScope {
         : ctx(c)
         , name(n)
     {
-        if (ctx->prof)
+        if (ctx->prof && (ctx->prof_only.empty() || ctx->prof_only == name))
             a = prof_event(ctx);
+        else
+            name = nullptr;
     }
     ~ProfScope()
     {
-        if (ctx->prof) {
+        if (name && ctx->prof) {
             const uint32_t b = prof_event(ctx);
             ctx->prof_recs.push_back({ name, a, b });
         }

@@ -320,15 +320,22 @@ int reyn_profile_enable(reyn_ctx* ctx, int on)
     return REYN_OK;
 }
 
+int reyn_profile_select(reyn_ctx* ctx, const char* name)
+{
+    if (!ctx)
+        return REYN_ERR_INVALID_ARGUMENT;
+    ctx->prof_only = name ? name : "";
+    return REYN_OK;
+}
+
 int reyn_profile_summary(reyn_ctx* ctx, char* buf, size_t len, size_t* needed)
 {
     if (!ctx)
         return REYN_ERR_INVALID_ARGUMENT;
     REYN_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    // aggregate per kernel name, in first-seen order
+    // per kernel name, in first-seen order: every launch's duration
     std::vector<const char*> names;
-    std::vector<double> total;
-    std::vector<uint32_t> count;
+    std::vector<std::vector<float>> times;
     for (const reyn_prof_rec& r : ctx->prof_recs) {
         float ms = 0.f;
         if (hipEventElapsedTime(&ms, ctx->prof_pool[r.a], ctx->prof_pool[r.b]) != hipSuccess)
@@ -338,16 +345,26 @@ int reyn_profile_summary(reyn_ctx* ctx, char* buf, size_t len, size_t* needed)
             ++k;
         if (k == names.size()) {
             names.push_back(r.name);
-            total.push_back(0.0);
-            count.push_back(0);
+            times.emplace_back();
         }
-        total[k] += ms;
-        count[k] += 1;
+        times[k].push_back(ms);
     }
     std::string out;
     char line[256];
     for (size_t k = 0; k < names.size(); ++k) {
-        snprintf(line, sizeof(line), "%s %u %.6f\n", names[k], count[k], total[k]);
+        double total = 0.0, work = 0.0;
+        float mx = 0.f;
+        for (float ms : times[k]) {
+            total += ms;
+            mx = std::max(mx, ms);
+        }
+        uint32_t nwork = 0;
+        for (float ms : times[k])
+            if (ms >= 0.2f * mx) {
+                work += ms;
+                ++nwork;
+            }
+        snprintf(line, sizeof(line), "%s %zu %.6f %u %.6f\n", names[k], times[k].size(), total, nwork, work);
         out += line;
     }
     if (needed)

