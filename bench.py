@@ -509,7 +509,8 @@ def batch_leg(ctx, args, log) -> dict:
 def serve_leg(ctx, args, log) -> dict:
     """Continuous batching of seeded IISPH jobs against FIFO one at a time."""
     from reyn_amd import scenes
-    from reyn_amd.serve import Server, SizeClassServer, make_jobs, poisson_arrivals, run_fifo, same_results
+    from reyn_amd.serve import (Server, SizeClassServer, make_jobs, poisson_arrivals, run_fifo, run_fifo_plain,
+                                same_as_plain, same_results)
     from reyn_amd.sim import Batch
 
     spec = scenes.config5_jobs(args.serve_jobs)
@@ -544,11 +545,31 @@ def serve_leg(ctx, args, log) -> dict:
     fifo = run_fifo(ctx, alone, device_setup=True, **opts)
     same, diff = same_results(served, alone)
     same1, diff1 = same_results(single, alone)
+    # the plain one-at-a-time baseline: every job through the single-scene
+    # entry points driven by the reference's main loop, set-up included
+    plain = make_jobs(spec, arrivals)
+    fifo_plain = run_fifo_plain(ctx, plain, **opts)
+    same_p, diff_p = same_as_plain(served, plain)
+    work = sum(j.particle_steps for j in plain)
+    plain_cap = work / fifo_plain["busy_s"]
+    # saturated arrivals: every job waiting at the start, so the served rate
+    # is the server's capacity on this job mix rather than the offered load
+    sat = make_jobs(spec, np.zeros(len(spec)))
+    srv2 = SizeClassServer(ctx, thresholds=thresholds, device_setup=True, **opts)
+    rep_sat = srv2.run(sat)
+    srv2.close()
+    same_s, _ = same_results(sat, alone)
     log(f"serving: p50 {rep['p50_s']:.2f} s / p99 {rep['p99_s']:.2f} s in size classes {thresholds}, "
         f"p50 {rep1['p50_s']:.2f} s / p99 {rep1['p99_s']:.2f} s in one batch, p50 {fifo['p50_s']:.2f} s / "
         f"p99 {fifo['p99_s']:.2f} s FIFO; {rep['particle_steps_per_s'] / 1e6:.1f} vs "
         f"{fifo['particle_steps_per_s'] / 1e6:.1f} M particle-steps/s; {same}/{len(spec)} and "
         f"{same1}/{len(spec)} results identical")
+    log(f"serving: plain one at a time p50 {fifo_plain['p50_s']:.2f} s / p99 {fifo_plain['p99_s']:.2f} s, "
+        f"{fifo_plain['particle_steps_per_s'] / 1e6:.1f} M particle-steps/s (capacity {plain_cap / 1e6:.1f} M); "
+        f"served/plain x{rep['particle_steps_per_s'] / fifo_plain['particle_steps_per_s']:.2f}; saturated: "
+        f"{rep_sat['particle_steps_per_s'] / 1e6:.1f} M, x{rep_sat['particle_steps_per_s'] / plain_cap:.2f} the "
+        f"plain capacity; {same_p}/{len(spec)} served results equal the plain results, {same_s}/{len(spec)} "
+        "saturated")
     return {
         "workload": f"{len(spec)} IISPH dam-break jobs (seed 5): N log-uniform in [5k, 500k], box sides U[1,3], "
                     "sphere obstacle with p 0.5, steps U{200..2000}, B3, stop at mean error <= 0.1% or 100 "
@@ -563,6 +584,17 @@ def serve_leg(ctx, args, log) -> dict:
         "throughput_speedup": round(rep["particle_steps_per_s"] / fifo["particle_steps_per_s"], 3),
         "results_identical": f"{same}/{len(spec)}", "jobs_differing": diff,
         "single_batch_results_identical": f"{same1}/{len(spec)}",
+        "fifo_plain_one_at_a_time": fifo_plain,
+        "p50_speedup_vs_plain": round(fifo_plain["p50_s"] / rep["p50_s"], 3),
+        "p99_speedup_vs_plain": round(fifo_plain["p99_s"] / rep["p99_s"], 3),
+        "throughput_speedup_vs_plain": round(rep["particle_steps_per_s"] / fifo_plain["particle_steps_per_s"], 3),
+        "plain_results_identical": f"{same_p}/{len(spec)}", "jobs_differing_from_plain": diff_p,
+        "saturated": {"arrivals": "all jobs waiting at the start", "served": rep_sat,
+                      "results_identical": f"{same_s}/{len(spec)}",
+                      "plain_capacity_particle_steps_per_s": round(plain_cap, 1),
+                      "throughput_vs_plain_capacity": round(rep_sat["particle_steps_per_s"] / plain_cap, 3),
+                      "throughput_vs_batch_of_one_fifo": round(rep_sat["particle_steps_per_s"]
+                                                               / (work / fifo["busy_s"]), 3)},
     }
 
 

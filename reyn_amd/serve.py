@@ -372,6 +372,83 @@ def run_fifo(ctx: Context, jobs: list[Job], keep_results: bool = True, clock=tim
     return out
 
 
+def run_fifo_plain(ctx: Context, jobs: list[Job], keep_results: bool = True, clock=time.perf_counter,
+                   solver: str = "iisph", kernel: str = "B3", nu: float = 0.001, rho_0: float | None = None,
+                   cell_ratio: float = 0.5, dt0: float = 0.0005, cfl_lambda: float = 1.0,
+                   change_factor: float = 0.001, iter_target: int = 10, eta_rho_max: float = 0.001,
+                   omega: float = 0.5, min_iter: int = 3, max_iter: int = 100, stop_rule: int | None = None,
+                   warm_start: bool = True, check_every: int = 4, cell_order: str = "xyz", **_unused) -> dict:
+    """One job at a time in arrival order through the plain single-scene
+    path: the reference's main loop (src/main.cu:59-78: time-step
+    controller, grid with reorder, solver step, bounds) on the single-scene
+    entry points, with the batch's options. A job's set-up (upload of its
+    scene, boundary grid and masses) is part of its service time. Results are
+    the same bits as the job alone in a batch."""
+    from . import capi
+    from .sim import IISPH, PCISPH, BoundarySamples, Particles, Simulation
+
+    if stop_rule is None:
+        stop_rule = capi.IISPH_STOP_MEAN
+    jobs = sorted(jobs, key=lambda j: (j.arrival, j.jid))
+    t_free = 0.0
+    busy = 0.0
+    for j in jobs:
+        sc = j.scene
+        t = clock()
+        h = sc.spacing
+        rho = sc.rho_0 if rho_0 is None else rho_0
+        state = Particles(ctx, sc.x, None, sc.mass, rho)
+        bdy = BoundarySamples(ctx, sc.xb, h, rho, cell_ratio=cell_ratio)
+        if solver == "iisph":
+            sol = IISPH(ctx, kernel, sc.n, nu, h, rho, omega=omega, eta_rho_max=eta_rho_max, min_iter=min_iter,
+                        max_iter=max_iter, stop_rule=stop_rule, warm_start=warm_start, check_every=check_every)
+            extra = ("p",)
+        else:
+            sol = PCISPH(ctx, kernel, sc.n, nu, h, rho, eta_rho_max=eta_rho_max, min_iter=min_iter,
+                         max_iter=max_iter, check_every=check_every)
+            extra = ()
+        sim = Simulation(ctx, sol, state, bdy, h, sc.bound_min, sc.bound_max, cfl_lambda=cfl_lambda,
+                         cell_ratio=cell_ratio, extra_reorder=extra, dt_policy="controller", dt0=dt0,
+                         iter_target=iter_target, change_factor=change_factor, cell_order=cell_order)
+        st = None
+        for _ in range(j.steps):
+            st = sim.step()
+        if keep_results:
+            res = state.host()
+            res = {k: res[k] for k in ("xx", "xy", "xz", "vx", "vy", "vz")}
+            res["rho"] = sol.rho.numpy()
+            if solver == "iisph":
+                res["p"] = sol.p.numpy()
+            j.result = res
+        else:
+            ctx.synchronize()
+        j.info = {"steps": j.steps, "iterations": st.iterations if st else 0}
+        for o in (sol, bdy, state, sim.builder):
+            getattr(o, "close", lambda: None)()
+        service = clock() - t
+        busy += service
+        j.admitted = max(t_free, j.arrival)
+        j.finished = j.admitted + service
+        j.taken = j.steps
+        t_free = j.finished
+    out = summary(jobs, busy)
+    out["busy_s"] = float(busy)
+    return out
+
+
+def same_as_plain(a: list[Job], plain: list[Job]) -> tuple[int, list[int]]:
+    """Jobs (by id) whose final fields or last iteration count differ between
+    a batched run and the plain single-scene run."""
+    pp = {j.jid: j for j in plain}
+    diff = []
+    for j in a:
+        o = pp[j.jid]
+        same = all(np.array_equal(j.result[k], o.result[k]) for k in o.result)
+        if not same or j.info.get("iterations") != o.info.get("iterations"):
+            diff.append(j.jid)
+    return len(a) - len(diff), diff
+
+
 def make_jobs(scene_steps, arrivals) -> list[Job]:
     return [Job(k, sc, int(st), float(a)) for k, ((sc, st), a) in enumerate(zip(scene_steps, arrivals))]
 
@@ -387,5 +464,5 @@ def same_results(a: list[Job], b: list[Job]) -> tuple[int, list[int]]:
     return len(a) - len(diff), diff
 
 
-__all__ = ["Job", "Server", "SizeClassServer", "run_fifo", "make_jobs", "poisson_arrivals", "summary", "same_results",
-           "device_memory", "batch_params", "RESERVE_BYTES"]
+__all__ = ["Job", "Server", "SizeClassServer", "run_fifo", "run_fifo_plain", "same_as_plain", "make_jobs",
+           "poisson_arrivals", "summary", "same_results", "device_memory", "batch_params", "RESERVE_BYTES"]

@@ -42,15 +42,26 @@ def test_scene_alone_equals_scene_in_batch(ctx, solver, kernel, order):
     mix.close()
 
 
-def test_batch_of_one_equals_single_scene_path(ctx):
+@pytest.mark.parametrize("solver", ["pcisph", "iisph"])
+def test_batch_of_one_equals_single_scene_path(ctx, solver):
+    """PCISPH with the batch defaults; IISPH with the serving options (cell
+    ratio 1, mean stop rule, warm start with the pressure reordered, check
+    every 2), the equivalence the serving leg's plain baseline rests on."""
     sc = small_scenes(count=1, seed=11)[0]
     h = sc.spacing
+    ratio = 0.5 if solver == "pcisph" else 1.0
     state = Particles(ctx, sc.x, None, sc.mass, sc.rho_0)
-    bdy = BoundarySamples(ctx, sc.xb, h, sc.rho_0)
-    solver = PCISPH(ctx, "W6", sc.n, 0.001, h, sc.rho_0, eta_rho_max=0.001, min_iter=3, max_iter=100)
-    sim = Simulation(ctx, solver, state, bdy, h, sc.bound_min, sc.bound_max, cfl_lambda=1.0,
-                     dt_policy="controller", dt0=0.0005, iter_target=10, change_factor=0.001)
-    b = Batch(ctx, [sc], solver="pcisph", kernel="W6")
+    bdy = BoundarySamples(ctx, sc.xb, h, sc.rho_0, cell_ratio=ratio)
+    if solver == "pcisph":
+        sol = PCISPH(ctx, "W6", sc.n, 0.001, h, sc.rho_0, eta_rho_max=0.001, min_iter=3, max_iter=100)
+        extra, opts = (), dict(solver="pcisph", kernel="W6")
+    else:
+        sol = IISPH(ctx, "B3", sc.n, 0.001, h, sc.rho_0, omega=0.5, eta_rho_max=0.001, min_iter=3, max_iter=100,
+                    stop_rule=capi.IISPH_STOP_MEAN, warm_start=True, check_every=2)
+        extra, opts = ("p",), dict(solver="iisph", kernel="B3", cell_ratio=1.0, check_every=2)
+    sim = Simulation(ctx, sol, state, bdy, h, sc.bound_min, sc.bound_max, cfl_lambda=1.0, cell_ratio=ratio,
+                     extra_reorder=extra, dt_policy="controller", dt0=0.0005, iter_target=10, change_factor=0.001)
+    b = Batch(ctx, [sc], **opts)
     for k in range(8):
         st = sim.step()
         b.step(1)
@@ -58,8 +69,10 @@ def test_batch_of_one_equals_single_scene_path(ctx):
         assert (info.iterations, info.dt) == (st.iterations, np.float32(st.dt)), k
     got = b.download(0)
     ref = state.host()
-    ref["rho"] = solver.rho.numpy()
+    ref["rho"] = sol.rho.numpy()
     assert_same(got, ref, "batch of one vs single-scene path")
+    if solver == "iisph":
+        assert np.array_equal(got["p"], sol.p.numpy()), "pressure"
     b.close()
 
 
