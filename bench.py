@@ -630,6 +630,9 @@ def main() -> int:
                     help="grid cell size / search radius (default: autotuned over 0.5 and 1.0, with the order)")
     ap.add_argument("--cell-order", choices=("xyz", "brick"), default=None,
                     help="cell order of the fluid grid (default: autotuned with the ratio)")
+    ap.add_argument("--check-every", type=int, default=2,
+                    help="pressure-loop iterations between host checks after the first chunk (the first chunk is "
+                         "one less than the previous step's iteration count, at least min_iter)")
     ap.add_argument("--bdy-cell-ratio", type=float, default=1.0,
                     help="cell size / search radius of the boundary grid (1.0: 9 strips per boundary walk)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -694,7 +697,8 @@ def main() -> int:
         state = Particles(ctx, sc.x, None, sc.mass, sc.rho_0)
         bdy = BoundarySamples(ctx, sc.xb, sc.spacing, sc.rho_0, args.bdy_cell_ratio)
         solver = IISPH(ctx, "B3", sc.n, 0.001, sc.spacing, sc.rho_0, omega=0.5, eta_rho_max=0.001, min_iter=3,
-                       max_iter=100, stop_rule=capi.IISPH_STOP_MEAN, warm_start=True, check_every=4)
+                       max_iter=100, stop_rule=capi.IISPH_STOP_MEAN, warm_start=True,
+                       check_every=args.check_every)
         sim = Simulation(ctx, solver, state, bdy, sc.spacing, sc.bound_min, sc.bound_max, cfl_lambda=lam,
                          cell_ratio=ratio, extra_reorder=("p",), dt_policy=args.dt_policy, dt0=0.0005,
                          iter_target=10, change_factor=0.001, cell_order=order)

@@ -245,6 +245,9 @@ struct reyn_ctx {
     std::vector<hipEvent_t> prof_pool;
     uint32_t prof_used = 0;
     std::vector<reyn_prof_rec> prof_recs;
+    // iterations of the last single-scene PCISPH [0] and IISPH [1] solve on
+    // this context: the first chunk of the next solve's loop
+    uint32_t loop_hint[2] = { 0, 0 };
     // scratch owned by the context, grown on demand and freed with it
     void* scratch = nullptr;
     size_t scratch_bytes = 0;

@@ -941,10 +941,15 @@ int launch_integrate(reyn_ctx* ctx, const Common& c, const float* ax, const floa
 }
 
 // Enqueue loop iterations in chunks until every scene's control block
-// reports done; `nscenes` control blocks at `ctl`, copied into `out`.
+// reports done; `nscenes` control blocks at `ctl`, copied into `out`. The
+// first chunk is min_iter iterations, or for one scene with a `hint` (the
+// iteration count of the previous solve of the same kind on this context)
+// one fewer than the hint: a step needs about as many iterations as the one
+// before, so most solves end after one or two host checks. Launches after
+// convergence return at once, so the chunking never changes a result.
 template <class Iter>
 int run_loop(reyn_ctx* ctx, const Common& c, const Scratch& s, uint32_t nscenes, uint32_t min_iter,
-    uint32_t max_iter, uint32_t check_every, Iter&& iter, SolveCtl* out)
+    uint32_t max_iter, uint32_t check_every, Iter&& iter, SolveCtl* out, uint32_t* hint = nullptr)
 {
     SolveCtl* ctl = s.ctl;
     SolveCtl* host = nullptr;
@@ -960,8 +965,9 @@ int run_loop(reyn_ctx* ctx, const Common& c, const Scratch& s, uint32_t nscenes,
     const uint32_t hard_cap = max_iter ? max_iter : 100000u;
     uint32_t launched = 0;
     for (;;) {
-        // the first chunk is exactly the iterations that must run anyway
-        uint32_t m = (launched == 0) ? std::max(min_iter, 1u) : chunk;
+        // the first chunk: the iterations that must run anyway, or the hint
+        const uint32_t guess = (hint && nscenes == 1 && *hint > 1) ? *hint - 1 : 1u;
+        uint32_t m = (launched == 0) ? std::max(min_iter, guess) : chunk;
         m = std::min(m, hard_cap - launched);
         for (uint32_t k = 0; k < m; ++k)
             REYN_TRY(iter());
@@ -978,6 +984,8 @@ int run_loop(reyn_ctx* ctx, const Common& c, const Scratch& s, uint32_t nscenes,
     }
     for (uint32_t k = 0; k < nscenes; ++k)
         out[k] = host[k];
+    if (hint && nscenes == 1)
+        *hint = host[0].iter;
     return REYN_OK;
 }
 
@@ -1070,7 +1078,7 @@ int pcisph_run(reyn_ctx* ctx, int kernel, const Common& c, const reyn_pcisph_wor
     REYN_TRY(run_loop(
         ctx, c, s, nscenes, min_iter, max_iter, check_every,
         [&]() -> int { return pcisph_iteration<BT>(ctx, kernel, c, work, s, nscenes, min_iter, max_iter); },
-        res));
+        res, BT ? nullptr : &ctx->loop_hint[0]));
     return pcisph_end<BT>(ctx, c, work);
 }
 
@@ -1180,7 +1188,8 @@ int iisph_run(reyn_ctx* ctx, int kind, const Common& c, const IisphLoop& lp, con
     REYN_TRY(iisph_begin<BT>(ctx, kind, c, lp, work, s));
     REYN_TRY(run_loop(
         ctx, c, s, nscenes, lp.min_iter, lp.max_iter, lp.check_every,
-        [&]() -> int { return iisph_iteration<BT>(ctx, kind, c, lp, work, s, nscenes); }, res));
+        [&]() -> int { return iisph_iteration<BT>(ctx, kind, c, lp, work, s, nscenes); }, res,
+        BT ? nullptr : &ctx->loop_hint[1]));
     return iisph_end<BT>(ctx, kind, c, work, s);
 }
 

@@ -259,6 +259,10 @@ def test_config3_full_size_late_step_parity(ctx):
             assert errs["v"] <= TOL_DIFF and errs["a_ii"] <= TOL_DIFF and errs["s_i"] <= TOL_DIFF, (step, errs)
             # the jittered initial lattice alone is compressed by about 0.2% on average
             assert compression <= 5e-3, (step, compression)
+            # fluid volume: sum m/rho changes only where free-surface particles
+            # lack neighbours (about +1.1% at step 250, while the column
+            # splashes; +0.01% at step 499)
+            assert abs(vol / vol0 - 1.0) <= 0.02, (step, vol, vol0)
         else:
             s = sim.step()
             assert s.iterations == 100 or s.mean_rel_error <= 1e-3, (step, s)
