@@ -201,6 +201,7 @@ struct reyn_batch {
     bool graphs = false;
     bool direct_next = true; // the first step after a layout change runs directly
     hipGraphExec_t g_begin[2] = {}, g_chunk[2] = {}, g_end[2] = {};
+    std::vector<uint32_t> hint; // per scene: the loop iterations of its last step
 };
 
 namespace {
@@ -753,6 +754,11 @@ int step(reyn_batch* b, bool use_graph)
     } else {
         REYN_TRY(launch_begin(b, nullptr));
     }
+    // each scene's iteration count of its last step: a scene's loop takes
+    // about as many iterations as in the step before
+    b->hint.resize(L.S);
+    for (uint32_t s = 0; s < L.S; ++s)
+        b->hint[s] = b->scenes[s]->iters;
     bool done = false;
     REYN_TRY(fetch(b, true, &done));
     uint32_t launched = min_it;
@@ -760,11 +766,22 @@ int step(reyn_batch* b, bool use_graph)
         if (launched >= cap)
             return set_error(ctx, REYN_ERR_NOT_CONVERGED, "batch: a scene did not converge in %u iterations", launched);
         const bool all_active = std::find(L.scene_done.begin(), L.scene_done.end(), 1) == L.scene_done.end();
-        if (use_graph && all_active)
+        // the next chunk runs until the first still-iterating scene is likely
+        // one iteration short of its last step's count (at least check_every
+        // iterations), over the workgroups of the scenes still iterating, so
+        // a step ends after few checks; iterations of a scene whose loop is
+        // done return at once, so results do not change
+        uint32_t soon = 0xffffffffu;
+        for (uint32_t s = 0; s < L.S; ++s)
+            if (!L.scene_done[s] && b->hint[s] > launched + 1)
+                soon = std::min(soon, b->hint[s] - 1 - launched);
+        uint32_t next = (soon == 0xffffffffu) ? chunk : std::max(chunk, soon);
+        next = std::min(next, cap - launched);
+        if (use_graph && all_active && next == chunk)
             REYN_HIP(ctx, hipGraphLaunch(b->g_chunk[L.cur], ctx->stream));
         else
-            REYN_TRY(launch_chunk_active(b, chunk));
-        launched += chunk;
+            REYN_TRY(launch_chunk_active(b, next));
+        launched += next;
         REYN_TRY(fetch(b, false, &done));
     }
     if (use_graph)

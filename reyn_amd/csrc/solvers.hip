@@ -690,6 +690,27 @@ __global__ __launch_bounds__(kBlock) REYN_OCC(REYN_OCC_PACC) void k_iisph_pacc(S
         st3(ax, ay, az, i, a);
 }
 
+// Dynamic LDS per workgroup (unused by the kernel) that caps a launch at
+// `waves` waves per SIMD: the workgroups of one more wave per SIMD (4 SIMDs
+// per CU) would not fit the 160 KB of LDS of a CU. 0: no cap.
+constexpr uint32_t occupancy_lds(uint32_t waves)
+{
+    return waves ? (160u * 1024u) / (std::max<uint32_t>(1u, waves * 4u / (kBlock / kWave)) + 1u) + 256u : 0u;
+}
+
+// Waves per SIMD of the Jacobi pass: its two record gathers per neighbour
+// keep more lines in flight than L1 and L2 serve well, so it runs fastest at
+// 3 waves per SIMD (late 2M state: 231 / 204.5 / 206.7 / 210.8 us at 2 / 3 /
+// 4 / 5 waves), below what its register count alone would allow. The
+// one-record pressure-acceleration pass gains a little from a cap as well
+// (122.4 us at its register-bound 7 waves, 119.6 us at 6).
+#ifndef REYN_JACOBI_WAVES
+#define REYN_JACOBI_WAVES 3
+#endif
+#ifndef REYN_PACC_WAVES
+#define REYN_PACC_WAVES 6
+#endif
+
 // PK: the accelerations come from u of X, and the new pressure goes to w of
 // P4 as well; otherwise the SoA work arrays.
 template <int K, bool IND, bool PK, bool BT>
@@ -1102,7 +1123,7 @@ int iisph_pacc(reyn_ctx* ctx, int kind, const Common& c, const reyn_iisph_work* 
             } else {
                 {
                     ProfScope _prof(ctx, "k_iisph_pacc");
-                    k_iisph_pacc<K, I, PK, BT><<<c.nb, kBlock, 0, ctx->stream>>>(c.a, c.f, work->p, s.gbx, s.gby,
+                    k_iisph_pacc<K, I, PK, BT><<<c.nb, kBlock, occupancy_lds(REYN_PACC_WAVES), ctx->stream>>>(c.a, c.f, work->p, s.gbx, s.gby,
                         s.gbz, work->ax, work->ay, work->az, ctl, s.ctl, s.L, s.P4, s.X);
                 }
                 REYN_LAUNCHED(ctx, "k_iisph_pacc");
@@ -1156,7 +1177,7 @@ int iisph_iteration(reyn_ctx* ctx, int kind, const Common& c, const IisphLoop& l
             } else {
                 {
                     ProfScope _prof(ctx, "k_iisph_jacobi");
-                    k_iisph_jacobi<K, I, PK, BT><<<c.nb, kBlock, 0, ctx->stream>>>(c.a, c.f, lp.omega, work->ax,
+                    k_iisph_jacobi<K, I, PK, BT><<<c.nb, kBlock, occupancy_lds(REYN_JACOBI_WAVES), ctx->stream>>>(c.a, c.f, lp.omega, work->ax,
                         work->ay, work->az, work->a_ii, work->s_i, s.gbx, s.gby, s.gbz, work->p, s.part_d,
                         s.part_u, s.ctl, s.L, s.X, s.P4);
                 }

@@ -626,13 +626,17 @@ template <int NP> __device__ __forceinline__ void plane_bases(const GridDev& g, 
     }
 }
 
-// q of entry e: e + B_u for the plane u in e's top bits.
+// q of entry e: e + B_u for the plane u in e's top bits. B_u is formed as B_0
+// plus the steps B_v - B_{v-1} of every plane v <= u (unsigned, so the sum
+// is B_u exactly): written as a chain of selects the compiler turns the
+// choice into an indexed read of B, which it then keeps in LDS, adding an
+// LDS round trip between every list word and the record it names.
 template <int NP> __device__ __forceinline__ uint32_t decode_entry(const uint32_t (&B)[NP], uint32_t e)
 {
     uint32_t b = B[0];
 #pragma unroll
     for (int u = 1; u < NP; ++u)
-        b = (e >= (uint32_t(u) << kPlaneShift)) ? B[u] : b;
+        b += (e >= (uint32_t(u) << kPlaneShift)) ? B[u] - B[u - 1] : 0u;
     return e + b;
 }
 
