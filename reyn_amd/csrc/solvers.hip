@@ -692,11 +692,16 @@ __global__ __launch_bounds__(kBlock) REYN_OCC(REYN_OCC_PACC) void k_iisph_pacc(S
 
 // Dynamic LDS per workgroup (unused by the kernel) that caps a launch at
 // `waves` waves per SIMD: the workgroups of one more wave per SIMD (4 SIMDs
-// per CU) would not fit the 160 KB of LDS of a CU. 0: no cap.
+// per CU) would not fit the 160 KB of LDS of a CU. 0: no cap. At most the
+// 64 KB a launch may ask for (a cap below two workgroups per CU is not made).
 constexpr uint32_t occupancy_lds(uint32_t waves)
 {
-    return waves ? (160u * 1024u) / (std::max<uint32_t>(1u, waves * 4u / (kBlock / kWave)) + 1u) + 256u : 0u;
+    return waves ? std::min<uint32_t>(64u * 1024u,
+                       (160u * 1024u) / (std::max<uint32_t>(1u, waves * 4u / (kBlock / kWave)) + 1u) + 256u)
+                 : 0u;
 }
+static_assert((occupancy_lds(3) == 41216u && occupancy_lds(6) == 23661u) || kBlock != 256,
+    "caps of 3 and 6 waves per SIMD with 256-lane workgroups");
 
 // Waves per SIMD of the Jacobi pass: its two record gathers per neighbour
 // keep more lines in flight than L1 and L2 serve well, so it runs fastest at
