@@ -1,0 +1,52 @@
+"""Per-launch profiling through the C ABI (reyn_profile_enable / _select /
+_summary), which bench.py's timed window relies on: one kernel selected
+records only that kernel, its working launches are the solver's iterations,
+and the launches that found the loop converged return at once."""
+
+import ctypes as C
+
+import pytest
+
+from reyn_amd import capi, scenes
+from reyn_amd.sim import IISPH, BoundarySamples, Particles, Simulation
+
+pytestmark = pytest.mark.gpu
+
+
+def summary(ctx):
+    need = C.c_size_t()
+    ctx.check(ctx._lib.reyn_profile_summary(ctx.h, None, 0, C.byref(need)))
+    buf = C.create_string_buffer(need.value + 16)
+    ctx.check(ctx._lib.reyn_profile_summary(ctx.h, buf, len(buf), None))
+    out = {}
+    for line in buf.value.decode().strip().splitlines():
+        name, cnt, ms, wcnt, wms = line.split()
+        out[name] = (int(cnt), float(ms), int(wcnt), float(wms))
+    return out
+
+
+def test_profile_select_records_one_kernel_and_its_working_launches(ctx):
+    sc = scenes.dam_break((24, 24, 24), 0.02, (1.0, 0.8, 0.6), jitter=0.01, seed=3)
+    print(f"IISPH dam break of {sc.n} particles", flush=True)
+    st = Particles(ctx, sc.x, None, sc.mass, sc.rho_0)
+    bdy = BoundarySamples(ctx, sc.xb, sc.spacing, sc.rho_0)
+    solver = IISPH(ctx, "B3", sc.n, 0.001, sc.spacing, sc.rho_0, eta_rho_max=0.001, max_iter=100,
+                   stop_rule=capi.IISPH_STOP_MEAN, check_every=4)
+    sim = Simulation(ctx, solver, st, bdy, sc.spacing, sc.bound_min, sc.bound_max, cfl_lambda=0.4,
+                     extra_reorder=("p",))
+    for _ in range(3):
+        sim.step()
+    ctx.synchronize()
+    ctx.check(ctx._lib.reyn_profile_select(ctx.h, b"k_iisph_jacobi"))
+    ctx.check(ctx._lib.reyn_profile_enable(ctx.h, 1))
+    iters = sum(sim.step().iterations for _ in range(5))
+    one = summary(ctx)
+    ctx.check(ctx._lib.reyn_profile_select(ctx.h, None))
+    ctx.check(ctx._lib.reyn_profile_enable(ctx.h, 1))
+    sim.step()
+    every = summary(ctx)
+    ctx.check(ctx._lib.reyn_profile_enable(ctx.h, 0))
+    assert set(one) == {"k_iisph_jacobi"}, one
+    cnt, ms, wcnt, wms = one["k_iisph_jacobi"]
+    assert wcnt == iters and cnt >= wcnt and 0.0 < wms <= ms
+    assert {"k_density", "k_iisph_pacc", "k_iisph_jacobi", "k_iisph_aii_s"} <= set(every)
