@@ -172,7 +172,8 @@ int reyn_timer_stop(reyn_ctx* ctx, float* ms);
  * most len bytes, NUL-terminated); *needed receives the full size. Working
  * launches are those that took at least a fifth of the kernel's longest
  * launch: a solver-loop launch that finds its loop already converged returns
- * at once and is not one of them. */
+ * at once and is not one of them (on scenes large enough that a working
+ * launch takes more than five times as long; on small scenes some are). */
 int reyn_profile_enable(reyn_ctx* ctx, int on);
 int reyn_profile_select(reyn_ctx* ctx, const char* name);
 int reyn_profile_summary(reyn_ctx* ctx, char* buf, size_t len, size_t* needed);
