@@ -992,7 +992,10 @@ int run_loop(reyn_ctx* ctx, const Common& c, const Scratch& s, uint32_t nscenes,
     uint32_t launched = 0;
     for (;;) {
         // the first chunk: the iterations that must run anyway, or the hint
-        const uint32_t guess = (hint && nscenes == 1 && *hint > 1) ? *hint - 1 : 1u;
+        // (not after a solve that stopped at max_iter: its count says nothing
+        // of how soon this one converges)
+        const bool use_hint = hint && nscenes == 1 && *hint > 1 && !(max_iter && *hint >= max_iter);
+        const uint32_t guess = use_hint ? *hint - 1 : 1u;
         uint32_t m = (launched == 0) ? std::max(min_iter, guess) : chunk;
         m = std::min(m, hard_cap - launched);
         for (uint32_t k = 0; k < m; ++k)
