@@ -259,12 +259,7 @@ __global__ __launch_bounds__(kBlock) REYN_OCC(REYN_OCC_DENSITY) void k_density(S
             flushed = end;
             lim = min(min(flushed + kRing, direct), L.cap);
         };
-        auto record = [&](uint32_t j, f3, float, float, const Cand& cd) {
-            if (!live || j == i)
-                return;
-            bool ok;
-            const uint32_t e = encode_entry(cd, &ok);
-            fits = fits && ok;
+        auto record = [&](uint32_t e) {
             if (cnt < lim) {
                 ring[wv][cnt % kRing][lane] = e;
             } else if (cnt < L.cap) {
@@ -284,9 +279,18 @@ __global__ __launch_bounds__(kBlock) REYN_OCC(REYN_OCC_DENSITY) void k_density(S
                 flush(flushed + kFlush);
         };
         if constexpr (IND || kStage == 0) {
-            walk4<IND>(g, xi, P4, record, rows_out);
+            walk4<IND>(g, xi, P4,
+                [&](uint32_t j, f3, float, float, const Cand& cd) {
+                    if (!live || j == i)
+                        return;
+                    bool ok;
+                    const uint32_t e = encode_entry(cd, &ok);
+                    fits = fits && ok;
+                    record(e);
+                },
+                rows_out);
         } else {
-            walk4_staged<kStage, 1>(g, xi, P4, stage[wv], live, record, rows_out);
+            list_walk_staged<kStage>(g, xi, P4, stage[wv], live, i, fits, record, rows_out);
         }
         flush(wave_max_u32(live ? min(cnt, direct) : 0u));
         if (!live)

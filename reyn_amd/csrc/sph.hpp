@@ -504,31 +504,11 @@ __device__ __forceinline__ void strip_loop_lds(const GridDev& g, f3 xi, uint32_t
     }
 }
 
-// walk4 over a cell-ordered state with every strip staged (stage_strip; `st`
-// holds WIN * CAP records of this wavefront). Every lane of the wavefront
-// must take part; lanes with `active` false visit nothing but reach every
-// strip_end with the others. The candidates, their order and the calls of f
-// are exactly walk4's.
-template <uint32_t CAP, int WIN, class F, class E>
-__device__ __forceinline__ void walk4_staged(const GridDev& g, f3 xi, const float4* __restrict__ P,
-    float4* __restrict__ st, bool active, F&& f, E&& strip_end)
-{
-    strips(g, xi, [&](uint32_t s, uint32_t e, uint32_t plane, uint32_t base) {
-        if (!active)
-            e = s;
-        const StageWin w = stage_strip<CAP, WIN>(s, e, st, [&](uint32_t q) { return P[q]; });
-        auto body = [&](uint32_t q, f3 xij, float d2, float4 p) { f(q, xij, d2, p.w, Cand { q, plane, base }); };
-        if (w.staged) {
-            strip_loop_lds(g, xi, s, e, st + (s - w.base), body);
-            wave_lds_sync(); // every lane has read before the next strip is staged
-        } else {
-            strip_loop(g, xi, s, e, [&](uint32_t q) { return gat(P, q); }, body);
-        }
-        strip_end();
-    });
-}
-
-// The same over SoA positions, staged as {x, y, z, 0}; f(j, x_ij, |x_ij|^2, x_j).
+// walk over SoA positions of a cell-ordered state with every strip staged
+// (stage_strip; `st` holds WIN * CAP records of this wavefront), staged as
+// {x, y, z, 0}; f(j, x_ij, |x_ij|^2, x_j). Every lane of the wavefront must
+// take part; lanes with `active` false visit nothing. The candidates and their
+// order are walk's.
 template <uint32_t CAP, int WIN, class F>
 __device__ __forceinline__ void walk_staged(const GridDev& g, f3 xi, const float* __restrict__ xs,
     const float* __restrict__ ys, const float* __restrict__ zs, float4* __restrict__ st, bool active, F&& f)
@@ -638,6 +618,79 @@ template <int NP> __device__ __forceinline__ uint32_t decode_entry(const uint32_
     for (int u = 1; u < NP; ++u)
         b += (e >= (uint32_t(u) << kPlaneShift)) ? B[u] - B[u - 1] : 0u;
     return e + b;
+}
+
+// Candidates of a staged strip, p = &st[s - base]: calls hit(q + eoff) for
+// every candidate q within reach, except the lane's own point `self` when
+// SELF (the strip may hold it); whole batches of kWalkBatch from one LDS
+// pointer, then the tail one by one (the candidates and their order are
+// strip_loop_lds's).
+template <bool SELF, class H>
+__device__ __forceinline__ void strip_hits_lds(const GridDev& g, f3 xi, uint32_t s, uint32_t e,
+    const float4* __restrict__ p, uint32_t self, uint32_t eoff, H&& hit)
+{
+    constexpr uint32_t B = kWalkBatch;
+    uint32_t q = s;
+    for (; q + B <= e; q += B, p += B) {
+        float4 pj[B];
+#pragma unroll
+        for (uint32_t u = 0; u < B; ++u)
+            pj[u] = p[u];
+#pragma unroll
+        for (uint32_t u = 0; u < B; ++u) {
+            const f3 xij = xi - f3 { pj[u].x, pj[u].y, pj[u].z };
+            if (dot(xij, xij) <= g.r2 && (!SELF || q + u != self))
+                hit(q + u + eoff);
+        }
+    }
+    for (; q < e; ++q, ++p) {
+        const float4 pj = *p;
+        const f3 xij = xi - f3 { pj.x, pj.y, pj.z };
+        if (dot(xij, xij) <= g.r2 && (!SELF || q != self))
+            hit(q + eoff);
+    }
+}
+
+// The listing walk of the density pass over a cell-ordered state (either order),
+// every strip staged (stage_strip; `st` holds CAP records of this
+// wavefront): calls hit(e) with the list entry e (`encode_entry`) of every
+// candidate within reach other than the lane's own point `self`, in walk
+// order, and strip_end() after each strip, a point every lane of the
+// wavefront reaches together. Lanes with `active` false visit nothing.
+// `fits` turns false when a strip's last candidate lies kPlaneSpan or more
+// above its plane's base: a test once per strip instead of once per
+// neighbour, on a candidate at or above every neighbour of the strip (a
+// particle is then at worst left unlisted where only a candidate out of reach
+// would not fit). The self test runs only in the strip that holds a lane's
+// own point (the strip order is the same for every lane, so the choice is
+// uniform). The neighbours and their order are walk4's.
+template <uint32_t CAP, class H, class E>
+__device__ __forceinline__ void list_walk_staged(const GridDev& g, f3 xi, const float4* __restrict__ P,
+    float4* __restrict__ st, bool active, uint32_t self, bool& fits, H&& hit, E&& strip_end)
+{
+    strips(g, xi, [&](uint32_t s, uint32_t e, uint32_t plane, uint32_t base) {
+        if (!active)
+            e = s;
+        fits = fits && (e <= s || e - 1u - base < kPlaneSpan);
+        const uint32_t eoff = (plane << kPlaneShift) - base;
+        const bool mine = self - s < e - s;
+        const StageWin w = stage_strip<CAP, 1>(s, e, st, [&](uint32_t q) { return P[q]; });
+        if (w.staged) {
+            const float4* p = st + (s - w.base);
+            if (__ballot(mine) != 0) // uniform
+                strip_hits_lds<true>(g, xi, s, e, p, self, eoff, hit);
+            else
+                strip_hits_lds<false>(g, xi, s, e, p, self, eoff, hit);
+            wave_lds_sync(); // every lane has read before the next strip is staged
+        } else {
+            strip_loop(g, xi, s, e, [&](uint32_t q) { return gat(P, q); },
+                [&](uint32_t q, f3, float, float4) {
+                    if (q != self)
+                        hit(q + eoff);
+                });
+        }
+        strip_end();
+    });
 }
 
 // Reads the list of slot t (count c <= cap, particle at xi) in rounds of B
