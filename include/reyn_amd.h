@@ -233,6 +233,22 @@ int reyn_grid_construct_and_reorder(reyn_ctx* ctx, reyn_grid_builder* b,
  * no bound, uniformgrid.cuh:113-178). */
 int reyn_grid_cell_capacity(int32_t range, uint32_t* cap);
 
+/* Deferred build checks (on != 0): the construct calls above enqueue the
+ * build and return REYN_OK at once, without waiting for its status; the grid
+ * they return may be used by work enqueued after them. The errors a build
+ * would have returned (REYN_ERR_OUT_OF_BOUNDS, REYN_ERR_CAPACITY) come from
+ * reyn_grid_builder_check, which waits for the stream; a construct call
+ * whose predecessor was not checked checks it first and, if that build
+ * failed, returns its error without building (nor reordering).
+ * A caller that reads the device anyway (a time-step read-back) checks the
+ * build after that read and saves one wait per step. Switching deferral off
+ * checks a pending build. No reference counterpart (the reference's build
+ * reports nothing, uniformgrid.cu:98-143). */
+int reyn_grid_builder_defer_check(reyn_grid_builder* b, int32_t on);
+/* The status of the last deferred build: REYN_OK, or the error its construct
+ * call would have returned; REYN_OK when no build is pending. */
+int reyn_grid_builder_check(reyn_ctx* ctx, reyn_grid_builder* b);
+
 /* Device copies of the builder's last per-cell counts (ncells) and per-point
  * cell keys (n), for tests. */
 int reyn_grid_debug_tables(reyn_grid_builder* b, const uint32_t** counts,

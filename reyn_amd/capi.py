@@ -200,6 +200,8 @@ _SIGS = {
     "reyn_grid_builder_create_ordered": (i32, [vp, Float3, Float3, f32, f32, i32, P(vp)]),
     "reyn_grid_builder_destroy": (i32, [vp]),
     "reyn_grid_builder_geometry": (i32, [vp, P(Grid)]),
+    "reyn_grid_builder_defer_check": (i32, [vp, i32]),
+    "reyn_grid_builder_check": (i32, [vp, vp]),
     "reyn_grid_construct": (i32, [vp, vp, f32, vp, vp, vp, u32, P(Grid)]),
     "reyn_grid_construct_and_reorder": (i32, [vp, vp, f32, P(vp), P(vp), u32, u32, P(Grid)]),
     "reyn_grid_debug_tables": (i32, [vp, P(vp), P(vp)]),

@@ -652,6 +652,8 @@ int reyn_grid_builder_destroy(reyn_grid_builder* b)
     for (uint32_t* a : arrs)
         if (a)
             (void)hipFree(a);
+    if (b->status_pinned)
+        (void)hipHostFree(b->status_pinned);
     delete b;
     return REYN_OK;
 }
@@ -664,16 +666,71 @@ int reyn_grid_builder_geometry(const reyn_grid_builder* b, reyn_grid* out)
     return REYN_OK;
 }
 
+int reyn_grid_builder_defer_check(reyn_grid_builder* b, int32_t on)
+{
+    if (!b || !b->ctx)
+        return REYN_ERR_INVALID_ARGUMENT;
+    reyn_ctx* ctx = b->ctx;
+    if (on && !b->status_pinned) {
+        REYN_HIP(ctx, hipSetDevice(ctx->device));
+        REYN_HIP(ctx, hipHostMalloc(reinterpret_cast<void**>(&b->status_pinned), 4 * sizeof(uint32_t),
+                          hipHostMallocDefault));
+    }
+    b->defer = on != 0;
+    return on ? REYN_OK : reyn_grid_builder_check(ctx, b);
+}
+
+int reyn_grid_builder_check(reyn_ctx* ctx, reyn_grid_builder* b)
+{
+    if (!ctx || !b || b->ctx != ctx)
+        return set_error(ctx, REYN_ERR_INVALID_ARGUMENT, "check: bad context or builder");
+    if (!b->pending)
+        return REYN_OK;
+    b->pending = false;
+    return finish_build(b, b->status_pinned);
+}
+
+} // extern "C"
+
+namespace {
+
+// The status buffer of a construct call: the builder's own pinned words when
+// its checks are deferred (after reading what an unchecked earlier build
+// left there), else the context's staging memory.
+int status_target(reyn_ctx* ctx, reyn_grid_builder* b, uint32_t** st)
+{
+    if (b->defer) {
+        REYN_TRY(reyn_grid_builder_check(ctx, b));
+        *st = b->status_pinned;
+        return REYN_OK;
+    }
+    return ctx_pinned(ctx, 64, reinterpret_cast<void**>(st));
+}
+
+// Deferred: the build is in flight and its status unread; else wait and check.
+int end_build(reyn_grid_builder* b, const uint32_t* st)
+{
+    if (b->defer) {
+        b->pending = true;
+        return REYN_OK;
+    }
+    return finish_build(b, st);
+}
+
+} // namespace
+
+extern "C" {
+
 int reyn_grid_construct(reyn_ctx* ctx, reyn_grid_builder* b, float search_radius,
     const float* xx, const float* xy, const float* xz, uint32_t n, reyn_grid* out)
 {
     if (!ctx || !b || !out || b->ctx != ctx)
         return set_error(ctx, REYN_ERR_INVALID_ARGUMENT, "construct: bad context, builder or output");
     uint32_t* st = nullptr;
-    REYN_TRY(ctx_pinned(ctx, 64, reinterpret_cast<void**>(&st)));
+    REYN_TRY(status_target(ctx, b, &st));
     REYN_TRY(build(b, search_radius, xx, xy, xz, n, st));
     *out = b->geom;
-    return finish_build(b, st);
+    return end_build(b, st);
 }
 
 int reyn_grid_construct_and_reorder(reyn_ctx* ctx, reyn_grid_builder* b, float search_radius,
@@ -683,7 +740,7 @@ int reyn_grid_construct_and_reorder(reyn_ctx* ctx, reyn_grid_builder* b, float s
         return set_error(ctx, REYN_ERR_INVALID_ARGUMENT,
             "construct_and_reorder: bad arguments (x, y, z must be the first three fields)");
     uint32_t* st = nullptr;
-    REYN_TRY(ctx_pinned(ctx, 64, reinterpret_cast<void**>(&st)));
+    REYN_TRY(status_target(ctx, b, &st));
     REYN_TRY(build(b, search_radius, fields[0], fields[1], fields[2], n, st));
     REYN_TRY(launch_gather(ctx, b->sorted, n, fields, tmp, nfields));
     for (uint32_t f = 0; f < nfields; ++f) {
@@ -693,7 +750,7 @@ int reyn_grid_construct_and_reorder(reyn_ctx* ctx, reyn_grid_builder* b, float s
     }
     *out = b->geom;
     out->sorted = nullptr; // Resort::yes
-    return finish_build(b, st);
+    return end_build(b, st);
 }
 
 int reyn_grid_debug_tables(reyn_grid_builder* b, const uint32_t** counts, const uint32_t** keys)

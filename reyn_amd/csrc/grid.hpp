@@ -14,6 +14,11 @@ struct reyn_grid_builder {
     uint32_t* prefix = nullptr; // ncells + 1
     uint32_t* block_sums = nullptr; // nblocks_scan
     uint32_t* status = nullptr; // [0] out-of-bounds points, [1] max cell count, [2] oversized cells
+    // deferred checks: construct calls copy the status here and return before
+    // it arrives; reyn_grid_builder_check (or the next construct) reads it
+    uint32_t* status_pinned = nullptr;
+    bool defer = false;
+    bool pending = false; // a deferred build whose status is still unread
     uint32_t cell_cap = 0; // most points per cell a build accepts (walk_cell_cap of the range); 0: no limit
     uint32_t cap = 0; // per-point capacity of the arrays below
     uint32_t* keys = nullptr;

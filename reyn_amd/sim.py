@@ -75,7 +75,18 @@ class UniformGridBuilder:
         self.h = h
         self.search_radius = float(search_radius)
         self._tmp: dict[int, list[DeviceArray]] = {}
+        self.deferred = False
         ctx.adopt(self)
+
+    def defer_check(self, on: bool = True) -> None:
+        """Deferred build checks: construct calls return before the build's
+        status is read, and ``check`` raises what they would have raised."""
+        self.ctx.check(self.ctx._lib.reyn_grid_builder_defer_check(self.h, int(bool(on))), "defer_check")
+        self.deferred = bool(on)
+
+    def check(self) -> None:
+        """Raises the error of the last deferred build, if it had one."""
+        self.ctx.check(self.ctx._lib.reyn_grid_builder_check(self.ctx.h, self.h), "construct (deferred check)")
 
     def geometry(self) -> capi.Grid:
         g = capi.Grid()
@@ -83,6 +94,8 @@ class UniformGridBuilder:
         return g
 
     def construct(self, search_radius: float, xx: DeviceArray, xy: DeviceArray, xz: DeviceArray) -> capi.Grid:
+        if self.deferred:
+            self.check()
         g = capi.Grid()
         self.ctx.check(
             self.ctx._lib.reyn_grid_construct(
@@ -96,6 +109,8 @@ class UniformGridBuilder:
         """Reorders every ``(owner, name)`` array in ``fields`` (x, y, z first)
         and swaps the reordered arrays in, as the reference swaps storage with
         its tmp buffer (src/buffer.cuh:184-209, src/main.cu:68)."""
+        if self.deferred:
+            self.check()  # an earlier build's error, before anything is swapped
         arrays = [getattr(o, nm) for o, nm in fields]
         n = arrays[0].n
         tmps = self._tmp.setdefault(n, [])
@@ -310,7 +325,9 @@ class StepStats:
 class Simulation:
     """The reference's per-step loop (src/main.cu:60-78): time step, grid
     build + reorder (every ``reorder_every`` steps, ``Resort::no`` between),
-    solver step, bounds.
+    solver step, bounds. The grid is built first and the time step taken
+    after it (the same dt: the velocity maximum does not depend on the
+    order), so that one wait for the GPU serves both.
 
     Time step: ``dt_policy="cfl"`` takes ``cfl_time_step(cfl_lambda)`` every
     step; ``dt_policy="controller"`` is the reference's main loop
@@ -335,6 +352,9 @@ class Simulation:
         self.cell_ratio = float(cell_ratio)
         self.cell_order = cell_order_id(cell_order)
         self.builder = UniformGridBuilder(ctx, self.bound_min, self.bound_max, 2.0 * h, cell_ratio, cell_order)
+        # the build's status is read after the time step's read-back: one wait
+        # per step instead of two
+        self.builder.defer_check(True)
         self.reorder_every = max(1, int(reorder_every))
         self.extra_reorder = extra_reorder
         self.dt_policy = dt_policy
@@ -366,8 +386,13 @@ class Simulation:
         return self.dt
 
     def step(self) -> StepStats:
-        dt = self.next_dt()
+        # the grid is built before the time step is read back (the reference
+        # takes dt first, main.cu:59-68): the CFL maximum over the reordered
+        # velocities is the same number, and the read-back's wait covers the
+        # build, whose status is checked right after it
         grid = self.get_grid()
+        dt = self.next_dt()
+        self.builder.check()
         iters = self.solver.step(self.state, grid, self.bdy, dt) or 0
         hard_enforce_bounds(self.ctx, self.bound_min, self.bound_max, self.state)
         self.steps += 1

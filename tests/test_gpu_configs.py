@@ -114,6 +114,7 @@ def test_config1_sesph_rollout_per_step_parity(ctx):
     for step in range(1000):
         dt = sim.next_dt()
         g = sim.get_grid()
+        sim.builder.check()
         if step in checks:
             hs = state.host()
             og = orc.grid_build(geom, hs["xx"], hs["xy"], hs["xz"])

@@ -133,6 +133,56 @@ def test_out_of_bounds_points_are_clamped_and_reported(ctx):
     assert np.array_equal(np.sort(sorted_), np.arange(x.shape[0]))
 
 
+def test_deferred_check_reports_what_construct_would(ctx):
+    """Deferred build checks: construct returns at once and the check returns
+    the error the waiting call returns (clamped points), once; a construct
+    after an unchecked failed build returns that build's error and builds
+    nothing; switching deferral off checks a pending build; the tables are
+    those of a waiting build."""
+    bad = scenes.lattice((8, 8, 8), 0.1)
+    bad[3] = (np.nan, 0.2, 0.2)
+    bad[7] = (50.0, 0.2, 0.2)
+    good = scenes.lattice((8, 8, 8), 0.1) + 0.01
+    n = bad.shape[0]
+    b = UniformGridBuilder(ctx, (0, 0, 0), (0.8, 0.8, 0.8), 0.2, 0.5)
+    ref = UniformGridBuilder(ctx, (0, 0, 0), (0.8, 0.8, 0.8), 0.2, 0.5)
+    assert ctx._lib.reyn_grid_builder_check(ctx.h, b.h) == capi.OK  # nothing pending
+    ctx.check(ctx._lib.reyn_grid_builder_defer_check(b.h, 1))
+    xb, xg = _upload_xyz(ctx, bad), _upload_xyz(ctx, good)
+
+    def construct(xs):
+        g = capi.Grid()
+        rc = ctx._lib.reyn_grid_construct(ctx.h, b.h, C.c_float(0.2), xs[0]._ptr, xs[1]._ptr, xs[2]._ptr, n,
+                                          C.byref(g))
+        return rc, g
+
+    rc, g = construct(xb)
+    assert rc == capi.OK
+    assert ctx._lib.reyn_grid_builder_check(ctx.h, b.h) == capi.ERR_OUT_OF_BOUNDS
+    assert "2 points" in ctx.last_error()
+    assert ctx._lib.reyn_grid_builder_check(ctx.h, b.h) == capi.OK  # reported once
+    prefix, sorted_ = grid_tables(ctx, g)
+    g_ref = capi.Grid()
+    assert ctx._lib.reyn_grid_construct(ctx.h, ref.h, C.c_float(0.2), xb[0]._ptr, xb[1]._ptr, xb[2]._ptr, n,
+                                        C.byref(g_ref)) == capi.ERR_OUT_OF_BOUNDS
+    prefix_ref, sorted_ref = grid_tables(ctx, g_ref)
+    np.testing.assert_array_equal(prefix, prefix_ref)
+    np.testing.assert_array_equal(sorted_, sorted_ref)
+    # an unchecked failed build: the next construct reports it and builds nothing
+    assert construct(xb)[0] == capi.OK
+    assert construct(xg)[0] == capi.ERR_OUT_OF_BOUNDS
+    assert ctx._lib.reyn_grid_builder_check(ctx.h, b.h) == capi.OK
+    rc, g = construct(xg)
+    assert rc == capi.OK and ctx._lib.reyn_grid_builder_check(ctx.h, b.h) == capi.OK
+    ref_g = ref.construct(0.2, *xg)
+    for a, r in zip(grid_tables(ctx, g), grid_tables(ctx, ref_g)):
+        np.testing.assert_array_equal(a, r)
+    # switching deferral off checks what is pending, and construct waits again
+    assert construct(xb)[0] == capi.OK
+    assert ctx._lib.reyn_grid_builder_defer_check(b.h, 0) == capi.ERR_OUT_OF_BOUNDS
+    assert construct(xb)[0] == capi.ERR_OUT_OF_BOUNDS
+
+
 def test_builder_rejects_bad_geometry(ctx):
     h = C.c_void_p()
     rc = ctx._lib.reyn_grid_builder_create(ctx.h, capi.f3((1, 1, 1)), capi.f3((0, 0, 0)), C.c_float(0.1),

@@ -217,3 +217,37 @@ def test_iisph_dam_break_rollout(ctx):
     assert reduce(ctx, st.m, "sum") == m0
     rho_mean = reduce(ctx, solver.rho, "avg")
     assert abs(rho_mean / sc.rho_0 - 1.0) < 0.05
+
+
+def test_simulation_step_order_same_results(ctx):
+    """Simulation.step builds the grid before reading the time step back (one
+    wait for both): the same bits as the reference's order (dt, then the grid,
+    src/main.cu:59-68) over 12 steps of a small IISPH dam break under the time
+    step controller."""
+    sc = scenes.dam_break((16, 16, 16), 0.025, (1.0, 0.8, 0.6), jitter=0.01, seed=3)
+
+    def make():
+        st = Particles(ctx, sc.x, None, sc.mass, sc.rho_0)
+        bdy = BoundarySamples(ctx, sc.xb, sc.spacing, sc.rho_0, 1.0)
+        solver = IISPH(ctx, "B3", sc.n, 0.001, sc.spacing, sc.rho_0, eta_rho_max=0.001, max_iter=100,
+                       stop_rule=capi.IISPH_STOP_MEAN, check_every=2)
+        return Simulation(ctx, solver, st, bdy, sc.spacing, sc.bound_min, sc.bound_max, cfl_lambda=1.0,
+                          cell_ratio=1.0, extra_reorder=("p",), dt_policy="controller", iter_target=10,
+                          change_factor=0.001)
+
+    a, b = make(), make()
+    for _ in range(12):
+        sa = a.step()
+        # the reference's order on the same kind of simulation
+        dt = b.next_dt()
+        g = b.get_grid()
+        b.builder.check()
+        iters = b.solver.step(b.state, g, b.bdy, dt)
+        hard_enforce_bounds(ctx, b.bound_min, b.bound_max, b.state)
+        b.steps += 1
+        b.last_iters = int(iters)
+        assert (sa.dt, sa.iterations) == (dt, iters)
+    for f in ("xx", "xy", "xz", "vx", "vy", "vz"):
+        np.testing.assert_array_equal(getattr(a.state, f).numpy(), getattr(b.state, f).numpy())
+    np.testing.assert_array_equal(a.solver.p.numpy(), b.solver.p.numpy())
+
