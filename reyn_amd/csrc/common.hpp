@@ -101,6 +101,24 @@ struct ksum3 {
     __device__ __forceinline__ f3 value() const { return { x.value(), y.value(), z.value() }; }
 };
 
+// Neumaier's accumulator for sums whose running value is never smaller in
+// magnitude than the next term (non-negative terms, the largest first): ksum's
+// branch then always takes (s - t) + v, the exact rounding error of s + v
+// (Fast2Sum), and for the first term, added to zero, both forms give zero; so
+// this gives ksum's bits with half the instructions. Where a later term does
+// exceed the running sum the correction is no longer exact (the sum stays
+// compensated, to a looser bound).
+struct ksum_desc {
+    float s = 0.f, c = 0.f;
+    __device__ __forceinline__ void add(float v)
+    {
+        const float t = s + v;
+        c += (s - t) + v;
+        s = t;
+    }
+    __device__ __forceinline__ float value() const { return s + c; }
+};
+
 // ---- launch geometry --------------------------------------------------------
 
 // lanes per workgroup: 4 wave64 (builds for measurement may set REYN_BLOCK

@@ -226,7 +226,9 @@ __global__ __launch_bounds__(kBlock) REYN_OCC(REYN_OCC_DENSITY) void k_density(S
     const f3 xi = live ? ld3(f.x, f.y, f.z, i) : g.bmin;
     const uint32_t wv = threadIdx.x / kWave;
     __shared__ float4 stage[kBlock / kWave][kStage > 0 ? kStage : 1]; // strip staging, per wavefront
-    ksum acc;
+    ksum acc; // the walks' sum (the particle itself among the candidates)
+    ksum_desc accl; // the listed sum: the particle itself first, then every
+                    // neighbour, each term at most the first when masses are equal
     bool summed = false;
     if (listing) {
         // the walk only tests candidates and records neighbours; the sum runs
@@ -299,9 +301,9 @@ __global__ __launch_bounds__(kBlock) REYN_OCC(REYN_OCC_DENSITY) void k_density(S
         const uint32_t stored = fits ? cnt : L.cap + 1;
         L.cnt[t] = stored;
         if (stored <= L.cap) {
-            acc.add(gat(f.m, i) * kW<K>(kp, 0.f)); // the particle itself
+            accl.add(gat(f.m, i) * kW<K>(kp, 0.f)); // the particle itself
             nbrs4<IND, kNbrBatch<K>>(g, L, t, xi, P4, f.x, f.y, f.z, f.m,
-                [&](uint32_t, f3, float d2, float mj) { acc.add(mj * kW<K>(kp, d2)); });
+                [&](uint32_t, f3, float d2, float mj) { accl.add(mj * kW<K>(kp, d2)); });
             summed = true;
         }
     }
@@ -329,7 +331,7 @@ __global__ __launch_bounds__(kBlock) REYN_OCC(REYN_OCC_DENSITY) void k_density(S
             walk<false>(b.grid, xi, b.x, b.y, b.z, [&](uint32_t k, f3 xik, float d2) { visit(xik, d2, b.m[k]); });
     }
     const float rbi = accb.value();
-    const float r = acc.value() + rbi;
+    const float r = (summed ? accl.value() : acc.value()) + rbi;
     rho[i] = r;
     if (X)
         put_auxw(X, i, r);
