@@ -47,6 +47,9 @@ HBM_PEAK_GBS = 8000.0
 # VALU issue peak: one wave64 FP32 instruction per CU per clock (256 CUs at
 # 2.4 GHz; packed FP32 is not used), i.e. 39.3 T lane-operations/s
 VALU_PEAK_INSTS = 256 * 2.4e9
+# the texture-address units: one per CU, busy cycles counted per CU
+# (TA_TA_BUSY summed over the 256 CUs)
+TA_CYCLES_PER_S = 256 * 2.4e9
 METRIC = "particle-steps/sec + % HBM roofline (IISPH 2M); serving p50/p99 vs plain"
 
 
@@ -107,6 +110,18 @@ def measured_valu(kernel: str) -> float | None:
     path = Path(__file__).resolve().parent / "profiles" / "valu_insts.json"
     try:
         table = json.loads(path.read_text())["valu_insts_per_launch"]
+    except (OSError, ValueError, KeyError):
+        return None
+    v = table.get(kernel)
+    return float(v) if v is not None else None
+
+
+def measured_ta_busy(kernel: str) -> float | None:
+    """Texture-address busy cycles (summed over the CUs) per working launch
+    of `kernel` from the committed rocprofv3 TA_TA_BUSY summary, or None."""
+    path = Path(__file__).resolve().parent / "profiles" / "address_path.json"
+    try:
+        table = json.loads(path.read_text())["ta_busy_cycles_per_launch"]
     except (OSError, ValueError, KeyError):
         return None
     v = table.get(kernel)
@@ -232,6 +247,14 @@ def rocprof_summary(kind: str, dirs: list[str], warmup: int) -> dict:
         return {"peak_wave_insts_per_s": VALU_PEAK_INSTS,
                 "valu_insts_per_launch": {k: round(v["SQ_INSTS_VALU"]) for k, v in per.items()
                                           if v["SQ_INSTS_VALU"] > 0},
+                "kernels": per}
+    if kind == "address":
+        per = per_kernel(dirs[0], "TA_TA_BUSY_sum")
+        return {"cycles_per_s": TA_CYCLES_PER_S,
+                "ta_busy_cycles_per_launch": {k: round(v["TA_TA_BUSY_sum"]) for k, v in per.items()
+                                              if v.get("TA_TA_BUSY_sum", 0) > 0},
+                "ta_stalled_by_l1_cycles_per_launch": {k: round(v.get("TA_ADDR_STALLED_BY_TC_CYCLES_sum", 0))
+                                                       for k, v in per.items() if v.get("TA_TA_BUSY_sum", 0) > 0},
                 "kernels": per}
     if kind == "traffic":
         n = 1 << 26  # the copy calibration: reads 8n bytes, writes 4n bytes
@@ -640,7 +663,8 @@ def main() -> int:
                          "plain copy kernel that scales rocprofv3 FETCH_SIZE/WRITE_SIZE to bytes")
     ap.add_argument("--rocprof-summary", nargs="+", metavar=("KIND", "DIR"), default=None,
                     help="summarise rocprofv3 CSV output of a run of this script over its timed window and print "
-                         "JSON: trace DIR | valu DIR | traffic FETCH_DIR WRITE_DIR CALIB_FETCH_DIR CALIB_WRITE_DIR "
+                         "JSON: trace DIR | valu DIR | address DIR (TA_TA_BUSY_sum, TA_ADDR_STALLED_BY_TC_CYCLES_sum) | "
+                         "traffic FETCH_DIR WRITE_DIR CALIB_FETCH_DIR CALIB_WRITE_DIR "
                          "(the calibration dirs from --copy-calibration runs); uses --warmup")
     args = ap.parse_args()
     if args.rocprof_summary:
@@ -809,6 +833,7 @@ def main() -> int:
             gbs = by / (avg_us * 1e-6) / 1e9 if by else float("nan")
             vi = measured_valu(name)
             vf = vi / (avg_us * 1e-6) / VALU_PEAK_INSTS if vi else float("nan")
+            ta = measured_ta_busy(name)
             log(f"{name:<18}{cnt:>9}{wcnt:>9}{tot:>11.3f}{avg_us:>10.1f}{gbs:>9.0f}{100 * gbs / HBM_PEAK_GBS:>7.1f}"
                 f"{100 * vf:>8.1f}")
             if vi:
@@ -816,7 +841,8 @@ def main() -> int:
             phases.append({"kernel": name, "launches": cnt, "working_launches": wcnt,
                            "working_avg_us": round(avg_us, 2), "total_ms": round(tot, 3),
                            "hbm_frac": round(gbs / HBM_PEAK_GBS, 5) if by else None,
-                           "valu_frac": round(vf, 4) if vi else None})
+                           "valu_frac": round(vf, 4) if vi else None,
+                           "ta_busy_frac": round(ta / (avg_us * 1e-6) / TA_CYCLES_PER_S, 4) if ta else None})
         step_valu_frac = valu_total / (phase_ms * 1e-3) / VALU_PEAK_INSTS
     if prof and dominant in prof:
         # the dominant kernel over the timed window (events around its launches only)
@@ -826,13 +852,18 @@ def main() -> int:
         gbs = by / (avg_us * 1e-6) / 1e9
         vi = measured_valu(dominant)
         tr = measured_traffic(dominant)
+        ta = measured_ta_busy(dominant)
         roofline = {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(gbs / HBM_PEAK_GBS, 5), "traffic": tr, "kernel": dominant,
                     "bytes_per_launch": by, "avg_launch_us": round(avg_us, 2), "launches": cnt,
                     "working_launches": wcnt, "noop_launches_ms": round(tot - wtot, 3),
                     "share_of_step_time": round(tot / gpu_ms, 4),
                     "traffic_over_algorithmic": round(tr / by, 3) if tr else None,
-                    "valu_frac": round(vi / (avg_us * 1e-6) / VALU_PEAK_INSTS, 4) if vi else None}
+                    "valu_frac": round(vi / (avg_us * 1e-6) / VALU_PEAK_INSTS, 4) if vi else None,
+                    # the unit that binds the listed loops: the texture-address
+                    # path, busy this fraction of the launch (counted cycles
+                    # from the committed profile over the live launch time)
+                    "ta_busy_frac": round(ta / (avg_us * 1e-6) / TA_CYCLES_PER_S, 4) if ta else None}
         log(f"timed window: {dominant} {wcnt} working launches of {cnt} (sum of iterations {sum(iters)}), "
             f"{avg_us:.1f} us each, {gbs:.0f} GB/s algorithmic ({100 * gbs / HBM_PEAK_GBS:.2f}% of HBM peak)")
     mean_iters = float(np.mean(iters))

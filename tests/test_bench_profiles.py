@@ -90,6 +90,30 @@ def test_counter_summaries_average_working_launches(bench, tmp_path):
     assert v["valu_insts_per_launch"]["k_density"] == 500
 
 
+def test_address_summary(bench, tmp_path):
+    disp = steps(warm=2, timed=2, iters=3)
+    write_counters(tmp_path / "a", disp, "TA_TA_BUSY_sum", {"k_iisph_jacobi": 4000.0, "k_density": 900.0})
+    a = bench.rocprof_summary("address", [str(tmp_path / "a")], 2)
+    assert a["ta_busy_cycles_per_launch"]["k_iisph_jacobi"] == 4000
+    assert a["kernels"]["k_iisph_jacobi"]["working_launches"] == 6
+    assert a["cycles_per_s"] == pytest.approx(256 * 2.4e9)
+
+
+def test_committed_address_profile_is_a_fraction_of_the_launch(bench):
+    """The committed TA busy cycles of the dominant kernel, over its launch
+    time in the committed kernel trace, are a fraction of one TA per CU."""
+    import json
+
+    ta = bench.measured_ta_busy("k_iisph_jacobi")
+    assert ta is not None
+    trace = (ROOT / "profiles" / "iisph_dambreak_2M_kernel_trace.md").read_text()
+    row = next(line for line in trace.splitlines() if line.startswith("| k_iisph_jacobi |"))
+    avg_us = float(row.split("|")[4])
+    frac = ta / (avg_us * 1e-6) / bench.TA_CYCLES_PER_S
+    assert 0.3 < frac <= 1.0, frac
+    assert json.loads((ROOT / "profiles" / "address_path.json").read_text())["commands"]
+
+
 def test_window_needs_the_warmup_steps(bench, tmp_path):
     write_trace(tmp_path / "kt", steps(warm=1, timed=1, iters=2))
     with pytest.raises(SystemExit):
