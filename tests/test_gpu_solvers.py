@@ -143,6 +143,60 @@ def test_list_cap_overflow_parity(ctx):
     assert errs["a"] <= TOL_DIFF and errs["v"] <= TOL_DIFF, errs
 
 
+def test_list_offsets_past_13_bits_parity(ctx):
+    """A bar of 1200 x 6 x 4 particles along x at cell ratio 1.0 (no
+    boundary): a row of cells holds about 4800 particles, three rows of cells
+    across y. The middle row's particles (and part of a side row's) have a
+    strip ending more than 2^13 candidates past their z-plane's first one:
+    they keep no list (their entries' offsets need not fit) and walk the grid
+    in every loop; the others are listed. One IISPH step against the FP64
+    oracle."""
+    h = 0.05
+    sc = scenes.dam_break((1200, 6, 4), h, (60.5, 0.4, 0.3), jitter=0.05, seed=5)
+    sc.x = (sc.x * np.float32(0.97) + np.float32(0.5 * h)).astype(np.float32)
+    v = np.random.default_rng(6).normal(0.0, 0.2, size=sc.x.shape).astype(np.float32)
+    case = parity.GpuCase(ctx, sc, v, 1.0, with_bdy=False)
+    solver = IISPH(ctx, "B3", sc.n, parity.NU, h, sc.rho_0, eta_rho_max=0.005, max_iter=50,
+                   stop_rule=capi.IISPH_STOP_ANY)
+    case.grid(solver, ("p",))
+    # per particle, the furthest end of a non-empty strip past its plane's
+    # first candidate (range 1, XYZ order), from the cell-start table
+    geom = orc.geometry(sc.bound_min, sc.bound_max, 2.0 * h, 1.0)
+    nx, ny, nz = (int(c) for c in geom.n)
+    pre = case.prefix.astype(np.int64)
+    hs = case.host
+    c = [np.floor((hs[f] - geom.bmin[a]) / geom.cell).astype(np.int64) for a, f in enumerate(("xx", "xy", "xz"))]
+    x0, x1 = np.maximum(c[0] - 1, 0), np.minimum(c[0] + 1, nx - 1)
+    span = np.zeros(sc.n, np.int64)
+    for dz in (-1, 0, 1):
+        zc = np.clip(c[2] + dz, 0, nz - 1)
+        zin = (c[2] + dz >= 0) & (c[2] + dz < nz)
+        base = pre[zc * nx * ny + np.clip(c[1] - 1, 0, ny - 1) * nx + x0]
+        for dy in (-1, 0, 1):
+            y = c[1] + dy
+            yc = np.clip(y, 0, ny - 1)
+            s0 = pre[zc * nx * ny + yc * nx + x0]
+            e0 = pre[zc * nx * ny + yc * nx + x1 + 1]
+            ok = zin & (y >= 0) & (y < ny) & (e0 > s0)
+            span = np.maximum(span, np.where(ok, e0 - 1 - base, 0))
+    assert (span >= 8192 + 200).sum() > 2000 and (span < 8192 - 200).sum() > 2000, (span.min(), span.max())
+    fg, st, ob = case.oracle_inputs()
+    p_in = solver.p.numpy()
+    dt = 2e-3
+    iters = solver.step(case.state, case.g, None, dt)
+    out = case.state.host()
+    ip = orc.OrcIisph(0.5, 0.005, 3, 50, capi.IISPH_STOP_ANY, 1)
+    ref = orc.iisph_step(fg, st, None, parity.sph_oracle(capi.KERNELS["B3"], h, sc.rho_0), ip, p_in, dt,
+                         fixed_iters=iters)
+    gpu = {"rho": solver.rho.numpy(), "p": solver.p.numpy(), "a_ii": solver.a_ii.numpy(),
+           "s_i": solver.s_i.numpy(), "x": np.stack([out["xx"], out["xy"], out["xz"]], 1)}
+    ref["x"] = np.stack([st.x, st.y, st.z], 1)
+    errs = {k: parity.rel_err(gpu[k], ref[k]) for k in gpu}
+    print(f"bar of 1200 x 6 x 4: {iters} iterations, {errs}", flush=True)
+    assert errs["rho"] <= TOL and errs["p"] <= TOL and errs["x"] <= TOL, errs
+    assert errs["a_ii"] <= TOL_DIFF and errs["s_i"] <= TOL_DIFF, errs
+
+
 def test_iisph_stop_rules(ctx):
     # the reference rule stops once no particle exceeds eta rho_0 (after >= 3)
     errs, gpu, ref = parity.run_iisph_case(ctx, stop_rule=capi.IISPH_STOP_ANY, eta=0.005)
