@@ -976,9 +976,10 @@ int launch_integrate(reyn_ctx* ctx, const Common& c, const float* ax, const floa
 // reports done; `nscenes` control blocks at `ctl`, copied into `out`. The
 // first chunk is min_iter iterations, or for one scene with a `hint` (the
 // iteration count of the previous solve of the same kind on this context)
-// one fewer than the hint: a step needs about as many iterations as the one
-// before, so most solves end after one or two host checks. Launches after
-// convergence return at once, so the chunking never changes a result.
+// the hint itself: a step needs about as many iterations as the one before,
+// so most solves end at their first host check, at the price of an
+// iteration or two that return at once when the step needs fewer. Launches
+// after convergence return at once, so the chunking never changes a result.
 template <class Iter>
 int run_loop(reyn_ctx* ctx, const Common& c, const Scratch& s, uint32_t nscenes, uint32_t min_iter,
     uint32_t max_iter, uint32_t check_every, Iter&& iter, SolveCtl* out, uint32_t* hint = nullptr)
@@ -1001,7 +1002,7 @@ int run_loop(reyn_ctx* ctx, const Common& c, const Scratch& s, uint32_t nscenes,
         // (not after a solve that stopped at max_iter: its count says nothing
         // of how soon this one converges)
         const bool use_hint = hint && nscenes == 1 && *hint > 1 && !(max_iter && *hint >= max_iter);
-        const uint32_t guess = use_hint ? *hint - 1 : 1u;
+        const uint32_t guess = use_hint ? *hint : 1u;
         uint32_t m = (launched == 0) ? std::max(min_iter, guess) : chunk;
         m = std::min(m, hard_cap - launched);
         for (uint32_t k = 0; k < m; ++k)
