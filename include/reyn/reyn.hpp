@@ -361,6 +361,8 @@ public:
     UniformGrid<Resort::no> construct(float search_radius, const DeviceBuffer<float>& xx,
         const DeviceBuffer<float>& xy, const DeviceBuffer<float>& xz)
     {
+        if (deferred_)
+            check();
         UniformGrid<Resort::no> g {};
         ctx_->check(reyn_grid_construct(ctx_->get(), b_, search_radius, xx.ptr(), xy.ptr(), xz.ptr(),
                         static_cast<uint32_t>(xx.size()), &g.grid),
@@ -394,9 +396,22 @@ public:
         return g;
     }
 
+    // Deferred build checks (no reference counterpart): construct calls return
+    // before the build's status is read, and check() throws what they would
+    // have thrown; a host loop that waits for the device anyway (a time-step
+    // read-back) checks after that wait.
+    void defer_check(bool on = true)
+    {
+        ctx_->check(reyn_grid_builder_defer_check(b_, on ? 1 : 0), "defer_check");
+        deferred_ = on;
+    }
+    void check() { ctx_->check(reyn_grid_builder_check(ctx_->get(), b_), "construct (deferred check)"); }
+
 private:
     UniformGrid<Resort::yes> reorder_all(float r, DeviceBuffer<float>& tmp, std::vector<DeviceBuffer<float>*>& bufs)
     {
+        if (deferred_)
+            check(); // an earlier build's error, before any buffer is swapped
         const size_t n = bufs[0]->size();
         for (auto* b : bufs)
             if (b->size() != n)
@@ -427,6 +442,7 @@ private:
 
     Context* ctx_;
     reyn_grid_builder* b_ = nullptr;
+    bool deferred_ = false;
     std::vector<DeviceBuffer<float>> scratch_;
 };
 

@@ -245,7 +245,9 @@ __global__ __launch_bounds__(kBlock) REYN_OCC(REYN_OCC_DENSITY) void k_density(S
         __shared__ uint32_t ring[kBlock / kWave][kRing][kWave];
         const uint32_t lane = threadIdx.x % kWave;
         uint32_t cnt = 0, flushed = 0, direct = L.cap; // `flushed` is uniform
-        bool fits = true; // every entry's offset fits its 13 bits
+        // false once an entry's offset may not fit its 13 bits (tested per
+        // neighbour in walk4, per strip in list_walk_staged)
+        bool fits = true;
         // entries below `lim` = min(flushed + kRing, direct, cap) go to the
         // ring: one test on the common path
         uint32_t lim = min(kRing, L.cap);

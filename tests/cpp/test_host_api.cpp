@@ -156,6 +156,33 @@ int main()
         CHECK(xs == ref);
     });
 
+    run("UniformGridBuilder with deferred build checks", [] {
+        const float rho_0 = 1000.f;
+        Lattice s = dam_break(10, 0.05f, v3(1.f, 1.f, 1.f));
+        Particles a(s.x.size(), rho_0), c(s.x.size(), rho_0);
+        upload(a, s, rho_0);
+        upload(c, s, rho_0);
+        DeviceBuffer<float> ta(1), tc(1);
+        UniformGridBuilder ba(v3(0.f), v3(1.f), 0.1f), bc(v3(0.f), v3(1.f), 0.1f);
+        bc.defer_check();
+        ba.construct_and_reorder(0.1f, ta, a);
+        bc.construct_and_reorder(0.1f, tc, c);
+        bc.check();
+        CHECK(a.xx.download() == c.xx.download() && a.vy.download() == c.vy.download());
+        // a point outside the grid: the deferred construct returns, the check throws
+        s.x[5] = 50.f;
+        Particles bad(s.x.size(), rho_0);
+        upload(bad, s, rho_0);
+        bc.construct(0.1f, bad.xx, bad.xy, bad.xz);
+        bool threw = false;
+        try {
+            bc.check();
+        } catch (const std::runtime_error& e) {
+            threw = std::string(e.what()).find("outside the grid") != std::string::npos;
+        }
+        CHECK(threw);
+    });
+
     run("IISPH dam break through the host API", [] {
         const float rho_0 = 1000.f, h = 1.f / 24;
         Lattice s = dam_break(16, h, v3(1.5f, 1.f, 1.f));
