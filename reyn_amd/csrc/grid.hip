@@ -257,8 +257,18 @@ __global__ __launch_bounds__(kBlock) void k_gather(uint32_t n, const uint32_t* _
     if (i >= n)
         return;
     const uint32_t s = sorted[i];
-    for (uint32_t f = 0; f < nfields; ++f)
-        a.dst[f][i] = a.src[f][s];
+    // every field's load before the first store: the stores may alias the
+    // sources as far as the compiler knows, so a load after a store would
+    // wait for it and each field would take its own round trip
+    float v[16];
+#pragma unroll
+    for (uint32_t f = 0; f < 16; ++f)
+        if (f < nfields)
+            v[f] = a.src[f][s];
+#pragma unroll
+    for (uint32_t f = 0; f < 16; ++f)
+        if (f < nfields)
+            a.dst[f][i] = v[f];
 }
 
 // near[c] = 1 when a point lies in the (2R+1)^3 cells around cell c: the

@@ -190,12 +190,16 @@ __global__ __launch_bounds__(kBlock) void k_pack(SceneArg sa, FluidDev f, float4
     bool mixed = false;
     if (live) {
         const uint32_t i = t;
-        const float m = f.m[i];
+        // every load before the first store: the stores may alias the
+        // loaded arrays as far as the compiler knows, and a load after them
+        // would wait for them
+        const float m = f.m[i], m0 = f.m[sc.first];
         const float4 r = make_float4(f.x[i], f.y[i], f.z[i], m);
+        const float4 u = make_float4(f.vx[i], f.vy[i], f.vz[i], 0.f);
         P4[i] = r;
         X[2 * size_t(i)] = r;
-        X[2 * size_t(i) + 1] = make_float4(f.vx[i], f.vy[i], f.vz[i], 0.f);
-        mixed = !(m == f.m[sc.first]);
+        X[2 * size_t(i) + 1] = u;
+        mixed = !(m == m0);
     }
     const int any = __syncthreads_or(mixed ? 1 : 0);
     if (threadIdx.x == 0)
@@ -389,12 +393,16 @@ __global__ __launch_bounds__(kBlock) void k_integrate(SceneArg sa, FluidDev f, c
         return;
     const uint32_t i = t;
     const float dt = sc.dt;
-    const f3 v = ld3(f.vx, f.vy, f.vz, i) + dt * ld3(ax, ay, az, i);
+    // every load before the first store: the stores may alias the loaded
+    // arrays as far as the compiler knows, and a load after them would wait
+    const f3 v0 = ld3(f.vx, f.vy, f.vz, i), a = ld3(ax, ay, az, i);
+    const f3 x0 = positions ? ld3(f.x, f.y, f.z, i) : f3 { 0.f, 0.f, 0.f };
+    const f3 v = v0 + dt * a;
     st3(f.vx, f.vy, f.vz, i, v);
     if (X)
         put_aux3(X, i, v);
     if (positions)
-        st3(f.x, f.y, f.z, i, ld3(f.x, f.y, f.z, i) + dt * v);
+        st3(f.x, f.y, f.z, i, x0 + dt * v);
 }
 
 // ---- SESPH ---------------------------------------------------------------------
