@@ -633,9 +633,10 @@ __global__ __launch_bounds__(kBlock) REYN_OCC(REYN_OCC_LOOP) void k_iisph_aii_s(
     const f3 G = grad.value() + gb;
     const float a = -dt * (dot(G, G) + f.m[i] * sq.value()) / (rho0 * rho0);
     const float s = (rho0 - rho[i]) / dt - (div.value() + dot(vi, gb));
+    const float pw = warm ? p[i] : 0.f; // loaded before the stores below
     aii[i] = a;
     si[i] = s;
-    const float pn = warm ? p[i] * 0.5f : jacobi_update(omega, a, 0.f, s);
+    const float pn = warm ? pw * 0.5f : jacobi_update(omega, a, 0.f, s);
     p[i] = pn;
     if (PK)
         reinterpret_cast<float*>(P4 + i)[3] = pn;
