@@ -817,6 +817,10 @@ def main() -> int:
     value = sc.n * args.steps / wall
     nfields = 8
     roofline = None
+    # the committed rocprofv3 counters (traffic, VALU and texture-address
+    # cycles per launch) were taken on the 128^3 scene; other sizes report
+    # none of them rather than a figure of another workload
+    profiled = args.side == 128
     phases = []  # per kernel: HBM fraction (algorithmic bytes) and VALU fraction (counted instructions)
     step_valu_frac = None
     if phase_prof:
@@ -831,9 +835,9 @@ def main() -> int:
             by = algorithmic_bytes(name, sc.n, sc.n_bdy, ncells, nfields)
             avg_us = wtot * 1e3 / wcnt
             gbs = by / (avg_us * 1e-6) / 1e9 if by else float("nan")
-            vi = measured_valu(name)
+            vi = measured_valu(name) if profiled else None
             vf = vi / (avg_us * 1e-6) / VALU_PEAK_INSTS if vi else float("nan")
-            ta = measured_ta_busy(name)
+            ta = measured_ta_busy(name) if profiled else None
             log(f"{name:<18}{cnt:>9}{wcnt:>9}{tot:>11.3f}{avg_us:>10.1f}{gbs:>9.0f}{100 * gbs / HBM_PEAK_GBS:>7.1f}"
                 f"{100 * vf:>8.1f}")
             if vi:
@@ -850,9 +854,9 @@ def main() -> int:
         by = algorithmic_bytes(dominant, sc.n, sc.n_bdy, ncells, nfields)
         avg_us = wtot * 1e3 / wcnt
         gbs = by / (avg_us * 1e-6) / 1e9
-        vi = measured_valu(dominant)
-        tr = measured_traffic(dominant)
-        ta = measured_ta_busy(dominant)
+        vi = measured_valu(dominant) if profiled else None
+        tr = measured_traffic(dominant) if profiled else None
+        ta = measured_ta_busy(dominant) if profiled else None
         roofline = {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(gbs / HBM_PEAK_GBS, 5), "traffic": tr, "kernel": dominant,
                     "bytes_per_launch": by, "avg_launch_us": round(avg_us, 2), "launches": cnt,
