@@ -92,6 +92,24 @@ def test_iisph_step_parity(ctx, kernel, warm):
     assert errs["dx"] <= TOL_DIFF, errs
 
 
+@pytest.mark.parametrize("kernel", ["C2", "B3", "W6", "COS"])
+def test_every_solver_every_kernel_at_the_tuned_ratio(ctx, kernel):
+    """One step of each solver with each smoothing kernel at cell ratio 1.0,
+    the ratio the autotuner picks (listed loops over the staged walk's
+    lists), against the FP64 oracle."""
+    e, _, _ = parity.run_sesph_case(ctx, kernel=kernel, cell_ratio=1.0)
+    assert e["rho"] <= TOL and e["x"] <= TOL, ("SESPH", e)
+    assert e["a"] <= TOL_DIFF and e["v"] <= TOL_DIFF and e["dx"] <= TOL_DIFF, ("SESPH", e)
+    e, g, _ = parity.run_pcisph_case(ctx, kernel=kernel, cell_ratio=1.0)
+    assert g["iterations"] >= 3
+    assert e["rho"] <= TOL and e["x"] <= TOL, ("PCISPH", e)
+    assert e["a"] <= TOL_DIFF and e["v"] <= TOL_DIFF and e["dx"] <= TOL_DIFF, ("PCISPH", e)
+    e, g, _ = parity.run_iisph_case(ctx, kernel=kernel, cell_ratio=1.0)
+    assert g["iterations"] >= 3
+    assert e["rho"] <= TOL and e["p"] <= TOL and e["x"] <= TOL, ("IISPH", e)
+    assert e["a_ii"] <= TOL_DIFF and e["s_i"] <= TOL_DIFF and e["v"] <= TOL_DIFF and e["dx"] <= TOL_DIFF, ("IISPH", e)
+
+
 @pytest.mark.parametrize("ratio", [0.25, 1.0, 2.0])
 def test_iisph_step_parity_cell_ratios(ctx, ratio):
     # larger cells: longer strips, so the density pass stages many list
