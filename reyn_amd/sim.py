@@ -59,7 +59,9 @@ class Particles:
 
 class UniformGridBuilder:
     """Owns counts/prefix/sorted; ``construct`` and ``construct_and_reorder``
-    return a ``capi.Grid`` view (uniformgrid.cu:65-181)."""
+    return a ``capi.Grid`` view (uniformgrid.cu:65-181). A returned grid holds
+    a reference to its builder, whose tables it points into; it stays valid
+    until the next construct call on the same builder."""
 
     def __init__(self, ctx: Context, bound_min, bound_max, search_radius: float, cell_ratio: float = 0.5,
                  order: str | int = "xyz"):
@@ -103,6 +105,7 @@ class UniformGridBuilder:
             ),
             "construct",
         )
+        g._builder = self  # the grid points into this builder's tables: keep it alive with the grid
         return g
 
     def construct_and_reorder(self, search_radius: float, fields) -> capi.Grid:
@@ -128,6 +131,7 @@ class UniformGridBuilder:
                 setattr(o, nm, tmps[k])
                 tmps[k] = arrays[k]
         self.ctx.check(rc, "construct_and_reorder")
+        g._builder = self  # the grid points into this builder's tables: keep it alive with the grid
         return g
 
     def debug_tables(self, n: int) -> tuple[np.ndarray, np.ndarray]:

@@ -183,6 +183,21 @@ def test_deferred_check_reports_what_construct_would(ctx):
     assert construct(xb)[0] == capi.ERR_OUT_OF_BOUNDS
 
 
+def test_grid_keeps_its_builder_alive(ctx):
+    """A grid from a builder that nothing else references still points at live
+    tables: the returned grid holds its builder (checked on the host before
+    the grid is used on the device)."""
+    import gc
+
+    x = scenes.lattice((8, 8, 8), 0.1) + 0.01
+    xx, xy, xz = _upload_xyz(ctx, x)
+    g = UniformGridBuilder(ctx, (0, 0, 0), (0.8, 0.8, 0.8), 0.2, 0.5).construct(0.2, xx, xy, xz)
+    gc.collect()
+    assert g._builder.h, "the builder was destroyed while its grid lives"
+    prefix, sorted_ = grid_tables(ctx, g)
+    assert prefix[-1] == x.shape[0] and np.array_equal(np.sort(sorted_), np.arange(x.shape[0]))
+
+
 def test_builder_rejects_bad_geometry(ctx):
     h = C.c_void_p()
     rc = ctx._lib.reyn_grid_builder_create(ctx.h, capi.f3((1, 1, 1)), capi.f3((0, 0, 0)), C.c_float(0.1),
