@@ -16,8 +16,8 @@ import pytest
 from oracle import binding as orc
 from oracle import parity
 from reyn_amd import capi, scenes
-from reyn_amd.sim import (IISPH, BoundarySamples, Particles, Simulation, cfl_time_step, hard_enforce_bounds,
-                          reduce)
+from reyn_amd.sim import (IISPH, PCISPH, BoundarySamples, Particles, Simulation, cfl_time_step,
+                          hard_enforce_bounds, reduce)
 
 pytestmark = pytest.mark.gpu
 
@@ -110,16 +110,14 @@ def test_every_solver_every_kernel_at_the_tuned_ratio(ctx, kernel):
     assert e["a_ii"] <= TOL_DIFF and e["s_i"] <= TOL_DIFF and e["v"] <= TOL_DIFF and e["dx"] <= TOL_DIFF, ("IISPH", e)
 
 
-@pytest.mark.parametrize("kernel", ["B3", "W6"])
-def test_iisph_step_parity_unsorted_state(ctx, kernel):
-    """Resort::no: the grid indexes a shuffled state through `sorted`, so every
-    loop maps its threads through it and the list entries name sorted
-    positions; one IISPH step against the FP64 oracle on the same unsorted
-    state (particle order in, particle order out)."""
+def _unsorted_case(ctx, seed):
+    """A shuffled state and a Resort::no grid over it (the grid indexes the
+    state through `sorted`), with the oracle's grid, state and boundary on the
+    same unsorted state."""
     from reyn_amd.sim import UniformGridBuilder
 
-    sc, v = parity.make_case(side=14, seed=9)
-    perm = np.random.default_rng(4).permutation(sc.n)
+    sc, v = parity.make_case(side=14, seed=seed)
+    perm = np.random.default_rng(seed + 4).permutation(sc.n)
     x, v = sc.x[perm], v[perm]
     h, r, ratio = sc.spacing, 2.0 * sc.spacing, 1.0
     state = Particles(ctx, x, v, sc.mass, sc.rho_0)
@@ -134,24 +132,57 @@ def test_iisph_step_parity_unsorted_state(ctx, kernel):
     xb, mb = bdy.positions(), bdy.m.numpy()
     bgrid = orc.grid_build(orc.geometry(bdy.bound_min, bdy.bound_max, r, ratio), xb[:, 0], xb[:, 1], xb[:, 2])
     ob = orc.Bdy(bgrid.resorted(), xb[:, 0], xb[:, 1], xb[:, 2], mb)
-    solver = IISPH(ctx, kernel, sc.n, parity.NU, h, sc.rho_0, eta_rho_max=0.005, max_iter=50,
+    return sc, state, bdy, builder, g, og, st, ob
+
+
+def _state_out(state, st):
+    out = state.host()
+    return ({"x": np.stack([out["xx"], out["xy"], out["xz"]], 1), "v": np.stack([out["vx"], out["vy"], out["vz"]], 1)},
+            {"x": np.stack([st.x, st.y, st.z], 1), "v": np.stack([st.vx, st.vy, st.vz], 1)})
+
+
+@pytest.mark.parametrize("kernel", ["B3", "W6"])
+def test_iisph_step_parity_unsorted_state(ctx, kernel):
+    """Resort::no: every loop maps its threads through `sorted` and the list
+    entries name sorted positions; one IISPH step against the FP64 oracle on
+    the same unsorted state (particle order in, particle order out)."""
+    sc, state, bdy, builder, g, og, st, ob = _unsorted_case(ctx, 9)
+    solver = IISPH(ctx, kernel, sc.n, parity.NU, sc.spacing, sc.rho_0, eta_rho_max=0.005, max_iter=50,
                    stop_rule=capi.IISPH_STOP_ANY)
     p_in = solver.p.numpy()
     dt = 2e-3
     iters = solver.step(state, g, bdy, dt)
-    out = state.host()
     ip = orc.OrcIisph(0.5, 0.005, 3, 50, capi.IISPH_STOP_ANY, 1)
-    ref = orc.iisph_step(og, st, ob, parity.sph_oracle(capi.KERNELS[kernel], h, sc.rho_0), ip, p_in, dt,
+    ref = orc.iisph_step(og, st, ob, parity.sph_oracle(capi.KERNELS[kernel], sc.spacing, sc.rho_0), ip, p_in, dt,
                          fixed_iters=iters)
+    gx, rx = _state_out(state, st)
     gpu = {"rho": solver.rho.numpy(), "p": solver.p.numpy(), "a_ii": solver.a_ii.numpy(),
-           "s_i": solver.s_i.numpy(), "x": np.stack([out["xx"], out["xy"], out["xz"]], 1),
-           "v": np.stack([out["vx"], out["vy"], out["vz"]], 1)}
-    ref["x"] = np.stack([st.x, st.y, st.z], 1)
-    ref["v"] = np.stack([st.vx, st.vy, st.vz], 1)
+           "s_i": solver.s_i.numpy(), **gx}
+    ref.update(rx)
     errs = {k: parity.rel_err(gpu[k], ref[k]) for k in gpu}
     assert iters >= 3
     assert errs["rho"] <= TOL and errs["p"] <= TOL and errs["x"] <= TOL, errs
     assert errs["a_ii"] <= TOL_DIFF and errs["s_i"] <= TOL_DIFF and errs["v"] <= TOL_DIFF, errs
+
+
+@pytest.mark.parametrize("kernel", ["W6", "C2"])
+def test_pcisph_step_parity_unsorted_state(ctx, kernel):
+    """The same for one PCISPH step (predicted densities and pressure
+    accelerations through `sorted`)."""
+    sc, state, bdy, builder, g, og, st, ob = _unsorted_case(ctx, 15)
+    solver = PCISPH(ctx, kernel, sc.n, parity.NU, sc.spacing, sc.rho_0, max_iter=50)
+    dt = 1e-3
+    iters = solver.step(state, g, bdy, dt)
+    it, rho, a, avg = orc.pcisph_step(og, st, ob, parity.sph_oracle(capi.KERNELS[kernel], sc.spacing, sc.rho_0),
+                                      solver.prm.delta, dt, fixed_iters=iters)
+    gx, rx = _state_out(state, st)
+    gpu = {"rho": solver.rho.numpy(), "a": np.stack([solver.ax.numpy(), solver.ay.numpy(), solver.az.numpy()], 1),
+           **gx}
+    ref = {"rho": rho, "a": a, **rx}
+    errs = {k: parity.rel_err(gpu[k], ref[k]) for k in gpu}
+    assert iters >= 3
+    assert errs["rho"] <= TOL and errs["x"] <= TOL, errs
+    assert errs["a"] <= TOL_DIFF and errs["v"] <= TOL_DIFF, errs
 
 
 @pytest.mark.parametrize("ratio", [0.25, 1.0, 2.0])
