@@ -205,9 +205,16 @@ def run_sesph_case(ctx, kernel="C2", dt=1e-3, side=14, k=1000.0, seed=11, cell_r
 
 
 def run_pcisph_case(ctx, kernel="W6", dt=1e-3, side=14, seed=13, min_iter=3, max_iter=0, cell_ratio=0.5,
-                    compress=0.97):
+                    compress=0.97, mass_spread=0.0):
+    """One PCISPH step on the GPU against the FP64 oracle from the same
+    inputs; `mass_spread` > 0 gives every particle its own mass, as in
+    `run_iisph_case`."""
     sc, v = make_case(side=side, seed=seed, compress=compress)
-    case = GpuCase(ctx, sc, v, cell_ratio)
+    masses = None
+    if mass_spread:
+        u = np.random.default_rng(seed + 5).uniform(-mass_spread, mass_spread, sc.n)
+        masses = (sc.mass * (1.0 + u)).astype(np.float32)
+    case = GpuCase(ctx, sc, v, cell_ratio, masses=masses)
     solver = PCISPH(ctx, kernel, sc.n, NU, sc.spacing, sc.rho_0, min_iter=min_iter, max_iter=max_iter)
     case.grid()
     fg, st, ob = case.oracle_inputs()

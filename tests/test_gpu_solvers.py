@@ -290,6 +290,16 @@ def test_list_offsets_past_13_bits_parity(ctx):
     assert errs["a_ii"] <= TOL_DIFF and errs["s_i"] <= TOL_DIFF, errs
 
 
+@pytest.mark.parametrize("ratio", [0.5, 1.0])
+def test_pcisph_step_parity_mixed_masses(ctx, ratio):
+    """Masses spread by 10%: the pressure-acceleration loop loads each
+    neighbour's mass instead of taking the particle's own."""
+    errs, gpu, ref = parity.run_pcisph_case(ctx, cell_ratio=ratio, mass_spread=0.1)
+    assert gpu["iterations"] >= 3
+    assert errs["rho"] <= TOL and errs["x"] <= TOL, errs
+    assert errs["a"] <= TOL_DIFF and errs["v"] <= TOL_DIFF and errs["dx"] <= TOL_DIFF, errs
+
+
 def test_iisph_stop_rules(ctx):
     # the reference rule stops once no particle exceeds eta rho_0 (after >= 3)
     errs, gpu, ref = parity.run_iisph_case(ctx, stop_rule=capi.IISPH_STOP_ANY, eta=0.005)
